@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""PISO steps/s on one MI355X: mixed-precision vs FP64, both at 1e-8 relative
+residual, on the 96^3 unit-cube Kuhn channel (5,308,416 tets, 21,567,171 P2
+velocity DOFs, 912,673 pressure DOFs).
+
+One step = one NavierStokesSolver::pisoStep (convection, momentum right-hand
+side, batched 3-component momentum solve, divergence, pressure solve,
+velocity correction, boundary resets, NaN check) on device-resident data.
+Prints ONE JSON line.  The CPU oracle is timed beside it on a bounded sample
+(see cpu_baseline.sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "cell-fluid-structure-analysis_amd"))
+sys.path.insert(0, str(REPO))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "PISO steps/s, mixed vs FP64 GPU at 1e-8 rel. residual; % HBM roofline"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz, w, k=3):
+    """Algorithmic bytes of one CSR SpMV with k right-hand sides sharing the
+    matrix: values + int32 columns + int32 row pointers + read x, write y."""
+    return (w + 4) * nnz + 4 * (n + 1) + 2 * k * w * n
+
+
+def make_solver(P, mesh, prof, args, precision):
+    g = P.NavierStokesSolver()
+    g.initialize(mesh, args.rho, args.mu)
+    g.setDt(args.dt)
+    g.setInletProfile(prof)
+    g.setOptions(momentum_tol=args.tol, pressure_tol=args.tol, precision=precision,
+                 timing=2, check_every=args.check_every)
+    return g
+
+
+def run(P, g, steps, warmup, torch):
+    for _ in range(warmup):
+        if not g.pisoStep():
+            raise RuntimeError(f"warmup step failed: {P.last_error()}")
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if not g.pisoStep():
+            raise RuntimeError(f"step failed: {P.last_error()}")
+        stats.append(g.stepStats())
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, stats
+
+
+def cpu_baseline(O, g, fp64_stats, sz):
+    """Eigen-form BiCGSTAB of the oracle (one core) timed for a few iterations
+    on the workload's own momentum and pressure operators, then scaled by the
+    iteration counts of the timed FP64 GPU steps (the oracle takes the same
+    counts: see tests/test_gpu_parity.py)."""
+    rp, col, val = g.getMatrix("lhs")
+    rng = np.random.default_rng(0)
+    k_mom = 3
+    b = rng.uniform(-1, 1, sz["nu"])
+    t0 = time.perf_counter()
+    O.csr_krylov(rp, col, val, b, method=0, tol=0.0, maxit=k_mom)
+    t_mom = (time.perf_counter() - t0) / k_mom
+    del rp, col, val
+    rp, col, val = g.getMatrix("Lbc")
+    bp = rng.uniform(-1, 1, sz["nv"])
+    k_p = 20
+    t0 = time.perf_counter()
+    O.csr_krylov(rp, col, val, bp, method=0, tol=0.0, maxit=k_p)
+    t_p = (time.perf_counter() - t0) / k_p
+    mom = np.mean([sum(s["momentum_iters"]) for s in fp64_stats])
+    pre = np.mean([sum(s["pressure_iters"]) for s in fp64_stats])
+    t_step = mom * t_mom + pre * t_p
+    return {"value": 1.0 / t_step, "unit": "PISO steps/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle Eigen-form BiCGSTAB, {k_mom} iterations on the {sz['nu']}-row "
+                       f"momentum matrix ({t_mom:.3f} s/iter per component) and {k_p} on the "
+                       f"{sz['nv']}-row pressure matrix ({t_p * 1e3:.2f} ms/iter), scaled by the "
+                       f"FP64 run's mean {mom:.1f} momentum + {pre:.1f} pressure iterations per "
+                       f"step; element kernels (convection, divergence, right-hand sides) "
+                       f"not counted, which favours the CPU"),
+            "cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=96, help="cells per side of the unit cube")
+    ap.add_argument("--rho", type=float, default=1060.0)
+    ap.add_argument("--mu", type=float, default=0.004)
+    ap.add_argument("--re", type=float, default=100.0)
+    ap.add_argument("--dt", type=float, default=1e-3)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--check-every", type=int, default=8)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the PISO "
+              f"path is built and measured on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cfsa_ns as P
+
+    t0 = time.perf_counter()
+    n = args.n
+    mesh = P.Mesh.structuredRectangularPrism(1.0, 1.0, 1.0, n, n, n)
+    mesh.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+    U = args.re * args.mu / (args.rho * 1.0)
+    X = np.vstack([mesh.vertices(), mesh.edgeNodes()])
+    prof = np.zeros((len(X), 3))
+    prof[:, 0] = U * 16.0 * X[:, 1] * (1 - X[:, 1]) * X[:, 2] * (1 - X[:, 2])
+    log(f"mesh {n}^3: {mesh.sizes()} in {time.perf_counter() - t0:.1f}s")
+
+    t0 = time.perf_counter()
+    g64 = make_solver(P, mesh, prof, args, 0)
+    sz = g64.sizes()
+    log(f"fp64 solver ready in {time.perf_counter() - t0:.1f}s: {sz}")
+    el64, st64 = run(P, g64, args.steps, args.warmup, torch)
+    log(f"fp64: {args.steps} steps in {el64:.3f}s; last {st64[-1]}")
+    res = {"fp64": {"steps_per_s": args.steps / el64, "ms_per_step": 1e3 * el64 / args.steps,
+                    "momentum_iters": st64[-1]["momentum_iters"],
+                    "pressure_iters": st64[-1]["pressure_iters"]}}
+    spmv64_ms = sum(s["spmv_ms"] for s in st64) / max(1, sum(s["spmv_launches"] for s in st64))
+    b64 = spmv_bytes(sz["nu"], sz["nnzA"], 8)
+
+    primary = "fp64"
+    if not args.fp64_only:
+        t0 = time.perf_counter()
+        g32 = make_solver(P, mesh, prof, args, 1)
+        log(f"mixed solver ready in {time.perf_counter() - t0:.1f}s")
+        el32, st32 = run(P, g32, args.steps, args.warmup, torch)
+        log(f"mixed: {args.steps} steps in {el32:.3f}s; last {st32[-1]}")
+        spmv32_ms = sum(s["spmv_ms"] for s in st32) / max(1, sum(s["spmv_launches"] for s in st32))
+        b32 = spmv_bytes(sz["nu"], sz["nnzA"], 4)
+        u64, u32 = g64.getVelocity(), g32.getVelocity()
+        p64, p32 = g64.getPressure(), g32.getPressure()
+        d64 = g64.computeDivergence(u64)
+        d32 = g32.computeDivergence(u32)
+        res["mixed"] = {"steps_per_s": args.steps / el32, "ms_per_step": 1e3 * el32 / args.steps,
+                        "momentum_inner_iters": st32[-1]["momentum_iters"],
+                        "momentum_refinements": st32[-1]["momentum_refinements"],
+                        "pressure_inner_iters": st32[-1]["pressure_iters"],
+                        "pressure_refinements": st32[-1]["pressure_refinements"]}
+        res["error_vs_fp64"] = {
+            "velocity_rel_l2": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+            "pressure_rel_l2": float(np.linalg.norm(p32 - p64) / np.linalg.norm(p64)),
+            "divergence_l2_fp64": float(np.linalg.norm(d64)),
+            "divergence_l2_mixed": float(np.linalg.norm(d32))}
+        res["speedup_mixed_vs_fp64"] = el64 / el32
+        res["spmv_fp32"] = {"avg_ms": spmv32_ms, "GB/s": b32 / spmv32_ms / 1e6}
+        primary = "mixed"
+    res["spmv_fp64"] = {"avg_ms": spmv64_ms, "GB/s": b64 / spmv64_ms / 1e6}
+
+    if primary == "mixed":
+        ach = b32 / spmv32_ms / 1e6
+        roof = {"bound": "hbm", "kernel": "k_csr_stream<float,3> (momentum SpMV, 3 rhs)",
+                "bytes_per_launch": b32, "avg_launch_ms": spmv32_ms}
+    else:
+        ach = b64 / spmv64_ms / 1e6
+        roof = {"bound": "hbm", "kernel": "k_csr_stream<double,3> (momentum SpMV, 3 rhs)",
+                "bytes_per_launch": b64, "avg_launch_ms": spmv64_ms}
+    roof.update({"achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": ach / HBM_PEAK_GBS, "traffic": None})
+
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(O, g64, st64, sz)
+        log(f"cpu baseline in {time.perf_counter() - t0:.1f}s")
+
+    value = res[primary]["steps_per_s"]
+    out = {"metric": METRIC, "value": value, "unit": "PISO steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res[primary]["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 inner / f64 refinement" if primary == "mixed" else "f64",
+           "data": "synthetic",
+           "config": {"workload": f"kuhn{n}_channel", "mesh": f"unit cube {n}^3 Kuhn 6-tet",
+                      "tets": sz["nt"], "velocity_dofs": 3 * sz["nu"],
+                      "pressure_dofs": sz["nv"], "nnz_momentum": sz["nnzA"],
+                      "nnz_pressure": sz["nnzP"], "density": args.rho, "viscosity": args.mu,
+                      "inlet_peak": U, "reynolds": args.re, "dt": args.dt, "tol": args.tol,
+                      "precision": primary, "parallelism": "single GPU"},
+           "dofs_per_s": value * (3 * sz["nu"] + sz["nv"]),
+           **res, "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

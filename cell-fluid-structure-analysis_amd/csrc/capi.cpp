@@ -1,0 +1,668 @@
+// extern "C" boundary of libcfsa_ns.so; declarations and reference citations
+// are in include/cfsa_ns.h.  No exception crosses this layer.
+#include "../../include/cfsa_ns.h"
+
+#include <cstring>
+#include <iostream>
+#include <memory>
+#include <string>
+
+#include "mesh.hpp"
+#include "solver.hpp"
+#include "vtk.hpp"
+
+struct cfsa_mesh {
+  cfsa::Mesh m;
+  uint64_t version = 0;  // bumped whenever labels or vertices change
+};
+struct cfsa_ns {
+  cfsa::NSSolver s;
+  const cfsa_mesh *mesh = nullptr;  // set when initialized from a handle
+  uint64_t synced = 0;              // mesh version last pushed to the device
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guard(F &&f) {
+  try {
+    return f();
+  } catch (const std::invalid_argument &e) {
+    g_last_error = e.what();
+    return CFSA_ERR_INVALID;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    const std::string w = e.what();
+    return w.rfind("HIP error", 0) == 0 ? CFSA_ERR_DEVICE : CFSA_ERR_INVALID;
+  } catch (...) {
+    g_last_error = "unknown exception";
+    return CFSA_ERR_INVALID;
+  }
+}
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define CHECK_PTR(p)                                             \
+  do {                                                           \
+    if (!(p)) return fail(CFSA_ERR_INVALID, "null argument " #p); \
+  } while (0)
+
+cfsa::SolverOptions to_opts(const cfsa_solver_options &o) {
+  cfsa::SolverOptions s;
+  s.momentum_tol = o.momentum_tol;
+  s.pressure_tol = o.pressure_tol;
+  s.max_iter = o.max_iter;
+  s.pressure_solver = o.pressure_solver;
+  s.precision = o.precision;
+  s.inner_tol = o.inner_tol;
+  s.max_refine = o.max_refine;
+  s.check_every = o.check_every;
+  s.timing = o.timing;
+  return s;
+}
+
+void to_result(const cfsa::SolveResult &r, cfsa_solve_result *o) {
+  for (int c = 0; c < 3; ++c) {
+    o->iters[c] = r.iters[c];
+    o->err[c] = r.err[c];
+    o->restarts[c] = r.restarts[c];
+  }
+  o->refinements = r.refinements;
+  o->inner_iters = r.inner_iters;
+  o->ok = r.ok ? 1 : 0;
+}
+
+// The reference reads labels and element geometry live from the shared Mesh
+// on every step; push them when the mesh handle changed since the last sync.
+void sync_mesh(cfsa_ns *h) {
+  if (!h->mesh || !h->s.initialized || h->mesh->version == h->synced) return;
+  const cfsa::Mesh &m = h->mesh->m;
+  h->s.update_mesh(m.vertices.data(), m.p1_bc.data(), m.p2_bc.data());
+  h->synced = h->mesh->version;
+}
+
+int check_init(const cfsa_ns *h) {
+  if (!h->s.initialized) return fail(CFSA_ERR_NOT_INIT, "[NS] Error: solver not initialized");
+  return CFSA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cfsa_last_error(void) { return g_last_error.c_str(); }
+int cfsa_abi_version(void) { return CFSA_ABI_VERSION; }
+
+// ---------------------------------------------------------------------------
+// Mesh
+// ---------------------------------------------------------------------------
+int cfsa_mesh_structured_prism(double length, double width, double height, int nx, int ny,
+                               int nz, cfsa_mesh **out) {
+  CHECK_PTR(out);
+  return guard([&]() -> int {
+    auto m = std::make_unique<cfsa_mesh>();
+    m->m = cfsa::Mesh::structuredRectangularPrism(length, width, height, nx, ny, nz);
+    *out = m.release();
+    return CFSA_OK;
+  });
+}
+
+int cfsa_mesh_cylinder(double length, double radius, int divisions, double max_volume,
+                       cfsa_mesh **out) {
+  CHECK_PTR(out);
+  return guard([&]() -> int {
+    auto m = std::make_unique<cfsa_mesh>();
+    m->m = cfsa::Mesh::cylinder(length, radius, divisions, max_volume);
+    *out = m.release();
+    return CFSA_OK;
+  });
+}
+
+int cfsa_mesh_from_msh(const char *path, cfsa_mesh **out) {
+  CHECK_PTR(path);
+  CHECK_PTR(out);
+  return guard([&]() -> int {
+    auto m = std::make_unique<cfsa_mesh>();
+    try {
+      m->m = cfsa::Mesh::fromMshFile(path);
+    } catch (const std::runtime_error &e) {
+      return fail(CFSA_ERR_IO, e.what());
+    }
+    *out = m.release();
+    return CFSA_OK;
+  });
+}
+
+int cfsa_mesh_from_arrays(int64_t nv, const double *coords, int64_t nt, const int32_t *tets,
+                          cfsa_mesh **out) {
+  CHECK_PTR(coords);
+  CHECK_PTR(tets);
+  CHECK_PTR(out);
+  return guard([&]() -> int {
+    auto m = std::make_unique<cfsa_mesh>();
+    m->m = cfsa::Mesh::fromArrays(coords, nv, tets, nt);
+    *out = m.release();
+    return CFSA_OK;
+  });
+}
+
+int cfsa_mesh_setup_boundary_conditions(cfsa_mesh *m, const double dir[3], double coverage) {
+  CHECK_PTR(m);
+  CHECK_PTR(dir);
+  return guard([&]() -> int {
+    cfsa::Mesh::setupBoundaryConditions(dir, coverage, m->m);
+    m->version++;
+    return CFSA_OK;
+  });
+}
+
+int cfsa_mesh_set_p1_bc(cfsa_mesh *m, const uint8_t *bc) {
+  CHECK_PTR(m);
+  CHECK_PTR(bc);
+  return guard([&]() -> int {
+    for (int64_t i = 0; i < m->m.nVertices(); ++i)
+      if (bc[i] > 4) return fail(CFSA_ERR_INVALID, "label out of range");
+    m->m.p1_bc.assign(bc, bc + m->m.nVertices());
+    m->m.setP2BoundariesFromP1Boundaries();
+    m->version++;
+    return CFSA_OK;
+  });
+}
+
+int cfsa_mesh_update_vertices(cfsa_mesh *m, const double *coords) {
+  CHECK_PTR(m);
+  CHECK_PTR(coords);
+  return guard([&]() -> int {
+    m->m.vertices.assign(coords, coords + m->m.nVertices() * 3);
+    m->m.updateGeometry();
+    m->version++;
+    return CFSA_OK;
+  });
+}
+
+int cfsa_mesh_sizes(const cfsa_mesh *m, int64_t out[5]) {
+  CHECK_PTR(m);
+  CHECK_PTR(out);
+  out[0] = m->m.nVertices();
+  out[1] = m->m.nTets();
+  out[2] = m->m.nEdgeNodes();
+  out[3] = m->m.nFaces();
+  out[4] = out[0] + out[2];
+  return CFSA_OK;
+}
+
+#define MESH_COPY(fn, T, field)                                      \
+  int fn(const cfsa_mesh *m, T *out) {                               \
+    CHECK_PTR(m);                                                    \
+    CHECK_PTR(out);                                                  \
+    std::memcpy(out, m->m.field.data(), m->m.field.size() * sizeof(T)); \
+    return CFSA_OK;                                                  \
+  }
+MESH_COPY(cfsa_mesh_get_vertices, double, vertices)
+MESH_COPY(cfsa_mesh_get_tets, int32_t, tets)
+MESH_COPY(cfsa_mesh_get_volumes, double, volumes)
+MESH_COPY(cfsa_mesh_get_gradients, double, gradients)
+MESH_COPY(cfsa_mesh_get_tet_edge_nodes, int32_t, tet_edge_nodes)
+MESH_COPY(cfsa_mesh_get_edge_nodes, double, edge_nodes)
+MESH_COPY(cfsa_mesh_get_edge_endpoints, int32_t, edge_endpoints)
+MESH_COPY(cfsa_mesh_get_p1_bc, uint8_t, p1_bc)
+MESH_COPY(cfsa_mesh_get_p2_bc, uint8_t, p2_bc)
+
+int cfsa_mesh_get_faces(const cfsa_mesh *m, int32_t *verts, int32_t *tets, double *geom) {
+  CHECK_PTR(m);
+  const auto &F = m->m.faces;
+  for (size_t f = 0; f < F.size(); ++f) {
+    if (verts)
+      for (int a = 0; a < 3; ++a) verts[f * 3 + a] = F[f].vertids[(size_t)a];
+    if (tets) {
+      tets[f * 4 + 0] = F[f].tet_a;
+      tets[f * 4 + 1] = F[f].tet_b;
+      tets[f * 4 + 2] = F[f].face_id_a;
+      tets[f * 4 + 3] = F[f].face_id_b;
+    }
+    if (geom) {
+      for (int a = 0; a < 3; ++a) geom[f * 7 + a] = F[f].normal[(size_t)a];
+      geom[f * 7 + 3] = F[f].area;
+      for (int a = 0; a < 3; ++a) geom[f * 7 + 4 + a] = F[f].center[(size_t)a];
+    }
+  }
+  return CFSA_OK;
+}
+
+int cfsa_mesh_get_tet_faces(const cfsa_mesh *m, int32_t *faceids, int32_t *neighbors) {
+  CHECK_PTR(m);
+  if (faceids)
+    std::memcpy(faceids, m->m.tet_faceids.data(), m->m.tet_faceids.size() * sizeof(int32_t));
+  if (neighbors)
+    std::memcpy(neighbors, m->m.tet_neighbors.data(),
+                m->m.tet_neighbors.size() * sizeof(int32_t));
+  return CFSA_OK;
+}
+
+int cfsa_mesh_get_min_edge_length(const cfsa_mesh *m, double *out) {
+  CHECK_PTR(m);
+  CHECK_PTR(out);
+  *out = m->m.min_edge_length;
+  return CFSA_OK;
+}
+
+int cfsa_mesh_export_vtk(const cfsa_mesh *m, const char *path, double scale) {
+  CHECK_PTR(m);
+  CHECK_PTR(path);
+  return guard([&]() -> int {
+    if (!cfsa::write_mesh_vtk(path, m->m.vertices.data(), m->m.nVertices(), m->m.tets.data(),
+                              m->m.nTets(), m->m.p1_bc.data(), scale))
+      return fail(CFSA_ERR_IO, std::string("[IO] Failed to open file: ") + path);
+    return CFSA_OK;
+  });
+}
+
+void cfsa_mesh_destroy(cfsa_mesh *m) { delete m; }
+
+// ---------------------------------------------------------------------------
+// Solver
+// ---------------------------------------------------------------------------
+int cfsa_ns_create(cfsa_ns **out) {
+  CHECK_PTR(out);
+  return guard([&]() -> int {
+    *out = new cfsa_ns();
+    return CFSA_OK;
+  });
+}
+
+void cfsa_ns_destroy(cfsa_ns *h) {
+  try {
+    delete h;
+  } catch (...) {
+  }
+}
+
+int cfsa_ns_initialize(cfsa_ns *h, const cfsa_mesh *mesh, double density, double viscosity) {
+  CHECK_PTR(h);
+  CHECK_PTR(mesh);
+  return guard([&]() -> int {
+    const cfsa::Mesh &m = mesh->m;
+    cfsa::MeshInput in;
+    in.nv = m.nVertices();
+    in.nt = m.nTets();
+    in.ne = m.nEdgeNodes();
+    in.coords = m.vertices.data();
+    in.tets = m.tets.data();
+    in.ten = m.tet_edge_nodes.data();
+    in.p1_bc = m.p1_bc.data();
+    in.p2_bc = m.p2_bc.data();
+    h->s.initialize(in, density, viscosity);
+    h->mesh = mesh;
+    h->synced = mesh->version;
+    return CFSA_OK;
+  });
+}
+
+int cfsa_ns_initialize_arrays(cfsa_ns *h, int64_t nv, const double *coords, int64_t nt,
+                              const int32_t *tets, int64_t ne, const int32_t *ten,
+                              const uint8_t *p1_bc, const uint8_t *p2_bc, double density,
+                              double viscosity) {
+  CHECK_PTR(h);
+  return guard([&]() -> int {
+    cfsa::MeshInput in;
+    in.nv = nv;
+    in.nt = nt;
+    in.ne = ne;
+    in.coords = coords;
+    in.tets = tets;
+    in.ten = ten;
+    in.p1_bc = p1_bc;
+    in.p2_bc = p2_bc;
+    h->s.initialize(in, density, viscosity);
+    h->mesh = nullptr;
+    return CFSA_OK;
+  });
+}
+
+int cfsa_ns_update_mesh(cfsa_ns *h, const double *coords, const uint8_t *p1_bc,
+                        const uint8_t *p2_bc) {
+  CHECK_PTR(h);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.update_mesh(coords, p1_bc, p2_bc);
+    return CFSA_OK;
+  });
+}
+
+int cfsa_ns_build_sparse_matrices(cfsa_ns *h) {
+  CHECK_PTR(h);
+  if (!h->s.initialized) {
+    std::cerr << "[NS] Error: mesh not initialized" << std::endl;
+    return fail(CFSA_ERR_NOT_INIT, "[NS] Error: mesh not initialized");
+  }
+  return guard([&]() -> int {
+    if (h->mesh) {
+      const cfsa::Mesh &m = h->mesh->m;
+      h->s.update_mesh(m.vertices.data(), m.p1_bc.data(), m.p2_bc.data());
+      h->synced = h->mesh->version;
+    }
+    h->s.build_sparse_matrices();
+    return CFSA_OK;
+  });
+}
+
+int cfsa_ns_piso_step(cfsa_ns *h) {
+  CHECK_PTR(h);
+  return guard([&]() -> int {
+    sync_mesh(h);
+    const int st = h->s.piso_step();
+    if (st != CFSA_OK && st != CFSA_ERR_NAN) g_last_error = h->s.last_error;
+    if (st == CFSA_ERR_NAN) g_last_error = "[NS] NaN in velocity or pressure after the step";
+    return st;
+  });
+}
+
+int cfsa_ns_simple_step(cfsa_ns *h) {
+  CHECK_PTR(h);
+  return guard([&]() -> int {
+    sync_mesh(h);
+    const int st = h->s.simple_step();
+    if (st != CFSA_OK) g_last_error = h->s.last_error;
+    return st;
+  });
+}
+
+int cfsa_ns_set_mean_inlet_velocity(cfsa_ns *h, const double v[3]) {
+  CHECK_PTR(h);
+  CHECK_PTR(v);
+  return guard([&]() -> int {
+    h->s.set_mean_inlet(v);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_set_inlet_profile(cfsa_ns *h, const double *g) {
+  CHECK_PTR(h);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.set_inlet_profile(g);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_set_outlet_type(cfsa_ns *h, int type) {
+  CHECK_PTR(h);
+  if (type < 0 || type > 2) return fail(CFSA_ERR_INVALID, "outlet type out of range");
+  h->s.set_outlet_type(type);
+  return CFSA_OK;
+}
+int cfsa_ns_set_outlet_pressure(cfsa_ns *h, double p) {
+  CHECK_PTR(h);
+  if (h->s.outlet_type != 0)
+    std::cout << "[NS Solver] Warning: setting outlet pressure without dirichlet outlet being "
+                 "the outlet type"
+              << std::endl;
+  h->s.outlet_pressure = p;
+  return CFSA_OK;
+}
+int cfsa_ns_set_inlet_type(cfsa_ns *h, int type) {
+  CHECK_PTR(h);
+  h->s.inlet_type = type;
+  return CFSA_OK;
+}
+int cfsa_ns_set_number_of_corrections(cfsa_ns *h, int n) {
+  CHECK_PTR(h);
+  h->s.n_corrections = n;
+  return CFSA_OK;
+}
+int cfsa_ns_set_relax_u(cfsa_ns *h, double r) {
+  CHECK_PTR(h);
+  h->s.relax_u = r;
+  return CFSA_OK;
+}
+int cfsa_ns_set_relax_p(cfsa_ns *h, double r) {
+  CHECK_PTR(h);
+  h->s.relax_p = r;
+  return CFSA_OK;
+}
+int cfsa_ns_set_reference_pressure_node(cfsa_ns *h, int node) {
+  CHECK_PTR(h);
+  h->s.set_reference_node(node);
+  return CFSA_OK;
+}
+int cfsa_ns_set_dt(cfsa_ns *h, double dt) {
+  CHECK_PTR(h);
+  h->s.set_dt(dt);
+  return CFSA_OK;
+}
+int cfsa_ns_set_tolerance(cfsa_ns *h, double tol) {
+  CHECK_PTR(h);
+  h->s.tolerance = tol;
+  return CFSA_OK;
+}
+int cfsa_ns_set_time(cfsa_ns *h, double t) {
+  CHECK_PTR(h);
+  h->s.time = t;
+  return CFSA_OK;
+}
+int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt) {
+  CHECK_PTR(h);
+  CHECK_PTR(opt);
+  if (opt->max_iter < 0 || opt->check_every < 1 || opt->precision < 0 || opt->precision > 1 ||
+      opt->pressure_solver < 0 || opt->pressure_solver > 1 || !(opt->momentum_tol >= 0) ||
+      !(opt->pressure_tol >= 0) || !(opt->inner_tol > 0) || opt->max_refine < 0)
+    return fail(CFSA_ERR_INVALID, "invalid solver options");
+  h->s.opt = to_opts(*opt);
+  return CFSA_OK;
+}
+int cfsa_ns_get_options(const cfsa_ns *h, cfsa_solver_options *o) {
+  CHECK_PTR(h);
+  CHECK_PTR(o);
+  const auto &s = h->s.opt;
+  o->momentum_tol = s.momentum_tol;
+  o->pressure_tol = s.pressure_tol;
+  o->max_iter = s.max_iter;
+  o->pressure_solver = s.pressure_solver;
+  o->precision = s.precision;
+  o->inner_tol = s.inner_tol;
+  o->max_refine = s.max_refine;
+  o->check_every = s.check_every;
+  o->timing = s.timing;
+  return CFSA_OK;
+}
+
+int cfsa_ns_get_sizes(const cfsa_ns *h, int64_t out[6]) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  out[0] = h->s.nv;
+  out[1] = h->s.nt;
+  out[2] = h->s.ne;
+  out[3] = h->s.nu;
+  out[4] = h->s.nnzA;
+  out[5] = h->s.nnzP;
+  return CFSA_OK;
+}
+
+#define NS_FIELD(fn, method, T)                  \
+  int fn(cfsa_ns *h, T *p) {                     \
+    CHECK_PTR(h);                                \
+    CHECK_PTR(p);                                \
+    if (int e = check_init(h)) return e;         \
+    return guard([&]() -> int {                           \
+      h->s.method(p);                            \
+      return CFSA_OK;                            \
+    });                                          \
+  }
+NS_FIELD(cfsa_ns_get_velocity, get_velocity, double)
+NS_FIELD(cfsa_ns_set_velocity, set_velocity, const double)
+NS_FIELD(cfsa_ns_get_velocity_star, get_velocity_star, double)
+NS_FIELD(cfsa_ns_get_pressure, get_pressure, double)
+NS_FIELD(cfsa_ns_set_pressure, set_pressure, const double)
+NS_FIELD(cfsa_ns_get_pressure_correction, get_pressure_correction, double)
+NS_FIELD(cfsa_ns_get_inv_lumped_mass, get_inv_lumped_mass, double)
+NS_FIELD(cfsa_ns_compute_advection, compute_advection, double)
+
+int cfsa_ns_get_velocity_at_node(cfsa_ns *h, int64_t node, double out[3]) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  if (int e = check_init(h)) return e;
+  if (node < 0 || node >= h->s.nu) return fail(CFSA_ERR_INVALID, "node out of range");
+  return guard([&]() -> int {
+    std::vector<double> u((size_t)h->s.nu * 3);
+    h->s.get_velocity(u.data());
+    for (int c = 0; c < 3; ++c) out[c] = u[(size_t)node * 3 + c];
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_get_pressure_at_node(cfsa_ns *h, int64_t node, double *out) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  if (int e = check_init(h)) return e;
+  if (node < 0 || node >= h->s.nv) return fail(CFSA_ERR_INVALID, "node out of range");
+  return guard([&]() -> int {
+    std::vector<double> p((size_t)h->s.nv);
+    h->s.get_pressure(p.data());
+    *out = p[(size_t)node];
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_get_scalar(const cfsa_ns *h, int which, double *out) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  const auto &s = h->s;
+  switch (which) {
+    case 0: *out = s.time; break;
+    case 1: *out = s.dt; break;
+    case 2: *out = s.viscosity; break;
+    case 3: *out = s.density; break;
+    case 4: *out = s.tolerance; break;
+    case 5: *out = s.outlet_pressure; break;
+    default: return fail(CFSA_ERR_INVALID, "unknown scalar id");
+  }
+  return CFSA_OK;
+}
+int cfsa_ns_get_reference_pressure_node(const cfsa_ns *h, int *out) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  *out = h->s.reference_node;
+  return CFSA_OK;
+}
+int cfsa_ns_has_nans(cfsa_ns *h, int *out) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    *out = h->s.has_nans() ? 1 : 0;
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_get_step_stats(const cfsa_ns *h, cfsa_step_stats *o) {
+  CHECK_PTR(h);
+  CHECK_PTR(o);
+  static_assert(sizeof(cfsa_step_stats) == sizeof(cfsa::StepStats), "stats layout");
+  std::memcpy(o, &h->s.last, sizeof(*o));
+  return CFSA_OK;
+}
+
+int cfsa_ns_export_vtk(cfsa_ns *h, const char *path, double scale) {
+  CHECK_PTR(h);
+  CHECK_PTR(path);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    const int64_t nv = h->s.nv, nt = h->s.nt;
+    std::vector<double> coords((size_t)nv * 3), u((size_t)h->s.nu * 3), p((size_t)nv);
+    std::vector<int32_t> tets((size_t)nt * 4);
+    std::vector<uint8_t> bc((size_t)nv);
+    h->s.get_mesh_arrays(coords.data(), tets.data(), bc.data());
+    h->s.get_velocity(u.data());
+    h->s.get_pressure(p.data());
+    if (!cfsa::write_mesh_vtk(path, coords.data(), nv, tets.data(), nt, bc.data(), scale))
+      return fail(CFSA_ERR_IO, std::string("[IO] Failed to open file: ") + path);
+    if (!cfsa::append_ns_fields_vtk(path, p.data(), u.data(), nv, scale))
+      return fail(CFSA_ERR_IO, std::string("[IO] Failed to open VTK file for appending: ") + path);
+    return CFSA_OK;
+  });
+}
+
+int cfsa_ns_get_matrix(cfsa_ns *h, int which, int32_t *rowptr, int32_t *col, double *val) {
+  CHECK_PTR(h);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.get_matrix(which, rowptr, col, val);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_get_num_colours(const cfsa_ns *h, int *out) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  *out = h->s.ncolours;
+  return CFSA_OK;
+}
+int cfsa_ns_get_colouring(const cfsa_ns *h, int32_t *colour) {
+  CHECK_PTR(h);
+  CHECK_PTR(colour);
+  if (!h->s.initialized) return fail(CFSA_ERR_NOT_INIT, "solver not initialized");
+  h->s.get_colouring(colour);
+  return CFSA_OK;
+}
+int cfsa_ns_get_incidence(cfsa_ns *h, int32_t *ptr, int32_t *tet) {
+  CHECK_PTR(h);
+  CHECK_PTR(ptr);
+  CHECK_PTR(tet);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.get_incidence(ptr, tet);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_compute_divergence(cfsa_ns *h, const double *u, double *out) {
+  CHECK_PTR(h);
+  CHECK_PTR(u);
+  CHECK_PTR(out);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.compute_divergence(u, out);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_apply_matrix(cfsa_ns *h, int which, const double *x, double *y, int nrhs) {
+  CHECK_PTR(h);
+  CHECK_PTR(x);
+  CHECK_PTR(y);
+  if (int e = check_init(h)) return e;
+  if (which < 0 || which > 7 || (nrhs != 1 && nrhs != 3) || (which >= 3 && which <= 4 && nrhs != 1))
+    return fail(CFSA_ERR_INVALID, "apply_matrix: bad matrix id or nrhs");
+  return guard([&]() -> int {
+    h->s.apply_matrix(which, x, y, nrhs);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_krylov_solve(cfsa_ns *h, int which, int method, int nrhs, const double *b,
+                         double *x, double tol, int maxit, int precision,
+                         cfsa_solve_result *res) {
+  CHECK_PTR(h);
+  CHECK_PTR(b);
+  CHECK_PTR(x);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    cfsa::SolveResult r = h->s.krylov_solve(which, method, nrhs, b, x, tol, maxit, precision);
+    if (res) to_result(r, res);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_time_momentum_spmv(cfsa_ns *h, int reps, int precision, double *ms) {
+  CHECK_PTR(h);
+  CHECK_PTR(ms);
+  if (int e = check_init(h)) return e;
+  if (reps < 1) return fail(CFSA_ERR_INVALID, "reps < 1");
+  return guard([&]() -> int {
+    *ms = h->s.time_momentum_spmv(reps, precision);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_get_stream(const cfsa_ns *h, void **stream) {
+  CHECK_PTR(h);
+  CHECK_PTR(stream);
+  *stream = (void *)h->s.stream();
+  return CFSA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1413 @@
+// HIP kernels of the PISO step (gfx950, wave64).
+//
+// CSR SpMV is "CSR-stream": a block owns a fixed run of rows; the block's
+// nonzeros are streamed in coalesced chunks, every product a_ij * x_j is
+// staged in LDS, then the thread owning (row, component) sums its row's
+// products in ascending column order.  That sequential order equals the
+// accumulation order of Eigen's column-major sparse * dense product
+// (res(i) += A(i,j) * x(j) over ascending j), so with FMA contraction off the
+// FP64 SpMV reproduces the reference bit for bit.
+#include "ops.hpp"
+
+#include <cfloat>
+
+namespace cfsa {
+
+// ===========================================================================
+// CSR-stream core
+// ===========================================================================
+template <class T, int KO>
+struct StreamCfg;
+template <>
+struct StreamCfg<double, 3> {
+  static constexpr int R = 85, B = 2048;  // 48 KiB of staged products
+};
+template <>
+struct StreamCfg<double, 1> {
+  static constexpr int R = 256, B = 4096;  // 32 KiB
+};
+template <>
+struct StreamCfg<float, 3> {
+  static constexpr int R = 85, B = 4096;  // 48 KiB
+};
+template <>
+struct StreamCfg<float, 1> {
+  static constexpr int R = 256, B = 8192;  // 32 KiB
+};
+
+template <class T, int KO, int ND, class Op>
+__global__ __launch_bounds__(kBlock) void k_csr_stream(int n,
+                                                       const int *__restrict__ rowptr,
+                                                       Op op,
+                                                       double *__restrict__ partials) {
+  constexpr int R = StreamCfg<T, KO>::R, B = StreamCfg<T, KO>::B;
+  constexpr int NDs = ND > 0 ? ND : 1;
+  __shared__ T lds[B * KO];
+  __shared__ double red[NDs][kBlock];
+  if (!op.any()) return;
+  double dots[NDs];
+#pragma unroll
+  for (int d = 0; d < NDs; ++d) dots[d] = 0.0;
+  const int tid = threadIdx.x;
+  const int myc = tid % KO;
+  const bool act = op.active(myc);
+  const int nrb = (n + R - 1) / R;
+  for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const int r0 = rb * R, r1 = min(n, r0 + R);
+    const int k0 = rowptr[r0], k1 = rowptr[r1];
+    const bool owner = tid < KO * (r1 - r0);
+    const int row = r0 + tid / KO;
+    int rs = 0, re = 0;
+    if (owner) {
+      rs = rowptr[row];
+      re = rowptr[row + 1];
+    }
+    T acc = T(0);
+    for (int kb = k0; kb < k1; kb += B) {
+      const int len = min(B, k1 - kb);
+      __syncthreads();
+#pragma unroll 4
+      for (int i = tid; i < len; i += kBlock) {
+        T p[KO];
+        op.prod(kb + i, p);
+#pragma unroll
+        for (int c = 0; c < KO; ++c) lds[i * KO + c] = p[c];
+      }
+      __syncthreads();
+      if (owner && act) {
+        const int a = max(rs, kb), b = min(re, kb + len);
+        for (int k = a; k < b; ++k) acc += lds[(k - kb) * KO + myc];
+      }
+    }
+    if (owner && act) op.epi(row, myc, acc, dots);
+  }
+  if constexpr (ND > 0) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) red[d][tid] = dots[d];
+    __syncthreads();
+    if (tid < KO * ND) {
+      const int c = tid / ND, d = tid % ND;
+      double s = 0.0;
+      for (int t = c; t < kBlock; t += KO) s += red[d][t];
+      partials[(size_t)(c * ND + d) * gridDim.x + blockIdx.x] = s;
+    }
+  }
+}
+
+template <class T, int KO>
+static int stream_grid(int n) {
+  const int nrb = div_up(n, StreamCfg<T, KO>::R);
+  return std::max(1, std::min(nrb, kSpmvGrid));
+}
+
+// ---- operator functors -----------------------------------------------------
+struct MaskBase {
+  const int *mask;
+  __device__ bool active(int c) const { return mask == nullptr || mask[c] != 0; }
+  __device__ bool any() const {
+    return mask == nullptr || (mask[0] | mask[1] | mask[2]) != 0;
+  }
+};
+struct Mask1Base {
+  const int *mask;
+  __device__ bool active(int) const { return mask == nullptr || mask[0] != 0; }
+  __device__ bool any() const { return mask == nullptr || mask[0] != 0; }
+};
+
+template <class T, int KO, class Base>
+struct OpAx : Base {
+  const int *__restrict__ col;
+  const T *__restrict__ val;
+  const T *__restrict__ x;
+  T *__restrict__ y;
+  __device__ void prod(int k, T *p) const {
+    const int j = col[k];
+    const T a = val[k];
+#pragma unroll
+    for (int c = 0; c < KO; ++c) p[c] = a * x[(size_t)j * KO + c];
+  }
+  __device__ void epi(int row, int c, T acc, double *) const {
+    y[(size_t)row * KO + c] = acc;
+  }
+};
+
+template <class T, int KO, class Base>
+struct OpAxDot : OpAx<T, KO, Base> {
+  const T *__restrict__ r0;
+  __device__ void epi(int row, int c, T acc, double *d) const {
+    const size_t i = (size_t)row * KO + c;
+    this->y[i] = acc;
+    d[0] += (double)r0[i] * (double)acc;
+  }
+};
+
+template <class T, int KO, class Base>
+struct OpAxDot2 : OpAx<T, KO, Base> {
+  const T *__restrict__ sv;
+  __device__ void epi(int row, int c, T acc, double *d) const {
+    const size_t i = (size_t)row * KO + c;
+    this->y[i] = acc;
+    const double t = (double)acc;
+    d[0] += t * t;
+    d[1] += t * (double)sv[i];
+  }
+};
+
+template <class T, int KO, class Base>
+struct OpResidual : OpAx<T, KO, Base> {
+  const T *__restrict__ b;
+  T *__restrict__ r0;
+  __device__ void epi(int row, int c, T acc, double *d) const {
+    const size_t i = (size_t)row * KO + c;
+    const T r = b[i] - acc;
+    this->y[i] = r;
+    r0[i] = r;
+    d[0] += (double)r * (double)r;
+  }
+};
+
+struct OpGrad3 : MaskBase {
+  const int *__restrict__ col;
+  const double *__restrict__ g;
+  const double *__restrict__ x;
+  double *__restrict__ y;
+  __device__ void prod(int k, double *p) const {
+    const double xj = x[col[k]];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p[c] = g[(size_t)k * 3 + c] * xj;
+  }
+  __device__ void epi(int row, int c, double acc, double *) const {
+    y[(size_t)row * 3 + c] = acc;
+  }
+};
+
+struct OpMomRhs : MaskBase {
+  const int *__restrict__ col;
+  const double *__restrict__ M;
+  const double *__restrict__ u;
+  const double *__restrict__ gp;
+  const double *__restrict__ conv;
+  const uint8_t *__restrict__ bc;
+  const double *__restrict__ ginlet;
+  double *__restrict__ rhs;
+  double dt, penalty;
+  int nv;
+  __device__ void prod(int k, double *p) const {
+    const int j = col[k];
+    const double a = M[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p[c] = a * u[(size_t)j * 3 + c];
+  }
+  __device__ void epi(int row, int c, double acc, double *) const {
+    const size_t i = (size_t)row * 3 + c;
+    // rhs = 0; rhs += (M u)/dt; rhs(P1) -= G p; rhs += conv; rhs += pen*u_bc
+    double v = 0.0 + acc / dt;
+    if (row < nv) v = v - gp[i];
+    v = v + conv[i];
+    const uint8_t b = bc[row];
+    if (b == kInlet)
+      v = v + penalty * ginlet[i];
+    else if (b == kWall)
+      v = v + penalty * 0.0;
+    rhs[i] = v;
+  }
+};
+
+// ---- launchers ---------------------------------------------------------------
+template <class T>
+void spmv3(hipStream_t s, const DevCsr &A, const T *val, const T *x, T *y,
+           const int *active) {
+  OpAx<T, 3, MaskBase> op{};
+  op.mask = active;
+  op.col = A.col;
+  op.val = val;
+  op.x = x;
+  op.y = y;
+  const int G = stream_grid<T, 3>(A.n);
+  hipLaunchKernelGGL((k_csr_stream<T, 3, 0, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
+                     A.n, A.rowptr, op, nullptr);
+  CFSA_LAUNCH_CHECK();
+}
+
+template <class T>
+void spmv3_dot(hipStream_t s, const DevCsr &A, const T *val, const T *y, T *v,
+               const T *r0, const int *active, double *partials, int *grid) {
+  OpAxDot<T, 3, MaskBase> op{};
+  op.mask = active;
+  op.col = A.col;
+  op.val = val;
+  op.x = y;
+  op.y = v;
+  op.r0 = r0;
+  const int G = stream_grid<T, 3>(A.n);
+  *grid = G;
+  hipLaunchKernelGGL((k_csr_stream<T, 3, 1, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
+                     A.n, A.rowptr, op, partials);
+  CFSA_LAUNCH_CHECK();
+}
+
+template <class T>
+void spmv3_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
+                const T *sv, const int *active, double *partials, int *grid) {
+  OpAxDot2<T, 3, MaskBase> op{};
+  op.mask = active;
+  op.col = A.col;
+  op.val = val;
+  op.x = z;
+  op.y = t;
+  op.sv = sv;
+  const int G = stream_grid<T, 3>(A.n);
+  *grid = G;
+  hipLaunchKernelGGL((k_csr_stream<T, 3, 2, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
+                     A.n, A.rowptr, op, partials);
+  CFSA_LAUNCH_CHECK();
+}
+
+template <class T>
+void spmv3_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
+                    const T *b, T *r, T *r0, const int *active, double *partials,
+                    int *grid) {
+  OpResidual<T, 3, MaskBase> op{};
+  op.mask = active;
+  op.col = A.col;
+  op.val = val;
+  op.x = x;
+  op.y = r;
+  op.b = b;
+  op.r0 = r0;
+  const int G = stream_grid<T, 3>(A.n);
+  *grid = G;
+  hipLaunchKernelGGL((k_csr_stream<T, 3, 1, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
+                     A.n, A.rowptr, op, partials);
+  CFSA_LAUNCH_CHECK();
+}
+
+template <class T>
+void spmv1(hipStream_t s, const DevCsr &A, const T *val, const T *x, T *y) {
+  OpAx<T, 1, Mask1Base> op{};
+  op.mask = nullptr;
+  op.col = A.col;
+  op.val = val;
+  op.x = x;
+  op.y = y;
+  const int G = stream_grid<T, 1>(A.n);
+  hipLaunchKernelGGL((k_csr_stream<T, 1, 0, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
+                     A.n, A.rowptr, op, nullptr);
+  CFSA_LAUNCH_CHECK();
+}
+
+template <class T>
+void spmv1_dot(hipStream_t s, const DevCsr &A, const T *val, const T *y, T *v,
+               const T *r0, const int *active, double *partials, int *grid) {
+  OpAxDot<T, 1, Mask1Base> op{};
+  op.mask = active;
+  op.col = A.col;
+  op.val = val;
+  op.x = y;
+  op.y = v;
+  op.r0 = r0;
+  const int G = stream_grid<T, 1>(A.n);
+  *grid = G;
+  hipLaunchKernelGGL((k_csr_stream<T, 1, 1, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
+                     A.n, A.rowptr, op, partials);
+  CFSA_LAUNCH_CHECK();
+}
+
+template <class T>
+void spmv1_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
+                const T *sv, const int *active, double *partials, int *grid) {
+  OpAxDot2<T, 1, Mask1Base> op{};
+  op.mask = active;
+  op.col = A.col;
+  op.val = val;
+  op.x = z;
+  op.y = t;
+  op.sv = sv;
+  const int G = stream_grid<T, 1>(A.n);
+  *grid = G;
+  hipLaunchKernelGGL((k_csr_stream<T, 1, 2, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
+                     A.n, A.rowptr, op, partials);
+  CFSA_LAUNCH_CHECK();
+}
+
+template <class T>
+void spmv1_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
+                    const T *b, T *r, T *r0, const int *active, double *partials,
+                    int *grid) {
+  OpResidual<T, 1, Mask1Base> op{};
+  op.mask = active;
+  op.col = A.col;
+  op.val = val;
+  op.x = x;
+  op.y = r;
+  op.b = b;
+  op.r0 = r0;
+  const int G = stream_grid<T, 1>(A.n);
+  *grid = G;
+  hipLaunchKernelGGL((k_csr_stream<T, 1, 1, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
+                     A.n, A.rowptr, op, partials);
+  CFSA_LAUNCH_CHECK();
+}
+
+void spmv_grad3(hipStream_t s, const DevCsr &P, const double *gval, const double *x,
+                double *y) {
+  OpGrad3 op{};
+  op.mask = nullptr;
+  op.col = P.col;
+  op.g = gval;
+  op.x = x;
+  op.y = y;
+  const int G = stream_grid<double, 3>(P.n);
+  hipLaunchKernelGGL((k_csr_stream<double, 3, 0, OpGrad3>), dim3(G), dim3(kBlock), 0, s,
+                     P.n, P.rowptr, op, nullptr);
+  CFSA_LAUNCH_CHECK();
+}
+
+void momentum_rhs(hipStream_t s, const DevCsr &A, const double *M, const double *u,
+                  const double *gp, const double *conv, const uint8_t *bc,
+                  const double *ginlet, double dt, int nv, double penalty,
+                  double *rhs) {
+  OpMomRhs op{};
+  op.mask = nullptr;
+  op.col = A.col;
+  op.M = M;
+  op.u = u;
+  op.gp = gp;
+  op.conv = conv;
+  op.bc = bc;
+  op.ginlet = ginlet;
+  op.rhs = rhs;
+  op.dt = dt;
+  op.penalty = penalty;
+  op.nv = nv;
+  const int G = stream_grid<double, 3>(A.n);
+  hipLaunchKernelGGL((k_csr_stream<double, 3, 0, OpMomRhs>), dim3(G), dim3(kBlock), 0, s,
+                     A.n, A.rowptr, op, nullptr);
+  CFSA_LAUNCH_CHECK();
+}
+
+#define CFSA_INST_SPMV(T)                                                              \
+  template void spmv3<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,        \
+                         const int *);                                                  \
+  template void spmv3_dot<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,    \
+                             const T *, const int *, double *, int *);                  \
+  template void spmv3_dot2<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,   \
+                              const T *, const int *, double *, int *);                 \
+  template void spmv3_residual<T>(hipStream_t, const DevCsr &, const T *, const T *,    \
+                                  const T *, T *, T *, const int *, double *, int *);   \
+  template void spmv1<T>(hipStream_t, const DevCsr &, const T *, const T *, T *);       \
+  template void spmv1_dot<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,    \
+                             const T *, const int *, double *, int *);                  \
+  template void spmv1_dot2<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,   \
+                              const T *, const int *, double *, int *);                 \
+  template void spmv1_residual<T>(hipStream_t, const DevCsr &, const T *, const T *,    \
+                                  const T *, T *, T *, const int *, double *, int *);
+CFSA_INST_SPMV(double)
+CFSA_INST_SPMV(float)
+
+// ===========================================================================
+// Deterministic reductions
+// ===========================================================================
+int vec_grid(int n) { return std::max(1, std::min(div_up(n, kBlock), kVecGrid)); }
+
+// Fixed-shape tree over one block; writes partials[v * gridDim.x + block].
+template <int NV>
+__device__ void block_partials(double (&v)[NV], double *__restrict__ partials) {
+  __shared__ double red[NV][kBlock];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) red[i][tid] = v[i];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+#pragma unroll
+      for (int i = 0; i < NV; ++i) red[i][tid] += red[i][tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid < NV) partials[(size_t)tid * gridDim.x + blockIdx.x] = red[tid][0];
+}
+
+// Single-block final reduction of NV partial arrays of length G into out[].
+template <int NV>
+__device__ void reduce_final(const double *__restrict__ partials, int G, double *out) {
+  __shared__ double red[NV][kBlock];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (int g = tid; g < G; g += kBlock) s += partials[(size_t)i * G + g];
+    red[i][tid] = s;
+  }
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+#pragma unroll
+      for (int i = 0; i < NV; ++i) red[i][tid] += red[i][tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) out[i] = red[i][0];
+  }
+  __syncthreads();
+}
+
+// ===========================================================================
+// BiCGSTAB (Eigen's right-preconditioned van der Vorst form)
+// ===========================================================================
+template <class T, int K>
+__global__ __launch_bounds__(kBlock) void k_bicg_init(int n, const T *__restrict__ b,
+                                                      T *__restrict__ x, T *__restrict__ r,
+                                                      T *__restrict__ r0, T *__restrict__ p,
+                                                      T *__restrict__ v,
+                                                      double *__restrict__ partials) {
+  double acc[2 * K];
+#pragma unroll
+  for (int i = 0; i < 2 * K; ++i) acc[i] = 0.0;
+  const size_t N = (size_t)n * K;
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
+       e += (size_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const size_t i = e * K + c;
+      const T bi = b[i];
+      x[i] = T(0);
+      r[i] = bi;
+      r0[i] = bi;
+      p[i] = T(0);
+      v[i] = T(0);
+      const double d = (double)bi * (double)bi;
+      acc[2 * c] += d;
+      acc[2 * c + 1] += d;
+    }
+  }
+  (void)N;
+  block_partials<2 * K>(acc, partials);
+}
+
+template <int K>
+__global__ void k_bicg_s_init(KrylovState *st, const double *partials, int G) {
+  __shared__ double tot[2 * K];
+  reduce_final<2 * K>(partials, G, tot);
+  if (threadIdx.x != 0) return;
+  int any = 0;
+  for (int c = 0; c < K; ++c) {
+    const double bb = tot[2 * c];
+    st->rhs_sq[c] = bb;
+    st->r0_sq[c] = bb;
+    st->rr[c] = bb;
+    st->rho[c] = 1.0;
+    st->alpha[c] = 1.0;
+    st->omega[c] = 1.0;
+    st->beta[c] = 0.0;
+    st->iters[c] = 0;
+    st->restarts[c] = 0;
+    st->active[c] = 0;
+    st->tol2[c] = st->tol * st->tol * bb;
+    if (bb == 0.0) {
+      st->status[c] = KS_DONE;  // x = 0 and success
+      st->err[c] = 0.0;
+    } else {
+      st->status[c] = KS_RUNNING;
+      st->err[c] = 1.0;
+      any = 1;
+    }
+  }
+  for (int c = K; c < 3; ++c) {
+    st->status[c] = KS_DONE;
+    st->active[c] = 0;
+  }
+  st->any_active = any;
+}
+
+template <int K>
+__global__ void k_bicg_s_begin(KrylovState *st, const double *partials, int G) {
+  __shared__ double tot[2 * K];
+  reduce_final<2 * K>(partials, G, tot);
+  if (threadIdx.x != 0) return;
+  const double eps = DBL_EPSILON;
+  const double eps2 = eps * eps;
+  int any = 0;
+  for (int c = 0; c < K; ++c) {
+    const int status = st->status[c];
+    if (status == KS_RESUME) {
+      st->beta[c] = (st->rho[c] / st->rho_old[c]) * (st->alpha[c] / st->omega[c]);
+      st->status[c] = KS_RUNNING;
+      st->active[c] = 1;
+      any = 1;
+      continue;
+    }
+    if (status != KS_RUNNING) {
+      st->active[c] = 0;
+      continue;
+    }
+    const double rr = tot[2 * c];
+    st->rr[c] = rr;
+    if (!(rr > st->tol2[c] && st->iters[c] < st->maxit)) {
+      st->status[c] = KS_DONE;
+      st->err[c] = sqrt(rr / st->rhs_sq[c]);
+      st->active[c] = 0;
+      continue;
+    }
+    st->rho_old[c] = st->rho[c];
+    const double rho = tot[2 * c + 1];
+    st->rho[c] = rho;
+    if (fabs(rho) < eps2 * st->r0_sq[c]) {
+      st->status[c] = KS_RESTART;
+      st->active[c] = 0;
+      continue;
+    }
+    st->beta[c] = (rho / st->rho_old[c]) * (st->alpha[c] / st->omega[c]);
+    st->active[c] = 1;
+    any = 1;
+  }
+  st->any_active = any;
+}
+
+template <int K>
+__global__ void k_bicg_s_alpha(KrylovState *st, const double *partials, int G) {
+  __shared__ double tot[K];
+  reduce_final<K>(partials, G, tot);
+  if (threadIdx.x != 0) return;
+  for (int c = 0; c < K; ++c) {
+    if (!st->active[c]) continue;
+    st->r0v[c] = tot[c];
+    st->alpha[c] = st->rho[c] / tot[c];
+  }
+}
+
+template <int K>
+__global__ void k_bicg_s_omega(KrylovState *st, const double *partials, int G) {
+  __shared__ double tot[2 * K];
+  reduce_final<2 * K>(partials, G, tot);
+  if (threadIdx.x != 0) return;
+  for (int c = 0; c < K; ++c) {
+    if (!st->active[c]) continue;
+    const double tt = tot[2 * c], ts = tot[2 * c + 1];
+    st->tt[c] = tt;
+    st->ts[c] = ts;
+    st->omega[c] = (tt > 0.0) ? ts / tt : 0.0;
+    st->iters[c] += 1;
+  }
+}
+
+// After r = b - A x, r0 = r for the masked right-hand sides.
+template <int K>
+__global__ void k_bicg_s_restart(KrylovState *st, const double *partials, int G,
+                                 const int *mask) {
+  __shared__ double tot[K];
+  reduce_final<K>(partials, G, tot);
+  if (threadIdx.x != 0) return;
+  for (int c = 0; c < K; ++c) {
+    if (!mask[c]) continue;
+    st->rho[c] = tot[c];
+    st->r0_sq[c] = tot[c];
+    if (st->restarts[c]++ == 0) st->iters[c] = 0;
+    st->status[c] = KS_RESUME;
+  }
+}
+
+template <class T, int K>
+__global__ __launch_bounds__(kBlock) void k_bicg_v1(int n, const KrylovState *__restrict__ st,
+                                                    const T *__restrict__ r, T *__restrict__ p,
+                                                    const T *__restrict__ v,
+                                                    const T *__restrict__ invd,
+                                                    T *__restrict__ y) {
+  if (!st->any_active) return;
+  int act[K];
+  double beta[K], om[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    act[c] = st->active[c];
+    beta[c] = st->beta[c];
+    om[c] = st->omega[c];
+  }
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
+       e += (size_t)gridDim.x * kBlock) {
+    const double di = (double)invd[e];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      if (!act[c]) continue;
+      const size_t i = e * K + c;
+      const double pi = (double)r[i] + beta[c] * ((double)p[i] - om[c] * (double)v[i]);
+      const T pt = (T)pi;
+      p[i] = pt;
+      y[i] = (T)(di * (double)pt);
+    }
+  }
+}
+
+template <class T, int K>
+__global__ __launch_bounds__(kBlock) void k_bicg_v2(int n, const KrylovState *__restrict__ st,
+                                                    const T *__restrict__ r,
+                                                    const T *__restrict__ v,
+                                                    const T *__restrict__ invd,
+                                                    T *__restrict__ sv, T *__restrict__ z) {
+  if (!st->any_active) return;
+  int act[K];
+  double al[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    act[c] = st->active[c];
+    al[c] = st->alpha[c];
+  }
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
+       e += (size_t)gridDim.x * kBlock) {
+    const double di = (double)invd[e];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      if (!act[c]) continue;
+      const size_t i = e * K + c;
+      const T si = (T)((double)r[i] - al[c] * (double)v[i]);
+      sv[i] = si;
+      z[i] = (T)(di * (double)si);
+    }
+  }
+}
+
+template <class T, int K>
+__global__ __launch_bounds__(kBlock) void k_bicg_v3(
+    int n, const KrylovState *__restrict__ st, const T *__restrict__ y,
+    const T *__restrict__ z, const T *__restrict__ sv, const T *__restrict__ t,
+    const T *__restrict__ r0, T *__restrict__ x, T *__restrict__ r,
+    double *__restrict__ partials) {
+  if (!st->any_active) return;
+  int act[K];
+  double al[K], om[K];
+  double acc[2 * K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    act[c] = st->active[c];
+    al[c] = st->alpha[c];
+    om[c] = st->omega[c];
+    acc[2 * c] = 0.0;
+    acc[2 * c + 1] = 0.0;
+  }
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
+       e += (size_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      if (!act[c]) continue;
+      const size_t i = e * K + c;
+      x[i] = (T)((double)x[i] + (al[c] * (double)y[i] + om[c] * (double)z[i]));
+      const T ri = (T)((double)sv[i] - om[c] * (double)t[i]);
+      r[i] = ri;
+      acc[2 * c] += (double)ri * (double)ri;
+      acc[2 * c + 1] += (double)r0[i] * (double)ri;
+    }
+  }
+  block_partials<2 * K>(acc, partials);
+}
+
+template <class T, int K>
+void bicg_init(hipStream_t s, int n, const T *b, T *x, T *r, T *r0, T *p, T *v,
+               double *partials, int *grid) {
+  const int G = vec_grid(n);
+  *grid = G;
+  hipLaunchKernelGGL((k_bicg_init<T, K>), dim3(G), dim3(kBlock), 0, s, n, b, x, r, r0, p,
+                     v, partials);
+  CFSA_LAUNCH_CHECK();
+}
+template <int K>
+void bicg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G) {
+  hipLaunchKernelGGL((k_bicg_s_init<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G);
+  CFSA_LAUNCH_CHECK();
+}
+template <int K>
+void bicg_s_begin(hipStream_t s, KrylovState *st, const double *partials, int G) {
+  hipLaunchKernelGGL((k_bicg_s_begin<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G);
+  CFSA_LAUNCH_CHECK();
+}
+template <int K>
+void bicg_s_alpha(hipStream_t s, KrylovState *st, const double *partials, int G) {
+  hipLaunchKernelGGL((k_bicg_s_alpha<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G);
+  CFSA_LAUNCH_CHECK();
+}
+template <int K>
+void bicg_s_omega(hipStream_t s, KrylovState *st, const double *partials, int G) {
+  hipLaunchKernelGGL((k_bicg_s_omega<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G);
+  CFSA_LAUNCH_CHECK();
+}
+template <int K>
+void bicg_s_restart(hipStream_t s, KrylovState *st, const double *partials, int G,
+                    const int *mask) {
+  hipLaunchKernelGGL((k_bicg_s_restart<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G,
+                     mask);
+  CFSA_LAUNCH_CHECK();
+}
+template <class T, int K>
+void bicg_v1(hipStream_t s, int n, const KrylovState *st, const T *r, T *p, const T *v,
+             const T *invd, T *y) {
+  hipLaunchKernelGGL((k_bicg_v1<T, K>), dim3(vec_grid(n)), dim3(kBlock), 0, s, n, st, r, p,
+                     v, invd, y);
+  CFSA_LAUNCH_CHECK();
+}
+template <class T, int K>
+void bicg_v2(hipStream_t s, int n, const KrylovState *st, const T *r, const T *v,
+             const T *invd, T *sv, T *z) {
+  hipLaunchKernelGGL((k_bicg_v2<T, K>), dim3(vec_grid(n)), dim3(kBlock), 0, s, n, st, r, v,
+                     invd, sv, z);
+  CFSA_LAUNCH_CHECK();
+}
+template <class T, int K>
+void bicg_v3(hipStream_t s, int n, const KrylovState *st, const T *y, const T *z,
+             const T *sv, const T *t, const T *r0, T *x, T *r, double *partials,
+             int *grid) {
+  const int G = vec_grid(n);
+  *grid = G;
+  hipLaunchKernelGGL((k_bicg_v3<T, K>), dim3(G), dim3(kBlock), 0, s, n, st, y, z, sv, t, r0,
+                     x, r, partials);
+  CFSA_LAUNCH_CHECK();
+}
+
+#define CFSA_INST_BICG(T, K)                                                                \
+  template void bicg_init<T, K>(hipStream_t, int, const T *, T *, T *, T *, T *, T *,        \
+                                double *, int *);                                           \
+  template void bicg_v1<T, K>(hipStream_t, int, const KrylovState *, const T *, T *,        \
+                              const T *, const T *, T *);                                   \
+  template void bicg_v2<T, K>(hipStream_t, int, const KrylovState *, const T *, const T *,  \
+                              const T *, T *, T *);                                         \
+  template void bicg_v3<T, K>(hipStream_t, int, const KrylovState *, const T *, const T *,  \
+                              const T *, const T *, const T *, T *, T *, double *, int *);
+CFSA_INST_BICG(double, 1)
+CFSA_INST_BICG(double, 3)
+CFSA_INST_BICG(float, 1)
+CFSA_INST_BICG(float, 3)
+#define CFSA_INST_BICG_S(K)                                                          \
+  template void bicg_s_init<K>(hipStream_t, KrylovState *, const double *, int);     \
+  template void bicg_s_begin<K>(hipStream_t, KrylovState *, const double *, int);    \
+  template void bicg_s_alpha<K>(hipStream_t, KrylovState *, const double *, int);    \
+  template void bicg_s_omega<K>(hipStream_t, KrylovState *, const double *, int);    \
+  template void bicg_s_restart<K>(hipStream_t, KrylovState *, const double *, int,   \
+                                  const int *);
+CFSA_INST_BICG_S(1)
+CFSA_INST_BICG_S(3)
+
+// ===========================================================================
+// Jacobi PCG (stopping rule of Eigen's conjugate_gradient)
+// ===========================================================================
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_init(int n, const T *__restrict__ b,
+                                                    const T *__restrict__ invd,
+                                                    T *__restrict__ x, T *__restrict__ r,
+                                                    T *__restrict__ p,
+                                                    double *__restrict__ partials) {
+  double acc[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n;
+       i += (size_t)gridDim.x * kBlock) {
+    const T bi = b[i];
+    x[i] = T(0);
+    r[i] = bi;
+    const T pi = (T)((double)invd[i] * (double)bi);
+    p[i] = pi;
+    acc[0] += (double)bi * (double)bi;
+    acc[1] += (double)bi * (double)pi;
+  }
+  block_partials<2>(acc, partials);
+}
+
+__global__ void k_cg_s_init(KrylovState *st, const double *partials, int G) {
+  __shared__ double tot[2];
+  reduce_final<2>(partials, G, tot);
+  if (threadIdx.x != 0) return;
+  const double bb = tot[0];
+  st->rhs_sq[0] = bb;
+  st->iters[0] = 0;
+  st->restarts[0] = 0;
+  st->active[0] = 0;
+  st->any_active = 0;
+  for (int c = 1; c < 3; ++c) {
+    st->status[c] = KS_DONE;
+    st->active[c] = 0;
+  }
+  if (bb == 0.0) {
+    st->status[0] = KS_DONE;
+    st->err[0] = 0.0;
+    return;
+  }
+  const double thr = fmax(st->tol * st->tol * bb, DBL_MIN);
+  st->tol2[0] = thr;
+  st->rr[0] = bb;
+  if (bb < thr) {
+    st->status[0] = KS_DONE;
+    st->err[0] = sqrt(bb / bb);
+    return;
+  }
+  st->rho[0] = tot[1];
+  st->status[0] = (st->maxit > 0) ? KS_RUNNING : KS_DONE;
+  st->err[0] = 1.0;
+  st->active[0] = st->maxit > 0;
+  st->any_active = st->active[0];
+}
+
+__global__ void k_cg_s_alpha(KrylovState *st, const double *partials, int G) {
+  __shared__ double tot[1];
+  reduce_final<1>(partials, G, tot);
+  if (threadIdx.x != 0 || !st->active[0]) return;
+  st->r0v[0] = tot[0];
+  st->alpha[0] = st->rho[0] / tot[0];
+}
+
+__global__ void k_cg_s_beta(KrylovState *st, const double *partials, int G) {
+  __shared__ double tot[2];
+  reduce_final<2>(partials, G, tot);
+  if (threadIdx.x != 0 || !st->active[0]) return;
+  const double rn2 = tot[0];
+  st->rr[0] = rn2;
+  if (rn2 < st->tol2[0]) {
+    st->status[0] = KS_DONE;
+    st->err[0] = sqrt(rn2 / st->rhs_sq[0]);
+    st->active[0] = 0;
+    st->any_active = 0;
+    return;
+  }
+  st->rho_old[0] = st->rho[0];
+  st->rho[0] = tot[1];
+  st->beta[0] = st->rho[0] / st->rho_old[0];
+  st->iters[0] += 1;
+  if (st->iters[0] >= st->maxit) {
+    // the direction update below still runs; the loop then ends
+    st->status[0] = KS_DONE;
+    st->err[0] = sqrt(rn2 / st->rhs_sq[0]);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__restrict__ st,
+                                                  const T *__restrict__ p,
+                                                  const T *__restrict__ q,
+                                                  const T *__restrict__ invd,
+                                                  T *__restrict__ x, T *__restrict__ r,
+                                                  T *__restrict__ z,
+                                                  double *__restrict__ partials) {
+  if (!st->active[0]) return;
+  const double al = st->alpha[0];
+  double acc[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n;
+       i += (size_t)gridDim.x * kBlock) {
+    x[i] = (T)((double)x[i] + al * (double)p[i]);
+    const T ri = (T)((double)r[i] - al * (double)q[i]);
+    r[i] = ri;
+    const T zi = (T)((double)invd[i] * (double)ri);
+    z[i] = zi;
+    acc[0] += (double)ri * (double)ri;
+    acc[1] += (double)ri * (double)zi;
+  }
+  block_partials<2>(acc, partials);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_v2(int n, KrylovState *__restrict__ st,
+                                                  const T *__restrict__ z,
+                                                  T *__restrict__ p) {
+  if (!st->active[0]) return;
+  const double be = st->beta[0];
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n;
+       i += (size_t)gridDim.x * kBlock)
+    p[i] = (T)((double)z[i] + be * (double)p[i]);
+}
+
+// Ends the iteration when the limit was reached in k_cg_s_beta.
+__global__ void k_cg_s_end(KrylovState *st) {
+  if (st->active[0] && st->status[0] == KS_DONE) {
+    st->active[0] = 0;
+    st->any_active = 0;
+  }
+}
+
+template <class T>
+void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p,
+             double *partials, int *grid) {
+  const int G = vec_grid(n);
+  *grid = G;
+  hipLaunchKernelGGL((k_cg_init<T>), dim3(G), dim3(kBlock), 0, s, n, b, invd, x, r, p,
+                     partials);
+  CFSA_LAUNCH_CHECK();
+}
+void cg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G) {
+  hipLaunchKernelGGL(k_cg_s_init, dim3(1), dim3(kBlock), 0, s, st, partials, G);
+  CFSA_LAUNCH_CHECK();
+}
+void cg_s_alpha(hipStream_t s, KrylovState *st, const double *partials, int G) {
+  hipLaunchKernelGGL(k_cg_s_alpha, dim3(1), dim3(kBlock), 0, s, st, partials, G);
+  CFSA_LAUNCH_CHECK();
+}
+void cg_s_beta(hipStream_t s, KrylovState *st, const double *partials, int G) {
+  hipLaunchKernelGGL(k_cg_s_beta, dim3(1), dim3(kBlock), 0, s, st, partials, G);
+  CFSA_LAUNCH_CHECK();
+}
+template <class T>
+void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
+           const T *invd, T *x, T *r, T *z, double *partials, int *grid) {
+  const int G = vec_grid(n);
+  *grid = G;
+  hipLaunchKernelGGL((k_cg_v1<T>), dim3(G), dim3(kBlock), 0, s, n, st, p, q, invd, x, r, z,
+                     partials);
+  CFSA_LAUNCH_CHECK();
+}
+template <class T>
+void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *z, T *p) {
+  hipLaunchKernelGGL((k_cg_v2<T>), dim3(vec_grid(n)), dim3(kBlock), 0, s, n,
+                     const_cast<KrylovState *>(st), z, p);
+  CFSA_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_cg_s_end, dim3(1), dim3(1), 0, s, const_cast<KrylovState *>(st));
+  CFSA_LAUNCH_CHECK();
+}
+#define CFSA_INST_CG(T)                                                                   \
+  template void cg_init<T>(hipStream_t, int, const T *, const T *, T *, T *, T *, double *, \
+                           int *);                                                        \
+  template void cg_v1<T>(hipStream_t, int, const KrylovState *, const T *, const T *,     \
+                         const T *, T *, T *, T *, double *, int *);                      \
+  template void cg_v2<T>(hipStream_t, int, const KrylovState *, const T *, T *);
+CFSA_INST_CG(double)
+CFSA_INST_CG(float)
+
+// ===========================================================================
+// Element assembly into the fixed patterns (graph-coloured, atomic-free)
+// ===========================================================================
+__device__ __forceinline__ int find_slot(const int *__restrict__ rowptr,
+                                         const int *__restrict__ col, int row, int c) {
+  int lo = rowptr[row], hi = rowptr[row + 1] - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (col[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ double dot3(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+constexpr int kEdgeV[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+
+// One thread per (tet, local node a): it owns global row node[a] for this tet.
+// Tets of one colour share no vertex, so no two threads touch the same row.
+// reference: buildMassMatrix (NavierStokes.cpp:227-310), buildStiffnessMatrix
+// (:135-225), buildGradientMatrices (:70-133), buildPoissonMatrix (:316-356).
+__global__ __launch_bounds__(kBlock) void k_assemble(
+    int ntc, const int *__restrict__ order, const int *__restrict__ tets,
+    const int *__restrict__ ten, int nv, const double *__restrict__ vol,
+    const double *__restrict__ grad, double rho, double inv_mu,
+    const int *__restrict__ Arp, const int *__restrict__ Acol, double *__restrict__ M,
+    double *__restrict__ K, const int *__restrict__ Prp, const int *__restrict__ Pcol,
+    double *__restrict__ L, double *__restrict__ G3) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  const int tl = idx / 10, a = idx % 10;
+  if (tl >= ntc) return;
+  const int t = order[tl];
+  int node[10];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) node[i] = tets[(size_t)t * 4 + i];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) node[4 + e] = nv + ten[(size_t)t * 6 + e];
+  double g[10][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[i][c] = grad[(size_t)t * 12 + i * 3 + c];
+#pragma unroll
+  for (int e = 0; e < 6; ++e)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[4 + e][c] = g[kEdgeV[e][0]][c] + g[kEdgeV[e][1]][c];
+  const double v = vol[t];
+  const double mass = rho * v;
+  const double muv = inv_mu * v;
+  const int row = node[a];
+  for (int b = 0; b < 10; ++b) {
+    double m;
+    if (a < 4 && b < 4)
+      m = (a == b) ? mass / 10.0 : mass / 20.0;
+    else if (a >= 4 && b >= 4)
+      m = (a == b) ? mass / 15.0 : mass / 60.0;
+    else
+      m = mass / 30.0;
+    // vertex-edge pairs use grad_edge . grad_vertex in the reference
+    const double kv = (a >= 4 && b < 4) || (a < 4 && b >= 4)
+                          ? muv * dot3(g[a >= 4 ? a : b], g[a >= 4 ? b : a])
+                          : muv * dot3(g[a], g[b]);
+    const int slot = find_slot(Arp, Acol, row, node[b]);
+    M[slot] += m;
+    K[slot] += kv;
+  }
+  if (a < 4) {
+    const double contrib = v / 4.0;
+    for (int b = 0; b < 4; ++b) {
+      const int slot = find_slot(Prp, Pcol, row, node[b]);
+      L[slot] += v * dot3(g[a], g[b]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) G3[(size_t)slot * 3 + c] += contrib * g[b][c];
+    }
+  }
+}
+
+void assemble_colour(hipStream_t s, int ntc, const int *order, const int *tets,
+                     const int *ten, int nv, const double *vol, const double *grad,
+                     double rho, double inv_mu, const DevCsr &A, double *M, double *K,
+                     const DevCsr &P, double *L, double *G3) {
+  if (ntc <= 0) return;
+  hipLaunchKernelGGL(k_assemble, dim3(div_up((int64_t)ntc * 10, kBlock)), dim3(kBlock), 0,
+                     s, ntc, order, tets, ten, nv, vol, grad, rho, inv_mu, A.rowptr, A.col,
+                     M, K, P.rowptr, P.col, L, G3);
+  CFSA_LAUNCH_CHECK();
+}
+
+// inv_lumped_mass_ = 1 / (M * ones), reference NavierStokes.cpp:290-306
+__global__ void k_lumped(int n, const int *__restrict__ rp, const double *__restrict__ M,
+                         double *__restrict__ ilm) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = rp[i]; k < rp[i + 1]; ++k) s += M[k] * 1.0;
+  ilm[i] = (fabs(s) > 1e-14) ? 1.0 / s : 1.0;
+}
+void lumped_mass_inverse(hipStream_t s, const DevCsr &A, const double *M, double *ilm) {
+  hipLaunchKernelGGL(k_lumped, dim3(div_up(A.n, kBlock)), dim3(kBlock), 0, s, A.n, A.rowptr,
+                     M, ilm);
+  CFSA_LAUNCH_CHECK();
+}
+
+// lhs = M/dt + K (reference NavierStokes.cpp:460)
+__global__ void k_lhs(int64_t nnz, const double *__restrict__ M,
+                      const double *__restrict__ K, double dt, double *__restrict__ out) {
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * kBlock)
+    out[k] = M[k] / dt + K[k];
+}
+void build_lhs(hipStream_t s, int64_t nnz, const double *M, const double *K, double dt,
+               double *lhs) {
+  hipLaunchKernelGGL(k_lhs, dim3(4096), dim3(kBlock), 0, s, nnz, M, K, dt, lhs);
+  CFSA_LAUNCH_CHECK();
+}
+
+// mode 0: Inlet/Wall rows (momentum); 1: Outlet rows (Dirichlet pressure);
+// 2: only row `ref` (Neumann pressure reference node)
+__global__ void k_diag_penalty(int n, const int *__restrict__ diag,
+                               const uint8_t *__restrict__ bc, int mode, int ref,
+                               double pen, double *__restrict__ A) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  bool on;
+  if (mode == 0)
+    on = bc[i] == kInlet || bc[i] == kWall;
+  else if (mode == 1)
+    on = bc[i] == kOutlet;
+  else
+    on = (i == ref);
+  if (on) A[diag[i]] += pen;
+}
+void add_diag_penalty(hipStream_t s, int n, const int *diag, const uint8_t *bc, int mode,
+                      int ref, double penalty, double *A) {
+  hipLaunchKernelGGL(k_diag_penalty, dim3(div_up(n, kBlock)), dim3(kBlock), 0, s, n, diag,
+                     bc, mode, ref, penalty, A);
+  CFSA_LAUNCH_CHECK();
+}
+
+// Eigen DiagonalPreconditioner: 1/A_jj when the entry exists and is nonzero.
+__global__ void k_invdiag(int n, const int *__restrict__ diag, const double *__restrict__ A,
+                          double *__restrict__ invd) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int d = diag[i];
+  invd[i] = (d >= 0 && A[d] != 0.0) ? 1.0 / A[d] : 1.0;
+}
+void inv_diag(hipStream_t s, int n, const int *diag, const double *A, double *invd) {
+  hipLaunchKernelGGL(k_invdiag, dim3(div_up(n, kBlock)), dim3(kBlock), 0, s, n, diag, A,
+                     invd);
+  CFSA_LAUNCH_CHECK();
+}
+
+// ===========================================================================
+// Convection / divergence: per-tet value, then a node gather over the
+// node->tet incidence in ascending tet order (the reference's scatter order).
+// ===========================================================================
+// grad_u(a,b) = sum_j u_j(a) * gphi_j(b), accumulated j = 0..3 from zero.
+__device__ __forceinline__ void tet_grad_u(const double (&u)[4][3], const double (&g)[4][3],
+                                           double (&G)[3][3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) G[a][b] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) G[a][b] = G[a][b] + u[j][a] * g[j][b];
+}
+
+// reference: computeAdvectionRHS per-tet part (NavierStokes.cpp:550-567)
+__global__ __launch_bounds__(kBlock) void k_conv_tet(int nt, const int *__restrict__ tets,
+                                                     const double *__restrict__ grad,
+                                                     const double *__restrict__ uvel,
+                                                     double *__restrict__ cT) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nt) return;
+  double u[4][3], g[4][3];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int v = tets[(size_t)t * 4 + j];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      u[j][c] = uvel[(size_t)v * 3 + c];
+      g[j][c] = grad[(size_t)t * 12 + j * 3 + c];
+    }
+  }
+  double ue[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ue[c] = ue[c] + 0.25 * u[j][c];
+  double G[3][3];
+  tet_grad_u(u, g, G);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    cT[(size_t)t * 3 + a] = (G[a][0] * ue[0] + G[a][1] * ue[1]) + G[a][2] * ue[2];
+}
+void conv_tet(hipStream_t s, int nt, const int *tets, const double *grad, const double *u,
+              double *cT) {
+  hipLaunchKernelGGL(k_conv_tet, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, s, nt, tets,
+                     grad, u, cT);
+  CFSA_LAUNCH_CHECK();
+}
+
+// conv[node] -= (vol/4) c_T for vertices, (vol/24) c_T for edge nodes
+// (NavierStokes.cpp:571-580)
+__global__ __launch_bounds__(kBlock) void k_conv_gather(int nu, int nv,
+                                                        const int *__restrict__ ip,
+                                                        const int *__restrict__ it,
+                                                        const double *__restrict__ vol,
+                                                        const double *__restrict__ cT,
+                                                        double *__restrict__ conv) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nu) return;
+  const double div = (i < nv) ? 4.0 : 24.0;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int q = ip[i]; q < ip[i + 1]; ++q) {
+    const int t = it[q];
+    const double w = vol[t] / div;
+    a0 = a0 - w * cT[(size_t)t * 3 + 0];
+    a1 = a1 - w * cT[(size_t)t * 3 + 1];
+    a2 = a2 - w * cT[(size_t)t * 3 + 2];
+  }
+  conv[(size_t)i * 3 + 0] = a0;
+  conv[(size_t)i * 3 + 1] = a1;
+  conv[(size_t)i * 3 + 2] = a2;
+}
+void conv_gather(hipStream_t s, int nu, int nv, const int *inc_ptr, const int *inc_tet,
+                 const double *vol, const double *cT, double *conv) {
+  hipLaunchKernelGGL(k_conv_gather, dim3(div_up(nu, kBlock)), dim3(kBlock), 0, s, nu, nv,
+                     inc_ptr, inc_tet, vol, cT, conv);
+  CFSA_LAUNCH_CHECK();
+}
+
+// reference: computeDivergence per-tet part (NavierStokes.cpp:684-697)
+__global__ __launch_bounds__(kBlock) void k_div_tet(int nt, const int *__restrict__ tets,
+                                                    const double *__restrict__ grad,
+                                                    const double *__restrict__ vol,
+                                                    const double *__restrict__ uvel,
+                                                    double *__restrict__ wT) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nt) return;
+  double u[4][3], g[4][3];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int v = tets[(size_t)t * 4 + j];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      u[j][c] = uvel[(size_t)v * 3 + c];
+      g[j][c] = grad[(size_t)t * 12 + j * 3 + c];
+    }
+  }
+  double G[3][3];
+  tet_grad_u(u, g, G);
+  const double d = (G[0][0] + G[1][1]) + G[2][2];
+  wT[t] = (vol[t] / 4.0) * d;
+}
+void div_tet(hipStream_t s, int nt, const int *tets, const double *grad, const double *vol,
+             const double *u, double *wT) {
+  hipLaunchKernelGGL(k_div_tet, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, s, nt, tets, grad,
+                     vol, u, wT);
+  CFSA_LAUNCH_CHECK();
+}
+
+// pressure rhs, reference NavierStokes.cpp:598-626
+__global__ __launch_bounds__(kBlock) void k_div_gather(
+    int nv, const int *__restrict__ ip, const int *__restrict__ it,
+    const double *__restrict__ wT, const uint8_t *__restrict__ bc, int mode, int ref,
+    double dt, double p_out, double pen, double *__restrict__ div,
+    double *__restrict__ rhs) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nv) return;
+  double a = 0.0;
+  for (int q = ip[i]; q < ip[i + 1]; ++q) a = a + wT[it[q]];
+  div[i] = a;
+  double r = -a / dt;
+  if (mode == 1 && bc[i] == kOutlet) r = r + pen * p_out;
+  if (mode == 2 && i == ref) r = r + pen * 0.0;
+  rhs[i] = r;
+}
+void div_gather_prhs(hipStream_t s, int nv, const int *inc_ptr, const int *inc_tet,
+                     const double *wT, const uint8_t *bc, int outlet_mode, int ref,
+                     double dt, double p_out, double penalty, double *div,
+                     double *rhs_p) {
+  hipLaunchKernelGGL(k_div_gather, dim3(div_up(nv, kBlock)), dim3(kBlock), 0, s, nv,
+                     inc_ptr, inc_tet, wT, bc, outlet_mode, ref, dt, p_out, penalty, div,
+                     rhs_p);
+  CFSA_LAUNCH_CHECK();
+}
+
+// p += relax_p p'; then Dirichlet outlet reset (NavierStokes.cpp:657-662,1011-1024)
+__global__ void k_p_update(int nv, double *__restrict__ p, const double *__restrict__ pc,
+                           double relax, const uint8_t *__restrict__ bc, int dirichlet,
+                           double p_out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nv) return;
+  double v = p[i] + relax * pc[i];
+  if (dirichlet && bc[i] == kOutlet) v = p_out;
+  p[i] = v;
+}
+void pressure_update(hipStream_t s, int nv, double *p, const double *pc, double relax_p,
+                     const uint8_t *bc, int dirichlet, double p_out) {
+  hipLaunchKernelGGL(k_p_update, dim3(div_up(nv, kBlock)), dim3(kBlock), 0, s, nv, p, pc,
+                     relax_p, bc, dirichlet, p_out);
+  CFSA_LAUNCH_CHECK();
+}
+
+__global__ void k_p_reset(int nv, double *__restrict__ p, const uint8_t *__restrict__ bc,
+                          double p_out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < nv && bc[i] == kOutlet) p[i] = p_out;
+}
+void pressure_outlet_reset(hipStream_t s, int nv, double *p, const uint8_t *bc, double p_out) {
+  hipLaunchKernelGGL(k_p_reset, dim3(div_up(nv, kBlock)), dim3(kBlock), 0, s, nv, p, bc, p_out);
+  CFSA_LAUNCH_CHECK();
+}
+
+__global__ void k_copy1(const double *src, double *dst) { *dst = *src; }
+__global__ void k_sub_scalar(int n, double *__restrict__ p, const double *__restrict__ s) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  p[i] = p[i] - *s;
+}
+// normalizePressire (NavierStokes.cpp:804-813)
+void pressure_normalize(hipStream_t s, int nv, double *p, int ref, double *scratch) {
+  if (ref < 0 || ref >= nv) return;
+  hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, s, p + ref, scratch);
+  CFSA_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sub_scalar, dim3(div_up(nv, kBlock)), dim3(kBlock), 0, s, nv, p,
+                     scratch);
+  CFSA_LAUNCH_CHECK();
+}
+
+// correctVelocity (NavierStokes.cpp:725-768)
+__global__ __launch_bounds__(kBlock) void k_correct(
+    int nu, int nv, const uint8_t *__restrict__ bc, const double *__restrict__ g,
+    const double *__restrict__ ilm, const double *__restrict__ gp,
+    const double *__restrict__ us, double dt, double relax, double *__restrict__ u) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nu) return;
+  const uint8_t b = bc[i];
+  const size_t o = (size_t)i * 3;
+  if (b == kInlet) {
+    u[o] = g[o];
+    u[o + 1] = g[o + 1];
+    u[o + 2] = g[o + 2];
+  } else if (b == kWall) {
+    u[o] = 0.0;
+    u[o + 1] = 0.0;
+    u[o + 2] = 0.0;
+  } else if (i < nv && ilm[i] > 1e-14) {
+    const double sc = -dt * ilm[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[o + c] = us[o + c] + relax * (sc * gp[o + c]);
+  } else {
+    u[o] = us[o];
+    u[o + 1] = us[o + 1];
+    u[o + 2] = us[o + 2];
+  }
+}
+void correct_velocity(hipStream_t s, int nu, int nv, const uint8_t *bc, const double *ginlet,
+                      const double *ilm, const double *gp, const double *us, double dt,
+                      double relax_u, double *u) {
+  hipLaunchKernelGGL(k_correct, dim3(div_up(nu, kBlock)), dim3(kBlock), 0, s, nu, nv, bc,
+                     ginlet, ilm, gp, us, dt, relax_u, u);
+  CFSA_LAUNCH_CHECK();
+}
+
+// reenforceVelocityBCs (NavierStokes.cpp:815-839)
+__global__ void k_bc_reset(int nu, const uint8_t *__restrict__ bc,
+                           const double *__restrict__ g, double *__restrict__ u,
+                           int inlet_only) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nu) return;
+  const uint8_t b = bc[i];
+  const size_t o = (size_t)i * 3;
+  if (b == kInlet) {
+    u[o] = g[o];
+    u[o + 1] = g[o + 1];
+    u[o + 2] = g[o + 2];
+  } else if (b == kWall && !inlet_only) {
+    u[o] = 0.0;
+    u[o + 1] = 0.0;
+    u[o + 2] = 0.0;
+  }
+}
+void reset_velocity_bc(hipStream_t s, int nu, const uint8_t *bc, const double *ginlet,
+                       double *u) {
+  hipLaunchKernelGGL(k_bc_reset, dim3(div_up(nu, kBlock)), dim3(kBlock), 0, s, nu, bc,
+                     ginlet, u, 0);
+  CFSA_LAUNCH_CHECK();
+}
+// initialize(): inlet nodes take the inlet value (NavierStokes.cpp:33-46)
+void set_inlet_nodes(hipStream_t s, int nu, const uint8_t *bc, const double *ginlet,
+                     double *u) {
+  hipLaunchKernelGGL(k_bc_reset, dim3(div_up(nu, kBlock)), dim3(kBlock), 0, s, nu, bc,
+                     ginlet, u, 1);
+  CFSA_LAUNCH_CHECK();
+}
+
+__global__ void k_fill3(int n, double a, double b, double c, double *__restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  out[(size_t)i * 3] = a;
+  out[(size_t)i * 3 + 1] = b;
+  out[(size_t)i * 3 + 2] = c;
+}
+void fill3(hipStream_t s, int n, const double v[3], double *out) {
+  hipLaunchKernelGGL(k_fill3, dim3(div_up(n, kBlock)), dim3(kBlock), 0, s, n, v[0], v[1],
+                     v[2], out);
+  CFSA_LAUNCH_CHECK();
+}
+
+__global__ void k_nan(int n, const double *__restrict__ a, int *__restrict__ flag) {
+  int f = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    f |= isnan(a[i]) ? 1 : 0;
+  if (f) *flag = 1;
+}
+void nan_check(hipStream_t s, int n, const double *a, int *flag) {
+  hipLaunchKernelGGL(k_nan, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, a, flag);
+  CFSA_LAUNCH_CHECK();
+}
+
+template <class TO, class TI>
+__global__ void k_convert(int64_t n, const TI *__restrict__ in, TO *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    out[i] = (TO)in[i];
+}
+template <class TO, class TI>
+void convert(hipStream_t s, int64_t n, const TI *in, TO *out) {
+  hipLaunchKernelGGL((k_convert<TO, TI>), dim3(4096), dim3(kBlock), 0, s, n, in, out);
+  CFSA_LAUNCH_CHECK();
+}
+template void convert<float, double>(hipStream_t, int64_t, const double *, float *);
+template void convert<double, float>(hipStream_t, int64_t, const float *, double *);
+
+}  // namespace cfsa

@@ -1,0 +1,133 @@
+// Host-side launchers for the HIP kernels in kernels.hip.  All launches are
+// asynchronous on the given stream.
+#pragma once
+
+#include "dev.hpp"
+
+namespace cfsa {
+
+struct DevCsr {
+  int n = 0;
+  int64_t nnz = 0;
+  const int *rowptr = nullptr;
+  const int *col = nullptr;
+};
+
+// ---- CSR-stream SpMV family (3 interleaved right-hand sides unless noted) ----
+template <class T>
+void spmv3(hipStream_t s, const DevCsr &A, const T *val, const T *x, T *y,
+           const int *active);
+// v = A y and per-rhs dot r0.v -> partials[3][G]
+template <class T>
+void spmv3_dot(hipStream_t s, const DevCsr &A, const T *val, const T *y, T *v,
+               const T *r0, const int *active, double *partials, int *grid);
+// t = A z and per-rhs dots t.t, t.s -> partials[3*2][G]
+template <class T>
+void spmv3_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
+                const T *sv, const int *active, double *partials, int *grid);
+// r = b - A x (rhs of the active mask only), r0 = r, partials r.r [3][G]
+template <class T>
+void spmv3_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
+                    const T *b, T *r, T *r0, const int *active, double *partials,
+                    int *grid);
+// single right-hand side variants
+template <class T>
+void spmv1(hipStream_t s, const DevCsr &A, const T *val, const T *x, T *y);
+template <class T>
+void spmv1_dot(hipStream_t s, const DevCsr &A, const T *val, const T *y, T *v,
+               const T *r0, const int *active, double *partials, int *grid);
+template <class T>
+void spmv1_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
+                const T *sv, const int *active, double *partials, int *grid);
+template <class T>
+void spmv1_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
+                    const T *b, T *r, T *r0, const int *active, double *partials,
+                    int *grid);
+// y[i][c] = sum_j G_c[i,j] x[j]   (G values interleaved [nnz][3])
+void spmv_grad3(hipStream_t s, const DevCsr &P, const double *gval, const double *x,
+                double *y);
+// momentum right-hand side, reference NavierStokes.cpp:430-495:
+// rhs = (M u)/dt - [G p] (P1 rows) + conv + penalty * u_bc (Inlet/Wall rows)
+void momentum_rhs(hipStream_t s, const DevCsr &A, const double *M, const double *u,
+                  const double *gp, const double *conv, const uint8_t *bc,
+                  const double *ginlet, double dt, int nv, double penalty,
+                  double *rhs);
+
+// ---- Krylov vector / scalar kernels ----
+int vec_grid(int n);
+template <class T, int K>
+void bicg_init(hipStream_t s, int n, const T *b, T *x, T *r, T *r0, T *p, T *v,
+               double *partials, int *grid);
+template <int K>
+void bicg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G);
+template <int K>
+void bicg_s_begin(hipStream_t s, KrylovState *st, const double *partials, int G);
+template <int K>
+void bicg_s_alpha(hipStream_t s, KrylovState *st, const double *partials, int G);
+template <int K>
+void bicg_s_omega(hipStream_t s, KrylovState *st, const double *partials, int G);
+template <int K>
+void bicg_s_restart(hipStream_t s, KrylovState *st, const double *partials, int G,
+                    const int *mask);
+template <class T, int K>
+void bicg_v1(hipStream_t s, int n, const KrylovState *st, const T *r, T *p,
+             const T *v, const T *invd, T *y);
+template <class T, int K>
+void bicg_v2(hipStream_t s, int n, const KrylovState *st, const T *r, const T *v,
+             const T *invd, T *sv, T *z);
+template <class T, int K>
+void bicg_v3(hipStream_t s, int n, const KrylovState *st, const T *y, const T *z,
+             const T *sv, const T *t, const T *r0, T *x, T *r, double *partials,
+             int *grid);
+// Jacobi PCG (single right-hand side)
+template <class T>
+void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p,
+             double *partials, int *grid);
+void cg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G);
+void cg_s_alpha(hipStream_t s, KrylovState *st, const double *partials, int G);
+void cg_s_beta(hipStream_t s, KrylovState *st, const double *partials, int G);
+template <class T>
+void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
+           const T *invd, T *x, T *r, T *z, double *partials, int *grid);
+template <class T>
+void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *z, T *p);
+
+// ---- FEM element / node kernels ----
+void assemble_colour(hipStream_t s, int ntc, const int *order, const int *tets,
+                     const int *ten, int nv, const double *vol, const double *grad,
+                     double rho, double inv_mu, const DevCsr &A, double *M, double *K,
+                     const DevCsr &P, double *L, double *G3);
+void lumped_mass_inverse(hipStream_t s, const DevCsr &A, const double *M, double *ilm);
+void build_lhs(hipStream_t s, int64_t nnz, const double *M, const double *K, double dt,
+               double *lhs);
+void add_diag_penalty(hipStream_t s, int n, const int *diag, const uint8_t *bc,
+                      int mode, int ref, double penalty, double *A);
+void inv_diag(hipStream_t s, int n, const int *diag, const double *A, double *invd);
+void conv_tet(hipStream_t s, int nt, const int *tets, const double *grad,
+              const double *u, double *cT);
+void conv_gather(hipStream_t s, int nu, int nv, const int *inc_ptr, const int *inc_tet,
+                 const double *vol, const double *cT, double *conv);
+void div_tet(hipStream_t s, int nt, const int *tets, const double *grad,
+             const double *vol, const double *u, double *wT);
+// div[i] = sum_T w_T;  rhs_p = -div/dt (+ penalty * p_out on Dirichlet rows)
+void div_gather_prhs(hipStream_t s, int nv, const int *inc_ptr, const int *inc_tet,
+                     const double *wT, const uint8_t *bc, int outlet_mode, int ref,
+                     double dt, double p_out, double penalty, double *div,
+                     double *rhs_p);
+void pressure_update(hipStream_t s, int nv, double *p, const double *pc, double relax_p,
+                     const uint8_t *bc, int dirichlet, double p_out);
+void pressure_outlet_reset(hipStream_t s, int nv, double *p, const uint8_t *bc, double p_out);
+void pressure_normalize(hipStream_t s, int nv, double *p, int ref, double *scratch);
+void correct_velocity(hipStream_t s, int nu, int nv, const uint8_t *bc,
+                      const double *ginlet, const double *ilm, const double *gp,
+                      const double *us, double dt, double relax_u, double *u);
+void reset_velocity_bc(hipStream_t s, int nu, const uint8_t *bc, const double *ginlet,
+                       double *u);
+void set_inlet_nodes(hipStream_t s, int nu, const uint8_t *bc, const double *ginlet,
+                     double *u);
+void fill3(hipStream_t s, int n, const double v[3], double *out);
+void nan_check(hipStream_t s, int n, const double *a, int *flag);
+template <class TO, class TI>
+void convert(hipStream_t s, int64_t n, const TI *in, TO *out);
+
+}  // namespace cfsa

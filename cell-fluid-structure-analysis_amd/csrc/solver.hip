@@ -1,0 +1,852 @@
+// PISO orchestration on the GPU.  Every device computation of the step runs
+// here through hand-written HIP kernels; the host only sequences launches and
+// reads back Krylov convergence flags every few iterations.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <iostream>
+
+#include "mesh.hpp"
+#include "pattern.hpp"
+
+namespace cfsa {
+
+namespace {
+
+__global__ void k_set_params(KrylovState *st, double tol, int maxit, int nrhs) {
+  st->tol = tol;
+  st->maxit = maxit;
+  st->nrhs = nrhs;
+}
+__global__ void k_set_mask(int *m, int a, int b, int c) {
+  m[0] = a;
+  m[1] = b;
+  m[2] = c;
+}
+// r32 = masked ? float(r64) : 0, interleaved [n][K]
+__global__ void k_demote_masked(int n, int K, const double *__restrict__ in,
+                                const int *__restrict__ mask, float *__restrict__ out) {
+  const int64_t N = (int64_t)n * K;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * kBlock)
+    out[i] = mask[i % K] ? (float)in[i] : 0.0f;
+}
+__global__ void k_accumulate(int64_t N, const float *__restrict__ d, double *__restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * kBlock)
+    x[i] = x[i] + (double)d[i];
+}
+template <int NV>
+__global__ void k_reduce_out(const double *__restrict__ partials, int G, double *out) {
+  __shared__ double red[NV][kBlock];
+  const int tid = threadIdx.x;
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (int g = tid; g < G; g += kBlock) s += partials[(size_t)i * G + g];
+    red[i][tid] = s;
+  }
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (tid < s)
+      for (int i = 0; i < NV; ++i) red[i][tid] += red[i][tid + s];
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int i = 0; i < NV; ++i) out[i] = red[i][0];
+}
+
+constexpr double kMomentumPenalty = 1e10;  // NavierStokes.cpp:463
+constexpr double kPressurePenalty = 1e12;  // NavierStokes.cpp:611,619
+
+}  // namespace
+
+NSSolver::NSSolver() {
+  CFSA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  CFSA_HIP(hipHostMalloc((void **)&h_state, sizeof(KrylovState), hipHostMallocDefault));
+  std::memset(h_state, 0, sizeof(KrylovState));
+  d_state.alloc(1);
+  d_part_a.alloc((size_t)6 * kMaxPartials);
+  d_part_b.alloc((size_t)6 * kMaxPartials);
+  d_part_c.alloc((size_t)6 * kMaxPartials);
+  d_part_r.alloc((size_t)6 * kMaxPartials);
+  d_mask.alloc(3);
+  d_flag.alloc(1);
+  d_scalar.alloc(8);
+}
+
+NSSolver::~NSSolver() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto e : ev_pool_) (void)hipEventDestroy(e);
+  if (h_state) (void)hipHostFree(h_state);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// ---------------------------------------------------------------------------
+// Setup: reference NavierStokesSolver::initialize (NavierStokes.cpp:10-68)
+// ---------------------------------------------------------------------------
+void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
+  if (m.nv <= 0 || m.nt <= 0 || m.ne < 0 || !m.coords || !m.tets || !m.ten || !m.p1_bc ||
+      (m.ne > 0 && !m.p2_bc))
+    throw std::invalid_argument("initialize: incomplete mesh arrays");
+  if (m.nv + m.ne >= INT32_MAX || m.nt * 10 >= INT32_MAX)
+    throw std::invalid_argument("initialize: mesh exceeds int32 indexing");
+  nv = (int)m.nv;
+  nt = (int)m.nt;
+  ne = (int)m.ne;
+  nu = nv + ne;
+  density = rho;
+  viscosity = mu;
+  h_coords.assign(m.coords, m.coords + (size_t)nv * 3);
+  h_tets.assign(m.tets, m.tets + (size_t)nt * 4);
+  h_ten.assign(m.ten, m.ten + (size_t)nt * 6);
+  for (size_t i = 0; i < h_tets.size(); ++i)
+    if (h_tets[i] < 0 || h_tets[i] >= nv) throw std::invalid_argument("tet vertex out of range");
+  for (size_t i = 0; i < h_ten.size(); ++i)
+    if (h_ten[i] < 0 || h_ten[i] >= ne) throw std::invalid_argument("edge node out of range");
+  h_bc.resize((size_t)nu);
+  std::memcpy(h_bc.data(), m.p1_bc, (size_t)nv);
+  if (ne) std::memcpy(h_bc.data() + nv, m.p2_bc, (size_t)ne);
+
+  std::vector<double> vol((size_t)nt), grad((size_t)nt * 12);
+  Mesh::tetGeometry(h_coords.data(), h_tets.data(), nt, vol.data(), grad.data());
+
+  std::vector<int32_t> elem10((size_t)nt * 10);
+  for (int t = 0; t < nt; ++t) {
+    for (int a = 0; a < 4; ++a) elem10[(size_t)t * 10 + a] = h_tets[(size_t)t * 4 + a];
+    for (int e = 0; e < 6; ++e) elem10[(size_t)t * 10 + 4 + e] = nv + h_ten[(size_t)t * 6 + e];
+  }
+  Incidence inc = build_incidence(elem10, 10, nu);
+  CsrPattern PA = build_pattern(elem10, 10, inc.ptr.data(), inc.tet.data(), nu);
+  std::vector<int32_t>().swap(elem10);
+  CsrPattern PP = build_pattern(h_tets, 4, inc.ptr.data(), inc.tet.data(), nv);
+  for (int i = 0; i < nu; ++i)
+    if (PA.diag[(size_t)i] < 0) throw std::runtime_error("velocity node without element");
+  Colouring C = colour_tets(h_tets.data(), nt, nv);
+  ncolours = C.ncolours();
+  h_colour_offsets = C.offsets;
+  h_order = C.order;
+
+  d_tets.upload(h_tets.data(), h_tets.size());
+  d_ten.upload(h_ten.data(), h_ten.size());
+  d_vol.upload(vol.data(), vol.size());
+  d_grad.upload(grad.data(), grad.size());
+  d_bc.upload(h_bc.data(), h_bc.size());
+  d_order.upload(C.order.data(), C.order.size());
+  d_incp.upload(inc.ptr.data(), inc.ptr.size());
+  d_inct.upload(inc.tet.data(), inc.tet.size());
+  d_Arp.upload(PA.rowptr.data(), PA.rowptr.size());
+  d_Acol.upload(PA.col.data(), PA.col.size());
+  d_Adiag.upload(PA.diag.data(), PA.diag.size());
+  d_Prp.upload(PP.rowptr.data(), PP.rowptr.size());
+  d_Pcol.upload(PP.col.data(), PP.col.size());
+  d_Pdiag.upload(PP.diag.data(), PP.diag.size());
+  nnzA = PA.nnz();
+  nnzP = PP.nnz();
+  A_ = DevCsr{nu, nnzA, d_Arp.p, d_Acol.p};
+  P_ = DevCsr{nv, nnzP, d_Prp.p, d_Pcol.p};
+
+  d_M.alloc((size_t)nnzA);
+  d_K.alloc((size_t)nnzA);
+  d_lhs.alloc((size_t)nnzA);
+  d_L.alloc((size_t)nnzP);
+  d_Lbc.alloc((size_t)nnzP);
+  d_G3.alloc((size_t)nnzP * 3);
+  d_ilm.alloc((size_t)nu);
+  d_invdA.alloc((size_t)nu);
+  d_invdL.alloc((size_t)nv);
+  d_u.alloc((size_t)nu * 3);
+  d_us.alloc((size_t)nu * 3);
+  d_p.alloc((size_t)nv);
+  d_pc.alloc((size_t)nv);
+  d_ginlet.alloc((size_t)nu * 3);
+  d_conv.alloc((size_t)nu * 3);
+  d_cT.alloc((size_t)nt * 3);
+  d_wT.alloc((size_t)nt);
+  d_div.alloc((size_t)nv);
+  d_rhs.alloc((size_t)nu * 3);
+  d_rhsp.alloc((size_t)nv);
+  d_gp.alloc((size_t)nv * 3);
+  d_xref.alloc((size_t)nu * 3);
+  w3_.alloc((size_t)nu * 3);
+  w1_.alloc((size_t)nv);
+  d_r64.alloc((size_t)nu * 3);
+
+  assemble();
+  d_u.zero(stream_);
+  d_us.zero(stream_);
+  d_p.zero(stream_);
+  d_pc.zero(stream_);
+  upload_inlet();
+  set_inlet_nodes(stream_, nu, d_bc.p, d_ginlet.p, d_u.p);
+  // Neumann outlets pin the first Internal vertex (NavierStokes.cpp:51-67)
+  if (reference_node < 0 && outlet_type == 1) {
+    for (int i = 0; i < nv; ++i)
+      if (h_bc[(size_t)i] == kInternal) {
+        reference_node = i;
+        break;
+      }
+    if (reference_node < 0 && nv > 0) {
+      reference_node = 0;
+      std::cout << "[NS] Warning: No internal node found, using vertex 0 as pressure reference"
+                << std::endl;
+    }
+  }
+  lhs_dirty = lbc_dirty = true;
+  initialized = true;
+  CFSA_HIP(hipStreamSynchronize(stream_));
+}
+
+void NSSolver::update_mesh(const double *coords, const uint8_t *p1_bc, const uint8_t *p2_bc) {
+  if (!initialized) throw std::runtime_error("update_mesh before initialize");
+  if (coords) {
+    h_coords.assign(coords, coords + (size_t)nv * 3);
+    std::vector<double> vol((size_t)nt), grad((size_t)nt * 12);
+    Mesh::tetGeometry(h_coords.data(), h_tets.data(), nt, vol.data(), grad.data());
+    d_vol.upload(vol.data(), vol.size());
+    d_grad.upload(grad.data(), grad.size());
+  }
+  if (p1_bc) std::memcpy(h_bc.data(), p1_bc, (size_t)nv);
+  if (p2_bc && ne) std::memcpy(h_bc.data() + nv, p2_bc, (size_t)ne);
+  if (p1_bc || p2_bc) d_bc.upload(h_bc.data(), h_bc.size());
+  lhs_dirty = lbc_dirty = true;
+}
+
+// Coloured, atomic-free assembly of M, K (P2 clique pattern) and L, Gx/y/z
+// (P1 pattern); reference build order G, M, L, K (NavierStokes.hpp:97-102).
+void NSSolver::assemble() {
+  d_M.zero(stream_);
+  d_K.zero(stream_);
+  d_L.zero(stream_);
+  d_G3.zero(stream_);
+  const double inv_mu = 1.0 / viscosity;  // NavierStokes.cpp:153
+  for (int c = 0; c < ncolours; ++c) {
+    const int a = h_colour_offsets[(size_t)c], b = h_colour_offsets[(size_t)c + 1];
+    assemble_colour(stream_, b - a, d_order.p + a, d_tets.p, d_ten.p, nv, d_vol.p, d_grad.p,
+                    density, inv_mu, A_, d_M.p, d_K.p, P_, d_L.p, d_G3.p);
+  }
+  lumped_mass_inverse(stream_, A_, d_M.p, d_ilm.p);
+  lhs_dirty = lbc_dirty = true;
+}
+
+void NSSolver::build_sparse_matrices() {
+  if (!initialized) throw std::runtime_error("build_sparse_matrices before initialize");
+  assemble();
+  CFSA_HIP(hipStreamSynchronize(stream_));
+}
+
+void NSSolver::ensure_mixed_buffers() {
+  if (d_lhs32.n == (size_t)nnzA && d_Lbc32.n == (size_t)nnzP) return;
+  d_lhs32.alloc((size_t)nnzA);
+  d_Lbc32.alloc((size_t)nnzP);
+  d_invdA32.alloc((size_t)nu);
+  d_invdL32.alloc((size_t)nv);
+  f3_.alloc((size_t)nu * 3);
+  f1_.alloc((size_t)nv);
+  d_b32.alloc((size_t)nu * 3);
+  d_x32.alloc((size_t)nu * 3);
+  lhs_dirty = lbc_dirty = true;
+}
+
+// lhs = M/dt + K with 1e10 on Inlet/Wall diagonals (NavierStokes.cpp:460-495);
+// L_bc = L with 1e12 on Outlet (or reference-node) diagonals (:603-626).
+void NSSolver::prepare_operators() {
+  if (opt.precision == 1) ensure_mixed_buffers();
+  if (lhs_dirty || lhs_dt != dt) {
+    build_lhs(stream_, nnzA, d_M.p, d_K.p, dt, d_lhs.p);
+    add_diag_penalty(stream_, nu, d_Adiag.p, d_bc.p, 0, -1, kMomentumPenalty, d_lhs.p);
+    inv_diag(stream_, nu, d_Adiag.p, d_lhs.p, d_invdA.p);
+    if (d_lhs32.n == (size_t)nnzA) {
+      convert<float, double>(stream_, nnzA, d_lhs.p, d_lhs32.p);
+      convert<float, double>(stream_, nu, d_invdA.p, d_invdA32.p);
+    }
+    lhs_dirty = false;
+    lhs_dt = dt;
+  }
+  if (lbc_dirty) {
+    CFSA_HIP(hipMemcpyAsync(d_Lbc.p, d_L.p, (size_t)nnzP * sizeof(double),
+                            hipMemcpyDeviceToDevice, stream_));
+    if (outlet_type == 0)
+      add_diag_penalty(stream_, nv, d_Pdiag.p, d_bc.p, 1, -1, kPressurePenalty, d_Lbc.p);
+    else if (outlet_type == 1 && reference_node >= 0 && reference_node < nv)
+      add_diag_penalty(stream_, nv, d_Pdiag.p, d_bc.p, 2, reference_node, kPressurePenalty,
+                       d_Lbc.p);
+    inv_diag(stream_, nv, d_Pdiag.p, d_Lbc.p, d_invdL.p);
+    if (d_Lbc32.n == (size_t)nnzP) {
+      convert<float, double>(stream_, nnzP, d_Lbc.p, d_Lbc32.p);
+      convert<float, double>(stream_, nv, d_invdL.p, d_invdL32.p);
+    }
+    lbc_dirty = false;
+  }
+}
+
+void NSSolver::upload_inlet() {
+  if (!inlet_profile) fill3(stream_, nu, mean_inlet, d_ginlet.p);
+}
+
+void NSSolver::set_dt(double v) { dt = v; }
+void NSSolver::set_mean_inlet(const double v[3]) {
+  mean_inlet[0] = v[0];
+  mean_inlet[1] = v[1];
+  mean_inlet[2] = v[2];
+  inlet_profile = false;
+  if (initialized) {
+    upload_inlet();
+    CFSA_HIP(hipStreamSynchronize(stream_));
+  }
+}
+void NSSolver::set_inlet_profile(const double *g) {
+  if (!initialized) throw std::runtime_error("set_inlet_profile needs an initialized solver");
+  if (!g) {
+    inlet_profile = false;
+    upload_inlet();
+    CFSA_HIP(hipStreamSynchronize(stream_));
+    return;
+  }
+  inlet_profile = true;
+  CFSA_HIP(hipMemcpy(d_ginlet.p, g, (size_t)nu * 3 * sizeof(double), hipMemcpyHostToDevice));
+}
+void NSSolver::set_outlet_type(int t) {
+  outlet_type = t;
+  lbc_dirty = true;
+}
+void NSSolver::set_reference_node(int r) {
+  reference_node = r;
+  lbc_dirty = true;
+}
+
+// ---------------------------------------------------------------------------
+// Krylov drivers
+// ---------------------------------------------------------------------------
+void NSSolver::set_state_params(double tol, int maxit, int nrhs) {
+  hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, stream_, d_state.p, tol, maxit, nrhs);
+  CFSA_LAUNCH_CHECK();
+}
+void NSSolver::set_mask(int a, int b, int c) {
+  hipLaunchKernelGGL(k_set_mask, dim3(1), dim3(1), 0, stream_, d_mask.p, a, b, c);
+  CFSA_LAUNCH_CHECK();
+}
+void NSSolver::read_state() {
+  CFSA_HIP(hipMemcpyAsync(h_state, d_state.p, sizeof(KrylovState), hipMemcpyDeviceToHost,
+                          stream_));
+  CFSA_HIP(hipStreamSynchronize(stream_));
+}
+
+void NSSolver::spmv_event_begin() {
+  if (opt.timing < 2) return;
+  while ((int)ev_pool_.size() < ev_used_ + 2) {
+    hipEvent_t e;
+    CFSA_HIP(hipEventCreate(&e));
+    ev_pool_.push_back(e);
+  }
+  CFSA_HIP(hipEventRecord(ev_pool_[(size_t)ev_used_], stream_));
+}
+void NSSolver::spmv_event_end() {
+  if (opt.timing < 2) return;
+  CFSA_HIP(hipEventRecord(ev_pool_[(size_t)ev_used_ + 1], stream_));
+  ev_used_ += 2;
+}
+
+template <class T, int K>
+SolveResult NSSolver::bicgstab(const DevCsr &A, const T *val, const T *invd, const T *b,
+                               T *x, double tol, int maxit, KrylovWork<T> &w) {
+  hipStream_t s = stream_;
+  const int n = A.n;
+  KrylovState *st = d_state.p;
+  const int *act = st->active;
+  const bool time_spmv = (K == 3) && opt.timing >= 2;
+  set_state_params(tol, maxit, K);
+  int Gv = 0, Gb = 0, Gc = 0, Gr = 0;
+  bicg_init<T, K>(s, n, b, x, w.r.p, w.r0.p, w.p.p, w.v.p, d_part_a.p, &Gv);
+  bicg_s_init<K>(s, st, d_part_a.p, Gv);
+  const int chunk = std::max(1, opt.check_every);
+  const long max_chunks = (2L * maxit + 16) / chunk + 16;
+  for (long ch = 0;; ++ch) {
+    if (ch > max_chunks) throw std::runtime_error("BiCGSTAB driver did not terminate");
+    for (int j = 0; j < chunk; ++j) {
+      bicg_s_begin<K>(s, st, d_part_a.p, Gv);
+      bicg_v1<T, K>(s, n, st, w.r.p, w.p.p, w.v.p, invd, w.y.p);
+      if (time_spmv) spmv_event_begin();
+      if (K == 3)
+        spmv3_dot<T>(s, A, val, w.y.p, w.v.p, w.r0.p, act, d_part_b.p, &Gb);
+      else
+        spmv1_dot<T>(s, A, val, w.y.p, w.v.p, w.r0.p, act, d_part_b.p, &Gb);
+      if (time_spmv) spmv_event_end();
+      bicg_s_alpha<K>(s, st, d_part_b.p, Gb);
+      bicg_v2<T, K>(s, n, st, w.r.p, w.v.p, invd, w.s.p, w.z.p);
+      if (time_spmv) spmv_event_begin();
+      if (K == 3)
+        spmv3_dot2<T>(s, A, val, w.z.p, w.t.p, w.s.p, act, d_part_c.p, &Gc);
+      else
+        spmv1_dot2<T>(s, A, val, w.z.p, w.t.p, w.s.p, act, d_part_c.p, &Gc);
+      if (time_spmv) spmv_event_end();
+      bicg_s_omega<K>(s, st, d_part_c.p, Gc);
+      bicg_v3<T, K>(s, n, st, w.y.p, w.z.p, w.s.p, w.t.p, w.r0.p, x, w.r.p, d_part_a.p, &Gv);
+    }
+    read_state();
+    bool all_done = true;
+    int rs[3] = {0, 0, 0};
+    for (int c = 0; c < K; ++c) {
+      if (h_state->status[c] == KS_RESTART) rs[c] = 1;
+      if (h_state->status[c] != KS_DONE) all_done = false;
+    }
+    if (all_done) break;
+    if (rs[0] | rs[1] | rs[2]) {
+      // r = b - A x; r0 = r; rho = r0_sqnorm = |r|^2 (Eigen restart branch)
+      set_mask(rs[0], rs[1], rs[2]);
+      if (K == 3)
+        spmv3_residual<T>(s, A, val, x, b, w.r.p, w.r0.p, d_mask.p, d_part_r.p, &Gr);
+      else
+        spmv1_residual<T>(s, A, val, x, b, w.r.p, w.r0.p, d_mask.p, d_part_r.p, &Gr);
+      bicg_s_restart<K>(s, st, d_part_r.p, Gr, d_mask.p);
+    }
+  }
+  SolveResult res;
+  for (int c = 0; c < K; ++c) {
+    res.iters[c] = h_state->iters[c];
+    res.err[c] = h_state->err[c];
+    res.restarts[c] = h_state->restarts[c];
+    if (!(h_state->err[c] <= tol)) res.ok = false;
+  }
+  return res;
+}
+
+template <class T>
+SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
+                          double tol, int maxit, KrylovWork<T> &w) {
+  hipStream_t s = stream_;
+  const int n = A.n;
+  KrylovState *st = d_state.p;
+  const int *act = st->active;
+  set_state_params(tol, maxit, 1);
+  int Gv = 0, Gb = 0;
+  cg_init<T>(s, n, b, invd, x, w.r.p, w.p.p, d_part_a.p, &Gv);
+  cg_s_init(s, st, d_part_a.p, Gv);
+  const int chunk = std::max(1, opt.check_every);
+  const long max_chunks = (long)maxit / chunk + 16;
+  for (long ch = 0;; ++ch) {
+    if (ch > max_chunks) throw std::runtime_error("PCG driver did not terminate");
+    read_state();
+    if (h_state->status[0] == KS_DONE) break;
+    for (int j = 0; j < chunk; ++j) {
+      spmv1_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
+      cg_s_alpha(s, st, d_part_b.p, Gb);
+      cg_v1<T>(s, n, st, w.p.p, w.v.p, invd, x, w.r.p, w.z.p, d_part_a.p, &Gv);
+      cg_s_beta(s, st, d_part_a.p, Gv);
+      cg_v2<T>(s, n, st, w.z.p, w.p.p);
+    }
+  }
+  SolveResult res;
+  res.iters[0] = h_state->iters[0];
+  res.err[0] = h_state->err[0];
+  res.ok = h_state->err[0] <= tol;
+  return res;
+}
+
+// FP64 iterative refinement around FP32 inner Krylov solves: the residual
+// b - A x, its norm and the stopping test are FP64; matrix values, vectors and
+// SpMV accumulation of the inner solve are FP32 (inner scalars stay FP64).
+template <int K>
+SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const float *val32,
+                             const float *invd32, const double *b, double *x, double tol,
+                             int maxit, int method) {
+  hipStream_t s = stream_;
+  const int n = A.n;
+  KrylovWork<double> &w64 = (K == 3) ? w3_ : w1_;
+  KrylovWork<float> &w32 = (K == 3) ? f3_ : f1_;
+  CFSA_HIP(hipMemsetAsync(x, 0, (size_t)n * K * sizeof(double), s));
+  int Gr = 0;
+  set_mask(1, K == 3, K == 3);
+  auto residual = [&]() {
+    if (K == 3)
+      spmv3_residual<double>(s, A, val64, x, b, d_r64.p, w64.r0.p, d_mask.p, d_part_r.p, &Gr);
+    else
+      spmv1_residual<double>(s, A, val64, x, b, d_r64.p, w64.r0.p, d_mask.p, d_part_r.p, &Gr);
+    hipLaunchKernelGGL((k_reduce_out<K>), dim3(1), dim3(kBlock), 0, s, d_part_r.p, Gr,
+                       d_scalar.p);
+    CFSA_LAUNCH_CHECK();
+    double out[3] = {0, 0, 0};
+    CFSA_HIP(hipMemcpyAsync(out, d_scalar.p, K * sizeof(double), hipMemcpyDeviceToHost, s));
+    CFSA_HIP(hipStreamSynchronize(s));
+    return std::vector<double>(out, out + 3);
+  };
+  std::vector<double> bb = residual();  // x = 0: r = b
+  std::vector<double> rr = bb;
+  SolveResult res;
+  int conv[3] = {1, 1, 1};
+  for (int k = 0;; ++k) {
+    bool all = true;
+    for (int c = 0; c < K; ++c) {
+      conv[c] = (bb[c] == 0.0) || (rr[c] <= tol * tol * bb[c]);
+      all = all && conv[c];
+    }
+    if (all || k >= opt.max_refine) {
+      res.refinements = k;
+      break;
+    }
+    set_mask(!conv[0], K == 3 && !conv[1], K == 3 && !conv[2]);
+    hipLaunchKernelGGL(k_demote_masked, dim3(2048), dim3(kBlock), 0, s, n, K, d_r64.p,
+                       d_mask.p, d_b32.p);
+    CFSA_LAUNCH_CHECK();
+    SolveResult in;
+    if (K == 1 && method == 1)
+      in = pcg<float>(A, val32, invd32, d_b32.p, d_x32.p, opt.inner_tol, maxit, w32);
+    else
+      in = bicgstab<float, K>(A, val32, invd32, d_b32.p, d_x32.p, opt.inner_tol, maxit, w32);
+    for (int c = 0; c < K; ++c) res.iters[c] += in.iters[c];
+    hipLaunchKernelGGL(k_accumulate, dim3(2048), dim3(kBlock), 0, s, (int64_t)n * K, d_x32.p,
+                       x);
+    CFSA_LAUNCH_CHECK();
+    set_mask(1, K == 3, K == 3);
+    rr = residual();
+  }
+  for (int c = 0; c < K; ++c) {
+    res.err[c] = bb[c] == 0.0 ? 0.0 : std::sqrt(rr[c] / bb[c]);
+    if (!(res.err[c] <= tol)) res.ok = false;
+  }
+  return res;
+}
+
+SolveResult NSSolver::solve_momentum(const double *b, double *x) {
+  if (opt.precision == 1)
+    return refine<3>(A_, d_lhs.p, d_lhs32.p, d_invdA32.p, b, x, opt.momentum_tol, opt.max_iter,
+                     0);
+  return bicgstab<double, 3>(A_, d_lhs.p, d_invdA.p, b, x, opt.momentum_tol, opt.max_iter, w3_);
+}
+
+SolveResult NSSolver::solve_pressure(const double *b, double *x) {
+  if (opt.precision == 1)
+    return refine<1>(P_, d_Lbc.p, d_Lbc32.p, d_invdL32.p, b, x, opt.pressure_tol, opt.max_iter,
+                     opt.pressure_solver);
+  if (opt.pressure_solver == 1)
+    return pcg<double>(P_, d_Lbc.p, d_invdL.p, b, x, opt.pressure_tol, opt.max_iter, w1_);
+  return bicgstab<double, 1>(P_, d_Lbc.p, d_invdL.p, b, x, opt.pressure_tol, opt.max_iter, w1_);
+}
+
+// ---------------------------------------------------------------------------
+// PISO step (NavierStokes.cpp:374-414)
+// ---------------------------------------------------------------------------
+int NSSolver::momentum_predictor() {
+  // computeAdvectionRHS (:526-584) from the P1 vertex velocities
+  conv_tet(stream_, nt, d_tets.p, d_grad.p, d_u.p, d_cT.p);
+  conv_gather(stream_, nu, nv, d_incp.p, d_inct.p, d_vol.p, d_cT.p, d_conv.p);
+  // G_c p for the P1 rows (:441-452)
+  spmv_grad3(stream_, P_, d_G3.p, d_p.p, d_gp.p);
+  momentum_rhs(stream_, A_, d_M.p, d_u.p, d_gp.p, d_conv.p, d_bc.p, d_ginlet.p, dt, nv,
+               kMomentumPenalty, d_rhs.p);
+  SolveResult r = solve_momentum(d_rhs.p, d_us.p);
+  for (int c = 0; c < 3; ++c) {
+    last.momentum_iters[c] = r.iters[c];
+    last.momentum_err[c] = r.err[c];
+  }
+  last.momentum_refinements = r.refinements;
+  if (!r.ok) {
+    int bad = 0;
+    while (bad < 3 && r.err[bad] <= opt.momentum_tol) ++bad;
+    last_error = "[NS] Momentum solve failed for component " + std::to_string(bad);
+    std::cerr << last_error << std::endl;
+    std::cerr << "[NS] Momentum predictor failed" << std::endl;
+    return ST_MOMENTUM_NOCONV;
+  }
+  return ST_OK;
+}
+
+int NSSolver::pressure_poisson() {
+  // computeDivergence (:670-700) and rhs = -div/dt with penalties (:598-626)
+  div_tet(stream_, nt, d_tets.p, d_grad.p, d_vol.p, d_us.p, d_wT.p);
+  const int mode = outlet_type == 0 ? 1 : (outlet_type == 1 ? 2 : 0);
+  div_gather_prhs(stream_, nv, d_incp.p, d_inct.p, d_wT.p, d_bc.p, mode, reference_node, dt,
+                  outlet_pressure, kPressurePenalty, d_div.p, d_rhsp.p);
+  SolveResult r = solve_pressure(d_rhsp.p, d_pc.p);
+  const int k = std::min(last.n_pressure, 7);
+  last.pressure_iters[k] = r.iters[0];
+  last.pressure_err[k] = r.err[0];
+  last.pressure_refinements[k] = r.refinements;
+  last.n_pressure++;
+  if (!r.ok) {
+    last_error = "[NS] Pressure Poisson solve failed";
+    std::cerr << last_error << std::endl;
+    return ST_PRESSURE_NOCONV;
+  }
+  // p += relax_p p'; Dirichlet outlet reset; normalize to reference node
+  pressure_update(stream_, nv, d_p.p, d_pc.p, relax_p, d_bc.p, outlet_type == 0,
+                  outlet_pressure);
+  pressure_normalize(stream_, nv, d_p.p, reference_node, d_scalar.p);
+  return ST_OK;
+}
+
+void NSSolver::correct_velocity() {
+  spmv_grad3(stream_, P_, d_G3.p, d_pc.p, d_gp.p);
+  cfsa::correct_velocity(stream_, nu, nv, d_bc.p, d_ginlet.p, d_ilm.p, d_gp.p, d_us.p, dt,
+                         relax_u, d_u.p);
+}
+
+bool NSSolver::has_nans() {
+  d_flag.zero(stream_);
+  nan_check(stream_, nu * 3, d_u.p, d_flag.p);
+  nan_check(stream_, nu * 3, d_us.p, d_flag.p);
+  nan_check(stream_, nv, d_p.p, d_flag.p);
+  nan_check(stream_, nv, d_pc.p, d_flag.p);
+  int f = 0;
+  CFSA_HIP(hipMemcpyAsync(&f, d_flag.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  return f != 0;
+}
+
+int NSSolver::finish_step() {
+  reset_velocity_bc(stream_, nu, d_bc.p, d_ginlet.p, d_u.p);
+  time += dt;
+  return has_nans() ? ST_NAN : ST_OK;
+}
+
+int NSSolver::piso_step() {
+  if (!initialized) {
+    last_error = "[NS] Error: solver not initialized";
+    std::cerr << last_error << std::endl;
+    return ST_NOT_INIT;
+  }
+  std::memset(&last, 0, sizeof(last));
+  ev_used_ = 0;
+  hipEvent_t ev[6] = {};
+  const bool timing = opt.timing >= 1;
+  if (timing) {
+    for (auto &e : ev) CFSA_HIP(hipEventCreate(&e));
+    CFSA_HIP(hipEventRecord(ev[0], stream_));
+  }
+  prepare_operators();
+  int st = momentum_predictor();
+  if (timing) CFSA_HIP(hipEventRecord(ev[1], stream_));
+  if (st == ST_OK) {
+    for (int k = 0; k < n_corrections && st == ST_OK; ++k) {
+      st = pressure_poisson();
+      if (st == ST_OK) correct_velocity();
+    }
+  }
+  if (timing) CFSA_HIP(hipEventRecord(ev[2], stream_));
+  if (st == ST_OK) st = finish_step();
+  if (timing) {
+    CFSA_HIP(hipEventRecord(ev[3], stream_));
+    CFSA_HIP(hipEventSynchronize(ev[3]));
+    float a = 0, b = 0, c = 0;
+    CFSA_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+    CFSA_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+    CFSA_HIP(hipEventElapsedTime(&c, ev[0], ev[3]));
+    last.ms_momentum = a;
+    last.ms_pressure = b;
+    last.ms_total = c;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+  }
+  if (opt.timing >= 2) {
+    double tot = 0.0;
+    for (int i = 0; i + 1 < ev_used_; i += 2) {
+      float ms = 0;
+      CFSA_HIP(hipEventElapsedTime(&ms, ev_pool_[(size_t)i], ev_pool_[(size_t)i + 1]));
+      tot += ms;
+    }
+    last.spmv_launches = ev_used_ / 2;
+    last.spmv_ms = tot;
+  }
+  return st;
+}
+
+// simpleStep (NavierStokes.cpp:968-1009): one pressure correction with an
+// extra outlet-pressure reset before the velocity correction.
+int NSSolver::simple_step() {
+  if (!initialized) {
+    last_error = "[NS-Simple] Error: solver not initialized";
+    std::cerr << last_error << std::endl;
+    return ST_NOT_INIT;
+  }
+  std::memset(&last, 0, sizeof(last));
+  prepare_operators();
+  int st = momentum_predictor();
+  if (st != ST_OK) return st;
+  st = pressure_poisson();
+  if (st != ST_OK) return st;
+  // reinforcePressureBCs (NavierStokes.cpp:1011-1024)
+  if (outlet_type == 0) pressure_outlet_reset(stream_, nv, d_p.p, d_bc.p, outlet_pressure);
+  correct_velocity();
+  return finish_step();
+}
+
+// ---------------------------------------------------------------------------
+// Field access and parity entry points
+// ---------------------------------------------------------------------------
+void NSSolver::get_velocity(double *out) {
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  d_u.download(out, (size_t)nu * 3);
+}
+void NSSolver::set_velocity(const double *in) {
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  CFSA_HIP(hipMemcpy(d_u.p, in, (size_t)nu * 3 * sizeof(double), hipMemcpyHostToDevice));
+}
+void NSSolver::get_velocity_star(double *out) {
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  d_us.download(out, (size_t)nu * 3);
+}
+void NSSolver::get_pressure(double *out) {
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  d_p.download(out, (size_t)nv);
+}
+void NSSolver::set_pressure(const double *in) {
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  CFSA_HIP(hipMemcpy(d_p.p, in, (size_t)nv * sizeof(double), hipMemcpyHostToDevice));
+}
+void NSSolver::get_pressure_correction(double *out) {
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  d_pc.download(out, (size_t)nv);
+}
+void NSSolver::get_inv_lumped_mass(double *out) {
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  d_ilm.download(out, (size_t)nu);
+}
+void NSSolver::get_colouring(int32_t *colour) const {
+  for (int c = 0; c < ncolours; ++c)
+    for (int q = h_colour_offsets[(size_t)c]; q < h_colour_offsets[(size_t)c + 1]; ++q)
+      colour[h_order[(size_t)q]] = c;
+}
+void NSSolver::get_mesh_arrays(double *coords, int32_t *tets, uint8_t *p1_bc) const {
+  if (coords) std::memcpy(coords, h_coords.data(), h_coords.size() * sizeof(double));
+  if (tets) std::memcpy(tets, h_tets.data(), h_tets.size() * sizeof(int32_t));
+  if (p1_bc) std::memcpy(p1_bc, h_bc.data(), (size_t)nv);
+}
+
+void NSSolver::get_incidence(int32_t *ptr, int32_t *tet) {
+  d_incp.download(ptr, (size_t)nu + 1);
+  d_inct.download(tet, (size_t)nt * 10);
+}
+
+// which: 0 M, 1 K, 2 momentum lhs (with penalties), 3 L, 4 L_bc, 5 Gx, 6 Gy, 7 Gz
+void NSSolver::get_matrix(int which, int32_t *rowptr, int32_t *col, double *val) {
+  prepare_operators();
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  const bool p2 = which <= 2;
+  const DBuf<int> &rp = p2 ? d_Arp : d_Prp;
+  const DBuf<int> &cl = p2 ? d_Acol : d_Pcol;
+  const int64_t nnz = p2 ? nnzA : nnzP;
+  if (rowptr) rp.download(rowptr, rp.n);
+  if (col) cl.download(col, (size_t)nnz);
+  if (!val) return;
+  if (which == 0) d_M.download(val, (size_t)nnz);
+  else if (which == 1) d_K.download(val, (size_t)nnz);
+  else if (which == 2) d_lhs.download(val, (size_t)nnz);
+  else if (which == 3) d_L.download(val, (size_t)nnz);
+  else if (which == 4) d_Lbc.download(val, (size_t)nnz);
+  else if (which >= 5 && which <= 7) {
+    std::vector<double> g((size_t)nnz * 3);
+    d_G3.download(g.data(), g.size());
+    for (int64_t k = 0; k < nnz; ++k) val[k] = g[(size_t)k * 3 + (which - 5)];
+  } else
+    throw std::invalid_argument("get_matrix: unknown matrix id");
+}
+
+void NSSolver::compute_advection(double *out) {
+  conv_tet(stream_, nt, d_tets.p, d_grad.p, d_u.p, d_cT.p);
+  conv_gather(stream_, nu, nv, d_incp.p, d_inct.p, d_vol.p, d_cT.p, d_conv.p);
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  d_conv.download(out, (size_t)nu * 3);
+}
+
+void NSSolver::compute_divergence(const double *u, double *out) {
+  CFSA_HIP(hipMemcpy(d_xref.p, u, (size_t)nu * 3 * sizeof(double), hipMemcpyHostToDevice));
+  div_tet(stream_, nt, d_tets.p, d_grad.p, d_vol.p, d_xref.p, d_wT.p);
+  div_gather_prhs(stream_, nv, d_incp.p, d_inct.p, d_wT.p, d_bc.p, 0, -1, 1.0, 0.0, 0.0,
+                  d_div.p, d_rhsp.p);
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  d_div.download(out, (size_t)nv);
+}
+
+// y = A x for matrix `which` (see get_matrix); nrhs 1 or 3 for P2 matrices,
+// and for G (5..7) x has nv entries and y is nv x 3 (all three components).
+void NSSolver::apply_matrix(int which, const double *x, double *y, int nrhs) {
+  prepare_operators();
+  const bool p2 = which <= 2;
+  const int n = p2 ? nu : nv;
+  const double *val = which == 0 ? d_M.p : which == 1 ? d_K.p : which == 2 ? d_lhs.p
+                      : which == 3 ? d_L.p : d_Lbc.p;
+  DBuf<double> dx, dy;
+  if (which >= 5) {
+    dx.upload(x, (size_t)nv);
+    dy.alloc((size_t)nv * 3);
+    spmv_grad3(stream_, P_, d_G3.p, dx.p, dy.p);
+    CFSA_HIP(hipStreamSynchronize(stream_));
+    dy.download(y, (size_t)nv * 3);
+    return;
+  }
+  dx.upload(x, (size_t)n * nrhs);
+  dy.alloc((size_t)n * nrhs);
+  if (nrhs == 3)
+    spmv3<double>(stream_, p2 ? A_ : P_, val, dx.p, dy.p, nullptr);
+  else
+    spmv1<double>(stream_, p2 ? A_ : P_, val, dx.p, dy.p);
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  dy.download(y, (size_t)n * nrhs);
+}
+
+// Solve matrix `which` (2 momentum lhs, 4 L_bc) with method 0 BiCGSTAB / 1 PCG
+// in FP64 (precision 0) or FP32-inner refinement (precision 1).
+SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double *b,
+                                   double *x, double tol, int maxit, int precision) {
+  if (which != 2 && which != 4) throw std::invalid_argument("krylov_solve: matrix 2 or 4");
+  if (nrhs != 1 && nrhs != 3) throw std::invalid_argument("krylov_solve: nrhs 1 or 3");
+  if (which == 4 && nrhs != 1) throw std::invalid_argument("krylov_solve: L_bc takes 1 rhs");
+  if (method == 1 && nrhs != 1) throw std::invalid_argument("krylov_solve: PCG takes 1 rhs");
+  const int saved = opt.precision;
+  opt.precision = precision;
+  prepare_operators();
+  opt.precision = saved;
+  const DevCsr &A = which == 2 ? A_ : P_;
+  const int n = A.n;
+  DBuf<double> db, dx;
+  db.upload(b, (size_t)n * nrhs);
+  dx.alloc((size_t)n * nrhs);
+  SolveResult r;
+  if (precision == 1) {
+    if (nrhs == 3)
+      r = refine<3>(A, d_lhs.p, d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit, 0);
+    else if (which == 2)
+      throw std::invalid_argument("krylov_solve: mixed momentum needs 3 rhs");
+    else
+      r = refine<1>(A, d_Lbc.p, d_Lbc32.p, d_invdL32.p, db.p, dx.p, tol, maxit, method);
+  } else if (method == 1) {
+    r = pcg<double>(A, which == 2 ? d_lhs.p : d_Lbc.p, which == 2 ? d_invdA.p : d_invdL.p,
+                    db.p, dx.p, tol, maxit, which == 2 ? w3_ : w1_);
+  } else if (nrhs == 3) {
+    r = bicgstab<double, 3>(A, d_lhs.p, d_invdA.p, db.p, dx.p, tol, maxit, w3_);
+  } else {
+    r = bicgstab<double, 1>(A, which == 2 ? d_lhs.p : d_Lbc.p,
+                            which == 2 ? d_invdA.p : d_invdL.p, db.p, dx.p, tol, maxit,
+                            which == 2 ? w3_ : w1_);
+  }
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  dx.download(x, (size_t)n * nrhs);
+  return r;
+}
+
+double NSSolver::time_momentum_spmv(int reps, int precision) {
+  prepare_operators();
+  if (precision == 1) {
+    ensure_mixed_buffers();
+    prepare_operators();
+  }
+  hipEvent_t a, b;
+  CFSA_HIP(hipEventCreate(&a));
+  CFSA_HIP(hipEventCreate(&b));
+  CFSA_HIP(hipEventRecord(a, stream_));
+  for (int i = 0; i < reps; ++i) {
+    if (precision == 1)
+      spmv3<float>(stream_, A_, d_lhs32.p, f3_.y.p, f3_.v.p, nullptr);
+    else
+      spmv3<double>(stream_, A_, d_lhs.p, w3_.y.p, w3_.v.p, nullptr);
+  }
+  CFSA_HIP(hipEventRecord(b, stream_));
+  CFSA_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  CFSA_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+}  // namespace cfsa

@@ -1,0 +1,229 @@
+// GPU Navier-Stokes PISO solver: device state and orchestration behind the
+// C ABI.  State, defaults and step order mirror NavierStokesSolver
+// (include/FEM/NavierStokes.hpp:53-193, src/FEM/NavierStokes.cpp:10-1024).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "dev.hpp"
+#include "ops.hpp"
+
+namespace cfsa {
+
+enum Status {
+  ST_OK = 0,
+  ST_NOT_INIT = 1,
+  ST_MOMENTUM_NOCONV = 2,
+  ST_PRESSURE_NOCONV = 3,
+  ST_NAN = 4,
+  ST_INVALID = 5,
+  ST_DEVICE = 6,
+  ST_IO = 7,
+};
+
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (p && count == n) return;
+    release();
+    n = count;
+    if (count) CFSA_HIP(hipMalloc(&p, count * sizeof(T)));
+  }
+  void upload(const T *h, size_t count) {
+    alloc(count);
+    if (count) CFSA_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void download(T *h, size_t count) const {
+    if (count) CFSA_HIP(hipMemcpy(h, p, count * sizeof(T), hipMemcpyDeviceToHost));
+  }
+  void zero(hipStream_t s) {
+    if (n) CFSA_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+};
+
+struct SolverOptions {
+  double momentum_tol = 1e-6;  // NavierStokes.cpp:500
+  double pressure_tol = 1e-8;  // NavierStokes.cpp:635
+  int max_iter = 1000;         // NavierStokes.cpp:499,634
+  int pressure_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG
+  int precision = 0;           // 0 FP64, 1 FP32 inner solves + FP64 refinement
+  double inner_tol = 1e-4;     // relative tolerance of each FP32 inner solve
+  int max_refine = 40;         // outer refinement sweeps (mixed)
+  int check_every = 8;         // Krylov iterations between host status reads
+  int timing = 0;              // 1: per-phase events; 2: also per-SpMV events
+};
+
+struct SolveResult {
+  int iters[3] = {0, 0, 0};
+  double err[3] = {0, 0, 0};
+  int restarts[3] = {0, 0, 0};
+  int refinements = 0;
+  int inner_iters = 0;
+  bool ok = true;
+};
+
+struct StepStats {
+  int momentum_iters[3];
+  double momentum_err[3];
+  int momentum_refinements;
+  int n_pressure;
+  int pressure_iters[8];
+  double pressure_err[8];
+  int pressure_refinements[8];
+  double ms_conv, ms_rhs, ms_momentum, ms_pressure, ms_correct, ms_total;
+  int spmv_launches;
+  double spmv_ms;  // summed event time of momentum-matrix SpMV launches
+};
+
+struct MeshInput {
+  int64_t nv = 0, nt = 0, ne = 0;
+  const double *coords = nullptr;     // nv x 3
+  const int32_t *tets = nullptr;      // nt x 4
+  const int32_t *ten = nullptr;       // nt x 6 edge-node ids (0..ne-1)
+  const uint8_t *p1_bc = nullptr;     // nv
+  const uint8_t *p2_bc = nullptr;     // ne
+};
+
+template <class T>
+struct KrylovWork {
+  DBuf<T> r, r0, p, v, s, t, y, z;
+  void alloc(size_t n) {
+    r.alloc(n); r0.alloc(n); p.alloc(n); v.alloc(n);
+    s.alloc(n); t.alloc(n); y.alloc(n); z.alloc(n);
+  }
+};
+
+class NSSolver {
+ public:
+  NSSolver();
+  ~NSSolver();
+
+  // ---- reference state and defaults (NavierStokes.hpp:54-83) ----
+  bool initialized = false;
+  int reference_node = -1;
+  double tolerance = 1e-12;
+  int n_corrections = 1;
+  double relax_u = 1.0, relax_p = 1.0, dt = 1e-4, time = 0.0;
+  double density = 1000.0, viscosity = 0.001;
+  double mean_inlet[3] = {0.0, 0.0, 0.0};
+  int inlet_type = 0;     // InletType::Uniform
+  double outlet_pressure = 0.0;
+  int outlet_type = 0;    // OutletType::DirichletPressure
+  bool inlet_profile = false;
+  SolverOptions opt;
+  StepStats last{};
+  std::string last_error;
+
+  // ---- sizes ----
+  int nv = 0, nt = 0, ne = 0, nu = 0;
+  int64_t nnzA = 0, nnzP = 0;
+  int ncolours = 0;
+
+  void initialize(const MeshInput &m, double rho, double mu);
+  void update_mesh(const double *coords, const uint8_t *p1_bc, const uint8_t *p2_bc);
+  void build_sparse_matrices();
+  int piso_step();
+  int simple_step();
+  bool has_nans();
+
+  // setters with side effects on cached operators
+  void set_dt(double v);
+  void set_mean_inlet(const double v[3]);
+  void set_inlet_profile(const double *g);  // nu x 3 host array, nullptr = uniform
+  void set_outlet_type(int t);
+  void set_reference_node(int r);
+
+  // host <-> device field access
+  void get_velocity(double *out);
+  void set_velocity(const double *in);
+  void get_velocity_star(double *out);
+  void get_pressure(double *out);
+  void set_pressure(const double *in);
+  void get_pressure_correction(double *out);
+  void get_inv_lumped_mass(double *out);
+  void get_colouring(int32_t *colour) const;
+  void get_matrix(int which, int32_t *rowptr, int32_t *col, double *val);
+  void get_incidence(int32_t *ptr, int32_t *tet);
+  void get_mesh_arrays(double *coords, int32_t *tets, uint8_t *p1_bc) const;
+  void compute_advection(double *out);             // computeAdvectionRHS
+  void compute_divergence(const double *u, double *out);  // computeDivergence
+  void apply_matrix(int which, const double *x, double *y, int nrhs);
+  SolveResult krylov_solve(int which, int method, int nrhs, const double *b, double *x,
+                           double tol, int maxit, int precision);
+  hipStream_t stream() const { return stream_; }
+  // Momentum-matrix SpMV launches repeated `reps` times between two events.
+  double time_momentum_spmv(int reps, int precision);
+
+ private:
+  hipStream_t stream_ = nullptr;
+  // mesh on device
+  DBuf<int> d_tets, d_ten, d_order, d_incp, d_inct;
+  DBuf<double> d_vol, d_grad;
+  DBuf<uint8_t> d_bc;
+  std::vector<int32_t> h_colour_offsets, h_order;
+  std::vector<uint8_t> h_bc;
+  std::vector<double> h_coords;
+  std::vector<int32_t> h_tets, h_ten;
+  // patterns
+  DBuf<int> d_Arp, d_Acol, d_Adiag, d_Prp, d_Pcol, d_Pdiag;
+  DevCsr A_{}, P_{};
+  // operators
+  DBuf<double> d_M, d_K, d_lhs, d_L, d_Lbc, d_G3, d_ilm, d_invdA, d_invdL;
+  DBuf<float> d_lhs32, d_Lbc32, d_invdA32, d_invdL32;
+  bool lhs_dirty = true, lbc_dirty = true;
+  double lhs_dt = 0.0;
+  // fields
+  DBuf<double> d_u, d_us, d_p, d_pc, d_ginlet, d_conv, d_cT, d_wT, d_div, d_rhs,
+      d_rhsp, d_gp, d_scalar, d_xref;
+  DBuf<int> d_flag, d_mask;
+  // Krylov
+  KrylovWork<double> w3_, w1_;
+  KrylovWork<float> f3_, f1_;
+  DBuf<float> d_b32, d_x32;
+  DBuf<double> d_r64, d_part_a, d_part_b, d_part_c, d_part_r;
+  DBuf<KrylovState> d_state;
+  KrylovState *h_state = nullptr;
+  std::vector<hipEvent_t> ev_pool_;
+  int ev_used_ = 0;
+
+  void assemble();
+  void prepare_operators();
+  void upload_inlet();
+  int momentum_predictor();
+  int pressure_poisson();
+  void correct_velocity();
+  int finish_step();
+  void ensure_mixed_buffers();
+  void set_state_params(double tol, int maxit, int nrhs);
+  void set_mask(int m0, int m1, int m2);
+  void read_state();
+  void spmv_event_begin();
+  void spmv_event_end();
+
+  template <class T, int K>
+  SolveResult bicgstab(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
+                       double tol, int maxit, KrylovWork<T> &w);
+  template <class T>
+  SolveResult pcg(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
+                  double tol, int maxit, KrylovWork<T> &w);
+  template <int K>
+  SolveResult refine(const DevCsr &A, const double *val64, const float *val32,
+                     const float *invd32, const double *b, double *x, double tol,
+                     int maxit, int method);
+  SolveResult solve_momentum(const double *b, double *x);
+  SolveResult solve_pressure(const double *b, double *x);
+};
+
+}  // namespace cfsa

@@ -1,0 +1,222 @@
+/*
+ * C ABI of the MI355X Navier-Stokes PISO library (libcfsa_ns.so).
+ *
+ * The library replaces the reference's C++ class NavierStokesSolver
+ * (include/FEM/NavierStokes.hpp:53-193 of sudo-shaka/Cell-Fluid-Structure-
+ * Analysis) and the parts of Mesh (include/Mesh/Mesh.hpp:35-220) that the
+ * solver reads.  Every entry point is extern "C", takes plain pointers and
+ * sizes, never throws, and returns CFSA_OK or an error code; the message of
+ * the last failure on the calling thread is returned by cfsa_last_error().
+ *
+ * Array layouts match the reference's std::vector storage:
+ *   velocity   double[nu][3]  (P1 vertices 0..nv-1, then P2 edge nodes;
+ *                              std::vector<Eigen::Vector3d>, NavierStokes.hpp:66)
+ *   pressure   double[nv]     (std::vector<double>, NavierStokes.hpp:68)
+ *   labels     uint8 FluidBCType values Wall=0 Inlet=1 Outlet=2 Internal=3
+ *              Undefined=4 (NavierStokes.hpp:13-19)
+ * Handles are not thread safe (the reference solver is single threaded).
+ */
+#ifndef CFSA_NS_H
+#define CFSA_NS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFSA_ABI_VERSION 1
+
+enum cfsa_status {
+  CFSA_OK = 0,
+  CFSA_ERR_NOT_INIT = 1,         /* pisoStep before initialize (NavierStokes.cpp:375-378) */
+  CFSA_ERR_MOMENTUM_NOCONV = 2,  /* BiCGSTAB NoConvergence (NavierStokes.cpp:511-515) */
+  CFSA_ERR_PRESSURE_NOCONV = 3,  /* NavierStokes.cpp:646-649 */
+  CFSA_ERR_NAN = 4,              /* hasNans() after the step (NavierStokes.hpp:116-138) */
+  CFSA_ERR_INVALID = 5,          /* bad argument */
+  CFSA_ERR_DEVICE = 6,           /* HIP runtime failure */
+  CFSA_ERR_IO = 7                /* file could not be read or written */
+};
+
+/* enum values of the reference (NavierStokes.hpp:13-31) */
+enum cfsa_fluid_bc { CFSA_WALL = 0, CFSA_INLET = 1, CFSA_OUTLET = 2, CFSA_INTERNAL = 3, CFSA_UNDEFINED = 4 };
+enum cfsa_outlet_type { CFSA_OUTLET_DIRICHLET = 0, CFSA_OUTLET_NEUMANN = 1, CFSA_OUTLET_UNDEFINED = 2 };
+enum cfsa_inlet_type { CFSA_INLET_UNIFORM = 0, CFSA_INLET_PULSATILE = 1, CFSA_INLET_UNDEFINED = 2 };
+
+typedef struct cfsa_mesh cfsa_mesh;
+typedef struct cfsa_ns cfsa_ns;
+
+/* Solver options.  The defaults reproduce the reference exactly; everything
+ * else is an extension (tolerances of NavierStokes.cpp:499-500,634-635). */
+typedef struct cfsa_solver_options {
+  double momentum_tol;   /* 1e-6 */
+  double pressure_tol;   /* 1e-8 */
+  int max_iter;          /* 1000 */
+  int pressure_solver;   /* 0 BiCGSTAB (reference), 1 Jacobi PCG */
+  int precision;         /* 0 FP64, 1 FP32 inner solves in FP64 iterative refinement */
+  double inner_tol;      /* relative tolerance of each FP32 inner solve, 1e-4 */
+  int max_refine;        /* refinement sweeps, 40 */
+  int check_every;       /* Krylov iterations between host convergence reads, 8 */
+  int timing;            /* 0 off, 1 per-phase events, 2 also per-SpMV events */
+} cfsa_solver_options;
+
+typedef struct cfsa_step_stats {
+  int momentum_iters[3];
+  double momentum_err[3];
+  int momentum_refinements;
+  int n_pressure;
+  int pressure_iters[8];
+  double pressure_err[8];
+  int pressure_refinements[8];
+  double ms_conv, ms_rhs, ms_momentum, ms_pressure, ms_correct, ms_total;
+  int spmv_launches;
+  double spmv_ms;
+} cfsa_step_stats;
+
+typedef struct cfsa_solve_result {
+  int iters[3];
+  double err[3];
+  int restarts[3];
+  int refinements;
+  int inner_iters;
+  int ok;
+} cfsa_solve_result;
+
+const char *cfsa_last_error(void);
+int cfsa_abi_version(void);
+
+/* ------------------------------------------------------------------------
+ * Mesh (reference Mesh, include/Mesh/Mesh.hpp:35-220)
+ * ---------------------------------------------------------------------- */
+/* Mesh::structuredRectangularPrism (Mesh.hpp:87-93, Mesh.cpp:146-225) */
+int cfsa_mesh_structured_prism(double length, double width, double height, int nx, int ny,
+                               int nz, cfsa_mesh **out);
+/* Cylinder along +x; replaces Mesh::fromPolyhedron(Polyhedron::cylendar(...),
+ * max_element_size) (src/main.cpp:25-33), which needs TetGen. */
+int cfsa_mesh_cylinder(double length, double radius, int divisions, double max_volume,
+                       cfsa_mesh **out);
+/* Mesh::fromMshFile (Mesh.hpp:82-86, Mesh.cpp:16-144), Gmsh MSH 4.1 ASCII */
+int cfsa_mesh_from_msh(const char *path, cfsa_mesh **out);
+/* vertices + tets from any source; runs the same connectivity/geometry/P2 setup */
+int cfsa_mesh_from_arrays(int64_t nv, const double *coords, int64_t nt, const int32_t *tets,
+                          cfsa_mesh **out);
+/* Mesh::setupBoundaryConditions (Mesh.hpp:94-96, Mesh.cpp:325-417) */
+int cfsa_mesh_setup_boundary_conditions(cfsa_mesh *m, const double dir[3], double coverage);
+/* Mesh::setFluidVertexBC for all vertices, then setP2BoundariesFromP1Boundaries
+ * (Mesh.hpp:180-215) */
+int cfsa_mesh_set_p1_bc(cfsa_mesh *m, const uint8_t *p1_bc);
+/* move vertices and Mesh::updateGeometry (Mesh.hpp:98-101) */
+int cfsa_mesh_update_vertices(cfsa_mesh *m, const double *coords);
+/* out[0..4] = nVertices, nTets, getNumberOfEdgeNodes, nFaces, nv + ne */
+int cfsa_mesh_sizes(const cfsa_mesh *m, int64_t out[5]);
+int cfsa_mesh_get_vertices(const cfsa_mesh *m, double *out);           /* nv x 3 */
+int cfsa_mesh_get_tets(const cfsa_mesh *m, int32_t *out);              /* nt x 4 */
+int cfsa_mesh_get_volumes(const cfsa_mesh *m, double *out);            /* nt */
+int cfsa_mesh_get_gradients(const cfsa_mesh *m, double *out);          /* nt x 4 x 3 */
+int cfsa_mesh_get_tet_edge_nodes(const cfsa_mesh *m, int32_t *out);    /* nt x 6 */
+int cfsa_mesh_get_edge_nodes(const cfsa_mesh *m, double *out);         /* ne x 3 */
+int cfsa_mesh_get_edge_endpoints(const cfsa_mesh *m, int32_t *out);    /* ne x 2 */
+int cfsa_mesh_get_p1_bc(const cfsa_mesh *m, uint8_t *out);             /* nv */
+int cfsa_mesh_get_p2_bc(const cfsa_mesh *m, uint8_t *out);             /* ne */
+/* faces in sorted-triple order: verts nf x 3; tets nf x 4 (tet_a, tet_b,
+ * face_id_a, face_id_b); geom nf x 7 (normal, area, center); any may be NULL */
+int cfsa_mesh_get_faces(const cfsa_mesh *m, int32_t *verts, int32_t *tets, double *geom);
+int cfsa_mesh_get_tet_faces(const cfsa_mesh *m, int32_t *faceids, int32_t *neighbors);
+int cfsa_mesh_get_min_edge_length(const cfsa_mesh *m, double *out);
+/* io::exportToVtk(filename, mesh, scale) (VtkExport.cpp:49-84) */
+int cfsa_mesh_export_vtk(const cfsa_mesh *m, const char *path, double scale);
+void cfsa_mesh_destroy(cfsa_mesh *m);
+
+/* ------------------------------------------------------------------------
+ * Navier-Stokes PISO solver (reference NavierStokesSolver)
+ * ---------------------------------------------------------------------- */
+int cfsa_ns_create(cfsa_ns **out);
+void cfsa_ns_destroy(cfsa_ns *h);
+/* NavierStokesSolver::initialize(mesh_ptr, fluid) (NavierStokes.hpp:91-92,
+ * NavierStokes.cpp:10-68).  The mesh handle must outlive the solver: like the
+ * reference's shared_ptr<Mesh>, it is re-read by build_sparse_matrices. */
+int cfsa_ns_initialize(cfsa_ns *h, const cfsa_mesh *mesh, double density, double viscosity);
+/* Same from raw arrays (for FFI callers that own their mesh). */
+int cfsa_ns_initialize_arrays(cfsa_ns *h, int64_t nv, const double *coords, int64_t nt,
+                              const int32_t *tets, int64_t ne, const int32_t *tet_edge_nodes,
+                              const uint8_t *p1_bc, const uint8_t *p2_bc, double density,
+                              double viscosity);
+/* Push moved vertices and/or new labels (NULL = unchanged).  Labels and
+ * element geometry take effect at the next step, the operators M, K, G, L at
+ * the next build_sparse_matrices, as in the reference where they are read
+ * live from the shared mesh. */
+int cfsa_ns_update_mesh(cfsa_ns *h, const double *coords, const uint8_t *p1_bc,
+                        const uint8_t *p2_bc);
+/* NavierStokesSolver::build_sparse_matrices (NavierStokes.hpp:97-102) */
+int cfsa_ns_build_sparse_matrices(cfsa_ns *h);
+/* NavierStokesSolver::pisoStep (NavierStokes.cpp:374-414); CFSA_OK == true */
+int cfsa_ns_piso_step(cfsa_ns *h);
+/* NavierStokesSolver::simpleStep (NavierStokes.cpp:968-1009) */
+int cfsa_ns_simple_step(cfsa_ns *h);
+
+/* setters, NavierStokes.hpp:167-186 */
+int cfsa_ns_set_mean_inlet_velocity(cfsa_ns *h, const double v[3]);
+/* per-velocity-node inlet values nu x 3 (extension; NULL restores uniform) */
+int cfsa_ns_set_inlet_profile(cfsa_ns *h, const double *g);
+int cfsa_ns_set_outlet_type(cfsa_ns *h, int type);
+int cfsa_ns_set_outlet_pressure(cfsa_ns *h, double p);
+int cfsa_ns_set_inlet_type(cfsa_ns *h, int type);
+int cfsa_ns_set_number_of_corrections(cfsa_ns *h, int n);
+int cfsa_ns_set_relax_u(cfsa_ns *h, double r);
+int cfsa_ns_set_relax_p(cfsa_ns *h, double r);
+int cfsa_ns_set_reference_pressure_node(cfsa_ns *h, int node);
+int cfsa_ns_set_dt(cfsa_ns *h, double dt);
+int cfsa_ns_set_tolerance(cfsa_ns *h, double tol);
+int cfsa_ns_set_time(cfsa_ns *h, double t);
+int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt);
+int cfsa_ns_get_options(const cfsa_ns *h, cfsa_solver_options *opt);
+
+/* getters, NavierStokes.hpp:146-189 */
+/* out[0..5] = nv, nt, ne, nu, nnz of the P2 pattern, nnz of the P1 pattern */
+int cfsa_ns_get_sizes(const cfsa_ns *h, int64_t out[6]);
+int cfsa_ns_get_velocity(cfsa_ns *h, double *out);         /* nu x 3 */
+int cfsa_ns_set_velocity(cfsa_ns *h, const double *in);    /* nu x 3 */
+int cfsa_ns_get_velocity_star(cfsa_ns *h, double *out);    /* nu x 3 */
+int cfsa_ns_get_pressure(cfsa_ns *h, double *out);         /* nv */
+int cfsa_ns_set_pressure(cfsa_ns *h, const double *in);    /* nv */
+int cfsa_ns_get_pressure_correction(cfsa_ns *h, double *out);
+int cfsa_ns_get_velocity_at_node(cfsa_ns *h, int64_t node, double out[3]);
+int cfsa_ns_get_pressure_at_node(cfsa_ns *h, int64_t node, double *out);
+/* which: 0 time, 1 dt, 2 viscosity, 3 density, 4 tolerance, 5 outlet pressure */
+int cfsa_ns_get_scalar(const cfsa_ns *h, int which, double *out);
+int cfsa_ns_get_reference_pressure_node(const cfsa_ns *h, int *out);
+int cfsa_ns_has_nans(cfsa_ns *h, int *out);
+int cfsa_ns_get_step_stats(const cfsa_ns *h, cfsa_step_stats *out);
+/* io::exportToVtk(filename, ns_solver, scale) (VtkExport.cpp:238-268) */
+int cfsa_ns_export_vtk(cfsa_ns *h, const char *path, double scale);
+
+/* ---- parity / diagnostics (operators the reference builds internally) ---- */
+/* inv_lumped_mass_ (NavierStokes.cpp:290-306), nu entries */
+int cfsa_ns_get_inv_lumped_mass(cfsa_ns *h, double *out);
+/* matrix which: 0 M, 1 K, 2 momentum lhs M/dt+K with penalties, 3 L, 4 L with
+ * pressure penalties, 5 Gx, 6 Gy, 7 Gz.  rowptr n+1, col/val nnz; any may be NULL */
+int cfsa_ns_get_matrix(cfsa_ns *h, int which, int32_t *rowptr, int32_t *col, double *val);
+int cfsa_ns_get_num_colours(const cfsa_ns *h, int *out);
+int cfsa_ns_get_colouring(const cfsa_ns *h, int32_t *colour);  /* nt */
+int cfsa_ns_get_incidence(cfsa_ns *h, int32_t *ptr, int32_t *tet); /* nu+1, 10 nt */
+/* computeAdvectionRHS on the current velocity (NavierStokes.cpp:526-584) */
+int cfsa_ns_compute_advection(cfsa_ns *h, double *out);
+/* computeDivergence(u, div) (NavierStokes.cpp:670-700); u is nu x 3 */
+int cfsa_ns_compute_divergence(cfsa_ns *h, const double *u, double *out);
+/* y = A x; nrhs 1 or 3 (interleaved); for which 5..7 y = [Gx x, Gy x, Gz x] */
+int cfsa_ns_apply_matrix(cfsa_ns *h, int which, const double *x, double *y, int nrhs);
+/* solve matrix 2 (nrhs 1 or 3) or 4 (nrhs 1) with method 0 BiCGSTAB / 1 PCG */
+int cfsa_ns_krylov_solve(cfsa_ns *h, int which, int method, int nrhs, const double *b,
+                         double *x, double tol, int maxit, int precision,
+                         cfsa_solve_result *res);
+/* mean duration (ms) of the momentum-matrix SpMV kernel over `reps` launches */
+int cfsa_ns_time_momentum_spmv(cfsa_ns *h, int reps, int precision, double *ms);
+/* the hipStream_t all solver work is issued on */
+int cfsa_ns_get_stream(const cfsa_ns *h, void **stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFSA_NS_H */

@@ -1,0 +1,249 @@
+"""GPU parity: every PISO building block and whole steps of the HIP library
+against the CPU oracle on the same mesh and inputs.
+
+Bars: integer work (patterns, colouring, incidence) bit-exact; kernels whose
+floating-point order reproduces the reference (SpMV, convection, divergence)
+bit-exact; assembled matrix values within 1e-13 relative (the coloured
+scatter sums element contributions in colour order instead of tet order);
+Krylov solves and whole steps: equal iteration counts and velocity /
+pressure relative L2 error <= 1e-8 per step (the FP64 tolerance of the
+step parity requirement).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BLOOD = dict(rho=1060.0, mu=0.004)
+
+
+def relerr(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def make_case(P, O, kind, n=4, rho=1060.0, mu=0.004, dt=0.01, uin=(0.5, 0.0, 0.0),
+              ncorr=1, outlet=0, p_out=0.0, relax=(1.0, 1.0), pressure_solver=0,
+              momentum_tol=1e-6, pressure_tol=1e-8, inlet_before_init=False):
+    if kind == "kuhn":
+        pm = P.Mesh.structuredRectangularPrism(1.0, 1.0, 1.0, n, n, n)
+    elif kind == "box":
+        pm = P.Mesh.structuredRectangularPrism(2.0, 1.0, 1.0, 2 * n, n, n)
+    else:
+        pm = P.Mesh.cylinder(10.0, 1.0, 16, 0.1)
+    om = O.OracleMesh.fromArrays(pm.vertices(), pm.tets())
+    pm.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+    om.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+    g = P.NavierStokesSolver()
+    inlet0 = uin if inlet_before_init else (0.0, 0.0, 0.0)
+    if inlet_before_init:
+        g.setMeanInletVelocity(uin)
+    g.setOutletType(outlet)
+    g.initialize(pm, rho, mu)
+    g.setDt(dt)
+    g.setMeanInletVelocity(uin)
+    g.setOutletPressure(p_out)
+    g.setNumberOfCorrections(ncorr)
+    g.setRelaxU(relax[0])
+    g.setRelaxP(relax[1])
+    g.setOptions(momentum_tol=momentum_tol, pressure_tol=pressure_tol,
+                 pressure_solver=pressure_solver)
+    o = O.OracleNS(om, rho, mu, inlet0) if outlet == 0 else None
+    if o is None:
+        # the oracle picks the Neumann reference node at initialize, like the reference
+        o = _oracle_neumann(O, om, rho, mu, inlet0)
+    o.setDt(dt)
+    o.setMeanInletVelocity(uin)
+    o.setOutlet(outlet, p_out)
+    o.setNumberOfCorrections(ncorr)
+    o.setRelax(*relax)
+    o.setSolver(momentum_tol, pressure_tol, 1000, pressure_solver)
+    return g, o, pm, om
+
+
+def _oracle_neumann(O, om, rho, mu, inlet0):
+    o = O.OracleNS(om, rho, mu, inlet0)
+    lab = om.p1BC()
+    ref = int(np.nonzero(lab == 3)[0][0])
+    o.setReferencePressureNode(ref)
+    return o
+
+
+@pytest.fixture(scope="module")
+def kuhn4(P, O):
+    return make_case(P, O, "kuhn", 4)
+
+
+@pytest.fixture(scope="module")
+def cyl(P, O):
+    return make_case(P, O, "cyl")
+
+
+# ------------------------------------------------------------------ operators
+@pytest.mark.parametrize("case", ["kuhn4", "cyl"])
+def test_patterns_values_colouring(P, O, case, request):
+    g, o, pm, om = request.getfixturevalue(case)
+    for name in ["M", "K", "lhs", "L", "Lbc", "Gx", "Gy", "Gz"]:
+        rg, cg, vg = g.getMatrix(name)
+        ro, co, vo = o.getMatrix(name)
+        assert np.array_equal(rg, ro) and np.array_equal(cg, co), name
+        scale = np.abs(vo).max()
+        np.testing.assert_allclose(vg, vo, rtol=1e-13, atol=1e-15 * scale, err_msg=name)
+    np.testing.assert_allclose(g.getInvLumpedMass(), o.getInvLumpedMass(), rtol=1e-13)
+    col, nc = O.colour_tets(pm.tets(), pm.nVertices())
+    assert np.array_equal(g.colouring(), col) and g.numColours() == nc
+    ptr, tet = g.incidence()
+    T = np.c_[pm.tets(), pm.nVertices() + pm.tetEdgeNodes()]
+    for node in np.random.default_rng(0).integers(0, len(ptr) - 1, 50):
+        want = np.nonzero((T == node).any(axis=1))[0]
+        assert np.array_equal(tet[ptr[node]:ptr[node + 1]], want)
+
+
+@pytest.mark.parametrize("case", ["kuhn4", "cyl"])
+def test_spmv_bit_exact(P, O, case, request):
+    g, o, pm, om = request.getfixturevalue(case)
+    rng = np.random.default_rng(0)
+    sz = g.sizes()
+    for name, nrhs in [("lhs", 3), ("M", 3), ("K", 1), ("L", 1), ("Lbc", 1)]:
+        n = sz["nu"] if name in ("M", "K", "lhs") else sz["nv"]
+        x = rng.uniform(-1, 1, (n, 3) if nrhs == 3 else n)
+        rp, col, val = g.getMatrix(name)
+        assert np.array_equal(g.applyMatrix(name, x, nrhs), O.csr_spmv(rp, col, val, x)), name
+    p = rng.uniform(-1, 1, sz["nv"])
+    y = g.applyMatrix("Gx", p)
+    for c, name in enumerate(["Gx", "Gy", "Gz"]):
+        rp, col, val = g.getMatrix(name)
+        assert np.array_equal(y[:, c], O.csr_spmv(rp, col, val, p)), name
+
+
+@pytest.mark.parametrize("case", ["kuhn4", "cyl"])
+def test_advection_divergence_bit_exact(P, O, case, request):
+    g, o, pm, om = request.getfixturevalue(case)
+    u = np.random.default_rng(1).uniform(-1, 1, (g.sizes()["nu"], 3))
+    g.setVelocity(u)
+    o.setVelocity(u)
+    assert np.array_equal(g.computeAdvectionRHS(), o.computeAdvectionRHS())
+    assert np.array_equal(g.computeDivergence(u), o.computeDivergence(u))
+
+
+@pytest.mark.parametrize("case", ["kuhn4", "cyl"])
+def test_krylov_parity(P, O, case, request):
+    g, o, pm, om = request.getfixturevalue(case)
+    sz = g.sizes()
+    rng = np.random.default_rng(2)
+    b = rng.uniform(-1, 1, (sz["nu"], 3))
+    x, info = g.krylovSolve("lhs", b, method=0, tol=1e-10)
+    for c in range(3):
+        xo, io = o.krylovSolve("lhs", b[:, c], method=0, tol=1e-10)
+        assert info["iters"][c] == io["iters"], (c, info, io)
+        assert relerr(x[:, c], xo) < 1e-12
+    bp = rng.uniform(-1, 1, sz["nv"])
+    for method in (0, 1):
+        xp, ip = g.krylovSolve("Lbc", bp, method=method, tol=1e-9)
+        xo, io = o.krylovSolve("Lbc", bp, method=method, tol=1e-9)
+        assert ip["iters"][0] == io["iters"], (method, ip, io)
+        assert relerr(xp, xo) < 1e-11
+    x0, i0 = g.krylovSolve("Lbc", np.zeros(sz["nv"]))
+    assert i0["iters"][0] == 0 and not x0.any()
+
+
+# ------------------------------------------------------------------ steps
+STEP_CASES = {
+    "kuhn4": dict(kind="kuhn", n=4, steps=5),
+    "kuhn6_re100": dict(kind="kuhn", n=6, rho=1.0, mu=0.01, dt=1e-3, uin=(1.0, 0, 0), steps=3),
+    "box3_ncorr2_relax": dict(kind="box", n=3, ncorr=2, relax=(0.5, 0.5), steps=4),
+    "kuhn4_neumann": dict(kind="kuhn", n=4, outlet=1, steps=3),
+    "kuhn4_cg_pout": dict(kind="kuhn", n=4, pressure_solver=1, p_out=3.0, steps=3),
+    "kuhn4_inlet_first": dict(kind="kuhn", n=4, inlet_before_init=True, steps=3),
+    "cyl_config1": dict(kind="cyl", steps=20, pressure_solver=1),
+    "cyl_bicgstab": dict(kind="cyl", steps=5),
+}
+
+
+@pytest.mark.parametrize("name", list(STEP_CASES))
+def test_piso_step_parity(P, O, name):
+    spec = dict(STEP_CASES[name])
+    steps = spec.pop("steps")
+    g, o, pm, om = make_case(P, O, **spec)
+    for s in range(steps):
+        assert g.pisoStepStatus() == 0, (name, s, P.last_error())
+        assert o.pisoStep() == 0, (name, s)
+        sg, so = g.stepStats(), o.stats()
+        assert sg["momentum_iters"] == so["momentum_iters"], (name, s, sg, so)
+        nc = spec.get("ncorr", 1)
+        assert sg["pressure_iters"] == so["pressure_iters"][:nc], (name, s, sg, so)
+        eu = relerr(g.getVelocity(), o.getVelocity())
+        ep = relerr(g.getPressure(), o.getPressure())
+        assert eu <= 1e-8 and ep <= 1e-8, (name, s, eu, ep)
+    assert abs(g.getTime() - steps * spec.get("dt", 0.01)) < 1e-12
+
+
+def test_simple_step_parity(P, O):
+    g, o, pm, om = make_case(P, O, "kuhn", 4, p_out=2.0)
+    for _ in range(3):
+        assert g.simpleStep() and o.simpleStep() == 0
+        assert relerr(g.getVelocity(), o.getVelocity()) <= 1e-8
+        assert relerr(g.getPressure(), o.getPressure()) <= 1e-8
+
+
+def test_inlet_profile_parity(P, O):
+    g, o, pm, om = make_case(P, O, "kuhn", 4)
+    X = np.vstack([pm.vertices(), pm.edgeNodes()])
+    prof = np.zeros((len(X), 3))
+    prof[:, 0] = 16 * X[:, 1] * (1 - X[:, 1]) * X[:, 2] * (1 - X[:, 2]) * 0.5
+    g.setInletProfile(prof)
+    o.setInletProfile(prof)
+    for _ in range(3):
+        assert g.pisoStep() and o.pisoStep() == 0
+        assert relerr(g.getVelocity(), o.getVelocity()) <= 1e-8
+    lab = np.r_[pm.p1BC(), pm.p2BC()]
+    np.testing.assert_array_equal(g.getVelocity()[lab == 1], prof[lab == 1])
+
+
+def test_repeat_runs_bitwise_identical(P, O):
+    out = []
+    for _ in range(2):
+        g, o, pm, om = make_case(P, O, "kuhn", 4)
+        for _ in range(3):
+            assert g.pisoStep()
+        out.append((g.getVelocity(), g.getPressure()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+# ------------------------------------------------------------------ failures
+def test_step_before_initialize(P):
+    g = P.NavierStokesSolver()
+    assert g.pisoStepStatus() == 1
+    assert "not initialized" in P.last_error()
+
+
+def test_momentum_noconvergence_reported(P, O):
+    g, o, pm, om = make_case(P, O, "kuhn", 4)
+    g.setOptions(max_iter=1)
+    assert g.pisoStepStatus() == 2
+    assert "Momentum" in P.last_error()
+
+
+def test_nan_reported(P, O):
+    g, o, pm, om = make_case(P, O, "kuhn", 4)
+    u = g.getVelocity()
+    u[pm.p1BC() == 3] = np.nan
+    g.setVelocity(u)
+    st = g.pisoStepStatus()
+    assert st in (2, 4)
+    assert g.hasNans() or st == 2
+
+
+def test_vtk_export_matches_fields(P, O, tmp_path):
+    g, o, pm, om = make_case(P, O, "kuhn", 2)
+    assert g.pisoStep()
+    path = tmp_path / "f.vtk"
+    g.exportVtk(path, scale=1.0)
+    txt = path.read_text().splitlines()
+    i = txt.index("SCALARS pressure double 1")
+    nv = pm.nVertices()
+    p = np.array([float(x) for x in txt[i + 2:i + 2 + nv]])
+    np.testing.assert_allclose(p, g.getPressure(), rtol=1e-5, atol=1e-12)
+    j = txt.index("VECTORS velocity double")
+    u = np.array([[float(v) for v in r.split()] for r in txt[j + 1:j + 1 + nv]])
+    np.testing.assert_allclose(u, g.getVelocity()[:nv], rtol=1e-5, atol=1e-12)
