@@ -21,7 +21,7 @@ INCLUDE = PKG.parent / "include"
 LIB = LIBDIR / "libcfsa_ns.so"
 DRIVER = LIBDIR / "cfsa_simulation"
 
-HIP_SOURCES = ["kernels.hip", "solver.hip", "capi.cpp"]
+HIP_SOURCES = ["kernels.hip", "solver.hip", "adjoint.hip", "capi.cpp"]
 HOST_SOURCES = ["mesh.cpp", "pattern.cpp", "vtk.cpp", "yaml_config.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -67,10 +67,9 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     driver_src = CSRC / "simulation_main.cpp"
     if driver_src.exists():
-        _run([CXX, *HOST_FLAGS, str(driver_src), str(CSRC / "NavierStokesSolver.cpp"),
-              "-o", str(DRIVER), f"-L{LIBDIR}", "-lcfsa_ns", f"-Wl,-rpath,{LIBDIR}",
-              "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib",
-              "-lamdhip64"])
+        _run([CXX, *HOST_FLAGS, str(driver_src), "-o", str(DRIVER), f"-L{LIBDIR}",
+              "-lcfsa_ns", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib",
+              "-Wl,-rpath,/opt/rocm/lib", "-lamdhip64"])
     if verbose:
         print(f"built {LIB}")
     return LIB

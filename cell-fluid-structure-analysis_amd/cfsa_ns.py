@@ -36,7 +36,8 @@ class SolverOptions(C.Structure):
     _fields_ = [("momentum_tol", C.c_double), ("pressure_tol", C.c_double),
                 ("max_iter", C.c_int), ("pressure_solver", C.c_int),
                 ("precision", C.c_int), ("inner_tol", C.c_double),
-                ("max_refine", C.c_int), ("check_every", C.c_int), ("timing", C.c_int)]
+                ("max_refine", C.c_int), ("check_every", C.c_int), ("timing", C.c_int),
+                ("stop_rule", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -64,6 +65,38 @@ class StepStats(C.Structure):
 class SolveResult(C.Structure):
     _fields_ = [("iters", C.c_int * 3), ("err", C.c_double * 3), ("restarts", C.c_int * 3),
                 ("refinements", C.c_int), ("inner_iters", C.c_int), ("ok", C.c_int)]
+
+
+class SimConfig(C.Structure):
+    _fields_ = [("mesh_type", C.c_int), ("mesh_file", C.c_char * 1024), ("obj_scale", C.c_double),
+                ("cylinder_length", C.c_double), ("cylinder_radius", C.c_double),
+                ("cylinder_divisions", C.c_int), ("icosphere_radius", C.c_double),
+                ("icosphere_recursion", C.c_int), ("max_element_size", C.c_double),
+                ("flow_direction", C.c_double * 3), ("inlet_outlet_coverage", C.c_double),
+                ("box_size", C.c_double * 3), ("box_cells", C.c_int * 3),
+                ("ns_enabled", C.c_int), ("density", C.c_double), ("viscosity", C.c_double),
+                ("is_newtonian", C.c_int), ("is_laminar", C.c_int),
+                ("inlet_velocity", C.c_double * 3), ("outlet_pressure", C.c_double),
+                ("use_dirichlet_pressure", C.c_int), ("dt", C.c_double), ("relax_p", C.c_double),
+                ("relax_u", C.c_double), ("n_corrections", C.c_int),
+                ("pressure_solver", C.c_int), ("precision", C.c_int),
+                ("momentum_tol", C.c_double), ("pressure_tol", C.c_double),
+                ("inlet_profile", C.c_int), ("output_prefix", C.c_char * 512),
+                ("total_steps", C.c_int), ("output_frequency", C.c_int),
+                ("vtk_scale", C.c_double), ("output_fluid", C.c_int), ("output_solid", C.c_int),
+                ("output_dpm", C.c_int), ("solid_enabled", C.c_int), ("dpm_enabled", C.c_int),
+                ("fsi_enabled", C.c_int)]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            if isinstance(v, bytes):
+                v = v.decode()
+            elif hasattr(v, "__len__"):
+                v = list(v)
+            out[name] = v
+        return out
 
 
 def _sig(name, res, *args):
@@ -136,6 +169,26 @@ _sig("cfsa_ns_krylov_solve", C.c_int, _p, C.c_int, C.c_int, C.c_int, _f64p, _f64
      C.c_int, C.c_int, C.POINTER(SolveResult))
 _sig("cfsa_ns_time_momentum_spmv", C.c_int, _p, C.c_int, C.c_int, C.POINTER(C.c_double))
 _sig("cfsa_ns_get_stream", C.c_int, _p, C.POINTER(_p))
+_sig("cfsa_ns_set_viscosity", C.c_int, _p, C.c_double)
+_sig("cfsa_ns_adjoint_forward", C.c_int, _p, C.c_int, _f64p, _p)
+_sig("cfsa_ns_adjoint_backward", C.c_int, _p, _p, _f64p, C.POINTER(C.c_double))
+_sig("cfsa_ns_adjoint_gradient", C.c_int, _p, C.c_int, C.c_int, _f64p, _f64p,
+     C.POINTER(C.c_double), _f64p, C.POINTER(C.c_double))
+_sig("cfsa_config_load", C.c_int, C.c_char_p, C.POINTER(SimConfig))
+_sig("cfsa_config_parse", C.c_int, C.c_char_p, C.POINTER(SimConfig))
+
+
+def loadConfig(path) -> dict:
+    """config::loadConfig (src/IO/YamlConfig.cpp:200-232)."""
+    c = SimConfig()
+    _check(_lib.cfsa_config_load(str(path).encode(), C.byref(c)))
+    return c.as_dict()
+
+
+def parseConfig(text: str) -> dict:
+    c = SimConfig()
+    _check(_lib.cfsa_config_parse(text.encode(), C.byref(c)))
+    return c.as_dict()
 
 
 class CfsaError(RuntimeError):
@@ -550,3 +603,33 @@ class NavierStokesSolver:
         out = _p()
         _check(_lib.cfsa_ns_get_stream(self._h, C.byref(out)))
         return out.value
+
+    # --- adjoint (extension) ---
+    def setViscosity(self, mu):
+        _check(_lib.cfsa_ns_set_viscosity(self._h, float(mu)))
+
+    def adjointForward(self, nsteps, g):
+        g = np.ascontiguousarray(g, np.float64)
+        u = np.empty_like(g)
+        _check(_lib.cfsa_ns_adjoint_forward(self._h, int(nsteps), g, u.ctypes.data_as(_p)))
+        return u
+
+    def adjointBackward(self, lam=None):
+        nu = self.sizes()["nu"]
+        dg = np.empty((nu, 3), np.float64)
+        dmu = C.c_double()
+        keep = None
+        if lam is not None:
+            keep = np.ascontiguousarray(lam, np.float64)
+        _check(_lib.cfsa_ns_adjoint_backward(
+            self._h, None if keep is None else keep.ctypes.data_as(_p), dg, C.byref(dmu)))
+        return dg, dmu.value
+
+    def adjointGradient(self, nsteps, basis, coef):
+        basis = np.ascontiguousarray(basis, np.float64)
+        coef = np.ascontiguousarray(coef, np.float64)
+        J, dmu = C.c_double(), C.c_double()
+        dc = np.empty(len(coef), np.float64)
+        _check(_lib.cfsa_ns_adjoint_gradient(self._h, int(nsteps), len(coef), basis, coef,
+                                             C.byref(J), dc, C.byref(dmu)))
+        return J.value, dc, dmu.value

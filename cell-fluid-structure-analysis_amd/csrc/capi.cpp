@@ -10,6 +10,7 @@
 #include "mesh.hpp"
 #include "solver.hpp"
 #include "vtk.hpp"
+#include "yaml_config.hpp"
 
 struct cfsa_mesh {
   cfsa::Mesh m;
@@ -63,6 +64,7 @@ cfsa::SolverOptions to_opts(const cfsa_solver_options &o) {
   s.max_refine = o.max_refine;
   s.check_every = o.check_every;
   s.timing = o.timing;
+  s.stop_scaled = o.stop_rule;
   return s;
 }
 
@@ -448,7 +450,8 @@ int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt) {
   CHECK_PTR(opt);
   if (opt->max_iter < 0 || opt->check_every < 1 || opt->precision < 0 || opt->precision > 1 ||
       opt->pressure_solver < 0 || opt->pressure_solver > 1 || !(opt->momentum_tol >= 0) ||
-      !(opt->pressure_tol >= 0) || !(opt->inner_tol > 0) || opt->max_refine < 0)
+      !(opt->pressure_tol >= 0) || !(opt->inner_tol > 0) || opt->max_refine < 0 ||
+      opt->stop_rule < 0 || opt->stop_rule > 1)
     return fail(CFSA_ERR_INVALID, "invalid solver options");
   h->s.opt = to_opts(*opt);
   return CFSA_OK;
@@ -466,6 +469,7 @@ int cfsa_ns_get_options(const cfsa_ns *h, cfsa_solver_options *o) {
   o->max_refine = s.max_refine;
   o->check_every = s.check_every;
   o->timing = s.timing;
+  o->stop_rule = s.stop_scaled;
   return CFSA_OK;
 }
 
@@ -663,6 +667,133 @@ int cfsa_ns_get_stream(const cfsa_ns *h, void **stream) {
   CHECK_PTR(stream);
   *stream = (void *)h->s.stream();
   return CFSA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint
+// ---------------------------------------------------------------------------
+int cfsa_ns_set_viscosity(cfsa_ns *h, double viscosity) {
+  CHECK_PTR(h);
+  if (int e = check_init(h)) return e;
+  if (!(viscosity > 0)) return fail(CFSA_ERR_INVALID, "viscosity must be > 0");
+  return guard([&]() -> int {
+    h->s.set_viscosity(viscosity);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_adjoint_forward(cfsa_ns *h, int nsteps, const double *g, double *u_final) {
+  CHECK_PTR(h);
+  CHECK_PTR(g);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    sync_mesh(h);
+    const int st = h->s.adjoint_forward(nsteps, g, u_final);
+    if (st != CFSA_OK) g_last_error = h->s.last_error;
+    return st;
+  });
+}
+int cfsa_ns_adjoint_backward(cfsa_ns *h, const double *lambda_u, double *dLdg,
+                             double *dLdviscosity) {
+  CHECK_PTR(h);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    const int st = h->s.adjoint_backward(lambda_u, dLdg, dLdviscosity);
+    if (st != CFSA_OK) g_last_error = h->s.last_error;
+    return st;
+  });
+}
+int cfsa_ns_adjoint_gradient(cfsa_ns *h, int nsteps, int nk, const double *basis,
+                             const double *coef, double *J, double *dJdcoef,
+                             double *dJdviscosity) {
+  CHECK_PTR(h);
+  CHECK_PTR(basis);
+  CHECK_PTR(coef);
+  CHECK_PTR(dJdcoef);
+  if (int e = check_init(h)) return e;
+  if (nk < 1) return fail(CFSA_ERR_INVALID, "nk >= 1");
+  return guard([&]() -> int {
+    sync_mesh(h);
+    const int st = h->s.adjoint_gradient(nsteps, nk, basis, coef, J, dJdcoef, dJdviscosity);
+    if (st != CFSA_OK) g_last_error = h->s.last_error;
+    return st;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Configuration
+// ---------------------------------------------------------------------------
+static void to_c_config(const cfsa::SimulationConfig &c, cfsa_sim_config *o) {
+  std::memset(o, 0, sizeof(*o));
+  const auto &m = c.mesh;
+  o->mesh_type = m.type;
+  const std::string &file = m.type == 4 ? m.msh_file_path : m.obj_file_path;
+  std::strncpy(o->mesh_file, file.c_str(), sizeof(o->mesh_file) - 1);
+  o->obj_scale = m.obj_scale;
+  o->cylinder_length = m.cylinder_length;
+  o->cylinder_radius = m.cylinder_radius;
+  o->cylinder_divisions = m.cylinder_divisions;
+  o->icosphere_radius = m.icosphere_radius;
+  o->icosphere_recursion = m.icosphere_recursion;
+  o->max_element_size = m.max_element_size;
+  for (int i = 0; i < 3; ++i) {
+    o->flow_direction[i] = m.flow_direction[i];
+    o->box_size[i] = m.box_size[i];
+    o->box_cells[i] = m.box_cells[i];
+  }
+  o->inlet_outlet_coverage = m.inlet_outlet_coverage;
+  const auto &n = c.navier_stokes;
+  o->ns_enabled = n.enabled;
+  o->density = n.density;
+  o->viscosity = n.viscosity;
+  o->is_newtonian = n.is_newtonian;
+  o->is_laminar = n.is_laminar;
+  for (int i = 0; i < 3; ++i) o->inlet_velocity[i] = n.inlet_velocity[i];
+  o->outlet_pressure = n.outlet_pressure;
+  o->use_dirichlet_pressure = n.use_dirichlet_pressure;
+  o->dt = n.dt;
+  o->relax_p = n.relax_p;
+  o->relax_u = n.relax_u;
+  o->n_corrections = n.n_corrections;
+  o->pressure_solver = n.pressure_solver;
+  o->precision = n.precision;
+  o->momentum_tol = n.momentum_tol;
+  o->pressure_tol = n.pressure_tol;
+  o->inlet_profile = n.inlet_profile;
+  const auto &out = c.output;
+  std::strncpy(o->output_prefix, out.output_prefix.c_str(), sizeof(o->output_prefix) - 1);
+  o->total_steps = out.total_steps;
+  o->output_frequency = out.output_frequency;
+  o->vtk_scale = out.vtk_scale;
+  o->output_fluid = out.output_fluid;
+  o->output_solid = out.output_solid;
+  o->output_dpm = out.output_dpm;
+  o->solid_enabled = c.solid_enabled;
+  o->dpm_enabled = c.dpm_enabled;
+  o->fsi_enabled = c.fsi_enabled;
+}
+
+int cfsa_config_load(const char *path, cfsa_sim_config *out) {
+  CHECK_PTR(path);
+  CHECK_PTR(out);
+  try {
+    to_c_config(cfsa::load_config(path), out);
+    return CFSA_OK;
+  } catch (const std::exception &e) {
+    return fail(std::string(e.what()).find("bad file") != std::string::npos ? CFSA_ERR_IO
+                                                                             : CFSA_ERR_INVALID,
+                e.what());
+  }
+}
+
+int cfsa_config_parse(const char *text, cfsa_sim_config *out) {
+  CHECK_PTR(text);
+  CHECK_PTR(out);
+  try {
+    to_c_config(cfsa::parse_config(text), out);
+    return CFSA_OK;
+  } catch (const std::exception &e) {
+    return fail(CFSA_ERR_INVALID, e.what());
+  }
 }
 
 }  // extern "C"

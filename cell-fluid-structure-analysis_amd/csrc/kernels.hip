@@ -363,6 +363,101 @@ void spmv_grad3(hipStream_t s, const DevCsr &P, const double *gval, const double
   CFSA_LAUNCH_CHECK();
 }
 
+// y = beta y + alpha sum_c G_c^T x_c with transposed values gT and x of
+// stride xs (the first 3 entries of each node record are the components)
+struct OpGradT3 : Mask1Base {
+  const int *__restrict__ col;
+  const double *__restrict__ g;
+  const double *__restrict__ x;
+  double *__restrict__ y;
+  int xs;
+  double alpha, beta;
+  __device__ void prod(int k, double *p) const {
+    const double *xj = x + (size_t)col[k] * xs;
+    const double *gk = g + (size_t)k * 3;
+    p[0] = gk[0] * xj[0] + gk[1] * xj[1] + gk[2] * xj[2];
+  }
+  __device__ void epi(int row, int, double acc, double *) const {
+    y[row] = (beta != 0.0 ? beta * y[row] : 0.0) + alpha * acc;
+  }
+};
+
+void spmv_gradT_in3(hipStream_t s, const DevCsr &P, const double *gT, const double *x, int xs,
+                    double *y, double alpha, double beta) {
+  OpGradT3 op{};
+  op.mask = nullptr;
+  op.col = P.col;
+  op.g = gT;
+  op.x = x;
+  op.y = y;
+  op.xs = xs;
+  op.alpha = alpha;
+  op.beta = beta;
+  const int G = stream_grid<double, 1>(P.n);
+  hipLaunchKernelGGL((k_csr_stream<double, 1, 0, OpGradT3>), dim3(G), dim3(kBlock), 0, s, P.n,
+                     P.rowptr, op, nullptr);
+  CFSA_LAUNCH_CHECK();
+}
+
+// y[i][c] += sum_j G_c[i,j] x[j] over the P1 rows of a [*][3] array
+struct OpGrad3Acc : OpGrad3 {
+  __device__ void epi(int row, int c, double acc, double *) const {
+    y[(size_t)row * 3 + c] += acc;
+  }
+};
+void spmv_grad3_acc(hipStream_t s, const DevCsr &P, const double *g, const double *x,
+                    double *y) {
+  OpGrad3Acc op{};
+  op.mask = nullptr;
+  op.col = P.col;
+  op.g = g;
+  op.x = x;
+  op.y = y;
+  const int G = stream_grid<double, 3>(P.n);
+  hipLaunchKernelGGL((k_csr_stream<double, 3, 0, OpGrad3Acc>), dim3(G), dim3(kBlock), 0, s,
+                     P.n, P.rowptr, op, nullptr);
+  CFSA_LAUNCH_CHECK();
+}
+
+// adjoint of the momentum right-hand side: lu = (M lb)/dt, lg += pen lb on Inlet rows
+struct OpMomAdj : MaskBase {
+  const int *__restrict__ col;
+  const double *__restrict__ M;
+  const double *__restrict__ lb;
+  const uint8_t *__restrict__ bc;
+  double *__restrict__ lu;
+  double *__restrict__ lg;
+  double dt, pen;
+  __device__ void prod(int k, double *p) const {
+    const int j = col[k];
+    const double a = M[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p[c] = a * lb[(size_t)j * 3 + c];
+  }
+  __device__ void epi(int row, int c, double acc, double *) const {
+    const size_t i = (size_t)row * 3 + c;
+    lu[i] = acc / dt;
+    if (bc[row] == kInlet) lg[i] += pen * lb[i];
+  }
+};
+void momentum_rhs_adjoint(hipStream_t s, const DevCsr &A, const double *M, const double *lb,
+                          const uint8_t *bc, double dt, double pen, double *lu, double *lg) {
+  OpMomAdj op{};
+  op.mask = nullptr;
+  op.col = A.col;
+  op.M = M;
+  op.lb = lb;
+  op.bc = bc;
+  op.lu = lu;
+  op.lg = lg;
+  op.dt = dt;
+  op.pen = pen;
+  const int G = stream_grid<double, 3>(A.n);
+  hipLaunchKernelGGL((k_csr_stream<double, 3, 0, OpMomAdj>), dim3(G), dim3(kBlock), 0, s, A.n,
+                     A.rowptr, op, nullptr);
+  CFSA_LAUNCH_CHECK();
+}
+
 void momentum_rhs(hipStream_t s, const DevCsr &A, const double *M, const double *u,
                   const double *gp, const double *conv, const uint8_t *bc,
                   const double *ginlet, double dt, int nv, double penalty,
@@ -462,13 +557,16 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(int n, const T *__restrict
                                                       T *__restrict__ x, T *__restrict__ r,
                                                       T *__restrict__ r0, T *__restrict__ p,
                                                       T *__restrict__ v,
+                                                      const T *__restrict__ sc,
                                                       double *__restrict__ partials) {
+  // partials: [2c] = stopping norm of b (|b|^2, or |D^-1 b|^2 when sc != null),
+  //           [2c+1] = |b|^2 = r0.r of the first iteration
   double acc[2 * K];
 #pragma unroll
   for (int i = 0; i < 2 * K; ++i) acc[i] = 0.0;
-  const size_t N = (size_t)n * K;
   for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
        e += (size_t)gridDim.x * kBlock) {
+    const double de = sc ? (double)sc[e] : 1.0;
 #pragma unroll
     for (int c = 0; c < K; ++c) {
       const size_t i = e * K + c;
@@ -479,11 +577,11 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(int n, const T *__restrict
       p[i] = T(0);
       v[i] = T(0);
       const double d = (double)bi * (double)bi;
-      acc[2 * c] += d;
+      const double ds = de * (double)bi;
+      acc[2 * c] += sc ? ds * ds : d;
       acc[2 * c + 1] += d;
     }
   }
-  (void)N;
   block_partials<2 * K>(acc, partials);
 }
 
@@ -496,7 +594,7 @@ __global__ void k_bicg_s_init(KrylovState *st, const double *partials, int G) {
   for (int c = 0; c < K; ++c) {
     const double bb = tot[2 * c];
     st->rhs_sq[c] = bb;
-    st->r0_sq[c] = bb;
+    st->r0_sq[c] = tot[2 * c + 1];
     st->rr[c] = bb;
     st->rho[c] = 1.0;
     st->alpha[c] = 1.0;
@@ -672,7 +770,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v3(
     int n, const KrylovState *__restrict__ st, const T *__restrict__ y,
     const T *__restrict__ z, const T *__restrict__ sv, const T *__restrict__ t,
     const T *__restrict__ r0, T *__restrict__ x, T *__restrict__ r,
-    double *__restrict__ partials) {
+    const T *__restrict__ sc, double *__restrict__ partials) {
   if (!st->any_active) return;
   int act[K];
   double al[K], om[K];
@@ -687,6 +785,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v3(
   }
   for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
        e += (size_t)gridDim.x * kBlock) {
+    const double de = sc ? (double)sc[e] : 1.0;
 #pragma unroll
     for (int c = 0; c < K; ++c) {
       if (!act[c]) continue;
@@ -694,7 +793,8 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v3(
       x[i] = (T)((double)x[i] + (al[c] * (double)y[i] + om[c] * (double)z[i]));
       const T ri = (T)((double)sv[i] - om[c] * (double)t[i]);
       r[i] = ri;
-      acc[2 * c] += (double)ri * (double)ri;
+      const double rs = de * (double)ri;
+      acc[2 * c] += sc ? rs * rs : (double)ri * (double)ri;
       acc[2 * c + 1] += (double)r0[i] * (double)ri;
     }
   }
@@ -703,11 +803,11 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v3(
 
 template <class T, int K>
 void bicg_init(hipStream_t s, int n, const T *b, T *x, T *r, T *r0, T *p, T *v,
-               double *partials, int *grid) {
+               const T *stop_scale, double *partials, int *grid) {
   const int G = vec_grid(n);
   *grid = G;
   hipLaunchKernelGGL((k_bicg_init<T, K>), dim3(G), dim3(kBlock), 0, s, n, b, x, r, r0, p,
-                     v, partials);
+                     v, stop_scale, partials);
   CFSA_LAUNCH_CHECK();
 }
 template <int K>
@@ -753,24 +853,26 @@ void bicg_v2(hipStream_t s, int n, const KrylovState *st, const T *r, const T *v
 }
 template <class T, int K>
 void bicg_v3(hipStream_t s, int n, const KrylovState *st, const T *y, const T *z,
-             const T *sv, const T *t, const T *r0, T *x, T *r, double *partials,
+             const T *sv, const T *t, const T *r0, T *x, T *r, const T *stop_scale,
+             double *partials,
              int *grid) {
   const int G = vec_grid(n);
   *grid = G;
   hipLaunchKernelGGL((k_bicg_v3<T, K>), dim3(G), dim3(kBlock), 0, s, n, st, y, z, sv, t, r0,
-                     x, r, partials);
+                     x, r, stop_scale, partials);
   CFSA_LAUNCH_CHECK();
 }
 
 #define CFSA_INST_BICG(T, K)                                                                \
-  template void bicg_init<T, K>(hipStream_t, int, const T *, T *, T *, T *, T *, T *,        \
+  template void bicg_init<T, K>(hipStream_t, int, const T *, T *, T *, T *, T *, T *, const T *, \
                                 double *, int *);                                           \
   template void bicg_v1<T, K>(hipStream_t, int, const KrylovState *, const T *, T *,        \
                               const T *, const T *, T *);                                   \
   template void bicg_v2<T, K>(hipStream_t, int, const KrylovState *, const T *, const T *,  \
                               const T *, T *, T *);                                         \
   template void bicg_v3<T, K>(hipStream_t, int, const KrylovState *, const T *, const T *,  \
-                              const T *, const T *, const T *, T *, T *, double *, int *);
+                              const T *, const T *, const T *, T *, T *, const T *, double *, \
+                              int *);
 CFSA_INST_BICG(double, 1)
 CFSA_INST_BICG(double, 3)
 CFSA_INST_BICG(float, 1)
@@ -792,7 +894,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_init(int n, const T *__restrict__ b,
                                                     const T *__restrict__ invd,
                                                     T *__restrict__ x, T *__restrict__ r,
-                                                    T *__restrict__ p,
+                                                    T *__restrict__ p, int scaled,
                                                     double *__restrict__ partials) {
   double acc[2] = {0.0, 0.0};
   for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n;
@@ -802,7 +904,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int n, const T *__restrict__
     r[i] = bi;
     const T pi = (T)((double)invd[i] * (double)bi);
     p[i] = pi;
-    acc[0] += (double)bi * (double)bi;
+    acc[0] += scaled ? (double)pi * (double)pi : (double)bi * (double)bi;
     acc[1] += (double)bi * (double)pi;
   }
   block_partials<2>(acc, partials);
@@ -880,7 +982,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__re
                                                   const T *__restrict__ q,
                                                   const T *__restrict__ invd,
                                                   T *__restrict__ x, T *__restrict__ r,
-                                                  T *__restrict__ z,
+                                                  T *__restrict__ z, int scaled,
                                                   double *__restrict__ partials) {
   if (!st->active[0]) return;
   const double al = st->alpha[0];
@@ -892,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__re
     r[i] = ri;
     const T zi = (T)((double)invd[i] * (double)ri);
     z[i] = zi;
-    acc[0] += (double)ri * (double)ri;
+    acc[0] += scaled ? (double)zi * (double)zi : (double)ri * (double)ri;
     acc[1] += (double)ri * (double)zi;
   }
   block_partials<2>(acc, partials);
@@ -918,11 +1020,11 @@ __global__ void k_cg_s_end(KrylovState *st) {
 }
 
 template <class T>
-void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p,
+void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p, int scaled,
              double *partials, int *grid) {
   const int G = vec_grid(n);
   *grid = G;
-  hipLaunchKernelGGL((k_cg_init<T>), dim3(G), dim3(kBlock), 0, s, n, b, invd, x, r, p,
+  hipLaunchKernelGGL((k_cg_init<T>), dim3(G), dim3(kBlock), 0, s, n, b, invd, x, r, p, scaled,
                      partials);
   CFSA_LAUNCH_CHECK();
 }
@@ -940,11 +1042,11 @@ void cg_s_beta(hipStream_t s, KrylovState *st, const double *partials, int G) {
 }
 template <class T>
 void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
-           const T *invd, T *x, T *r, T *z, double *partials, int *grid) {
+           const T *invd, T *x, T *r, T *z, int scaled, double *partials, int *grid) {
   const int G = vec_grid(n);
   *grid = G;
   hipLaunchKernelGGL((k_cg_v1<T>), dim3(G), dim3(kBlock), 0, s, n, st, p, q, invd, x, r, z,
-                     partials);
+                     scaled, partials);
   CFSA_LAUNCH_CHECK();
 }
 template <class T>
@@ -956,10 +1058,10 @@ void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *z, T *p) {
   CFSA_LAUNCH_CHECK();
 }
 #define CFSA_INST_CG(T)                                                                   \
-  template void cg_init<T>(hipStream_t, int, const T *, const T *, T *, T *, T *, double *, \
-                           int *);                                                        \
+  template void cg_init<T>(hipStream_t, int, const T *, const T *, T *, T *, T *, int,      \
+                           double *, int *);                                              \
   template void cg_v1<T>(hipStream_t, int, const KrylovState *, const T *, const T *,     \
-                         const T *, T *, T *, T *, double *, int *);                      \
+                         const T *, T *, T *, T *, int, double *, int *);                 \
   template void cg_v2<T>(hipStream_t, int, const KrylovState *, const T *, T *);
 CFSA_INST_CG(double)
 CFSA_INST_CG(float)

@@ -53,11 +53,22 @@ void momentum_rhs(hipStream_t s, const DevCsr &A, const double *M, const double 
                   const double *ginlet, double dt, int nv, double penalty,
                   double *rhs);
 
+// adjoint building blocks (transposed G values gT on the symmetric P1 pattern)
+// y = beta y + alpha sum_c G_c^T x_c   (x of stride xs, components 0..2)
+void spmv_gradT_in3(hipStream_t s, const DevCsr &P, const double *gT, const double *x, int xs,
+                    double *y, double alpha, double beta);
+// y[i][c] += sum_j G_c[i,j] x[j] on the P1 rows of a [*][3] array
+void spmv_grad3_acc(hipStream_t s, const DevCsr &P, const double *g, const double *x,
+                    double *y);
+// lu = (M lb)/dt; lg += pen lb on Inlet rows
+void momentum_rhs_adjoint(hipStream_t s, const DevCsr &A, const double *M, const double *lb,
+                          const uint8_t *bc, double dt, double pen, double *lu, double *lg);
+
 // ---- Krylov vector / scalar kernels ----
 int vec_grid(int n);
 template <class T, int K>
 void bicg_init(hipStream_t s, int n, const T *b, T *x, T *r, T *r0, T *p, T *v,
-               double *partials, int *grid);
+               const T *stop_scale, double *partials, int *grid);
 template <int K>
 void bicg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G);
 template <int K>
@@ -77,18 +88,18 @@ void bicg_v2(hipStream_t s, int n, const KrylovState *st, const T *r, const T *v
              const T *invd, T *sv, T *z);
 template <class T, int K>
 void bicg_v3(hipStream_t s, int n, const KrylovState *st, const T *y, const T *z,
-             const T *sv, const T *t, const T *r0, T *x, T *r, double *partials,
-             int *grid);
+             const T *sv, const T *t, const T *r0, T *x, T *r, const T *stop_scale,
+             double *partials, int *grid);
 // Jacobi PCG (single right-hand side)
 template <class T>
-void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p,
+void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p, int scaled,
              double *partials, int *grid);
 void cg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G);
 void cg_s_alpha(hipStream_t s, KrylovState *st, const double *partials, int G);
 void cg_s_beta(hipStream_t s, KrylovState *st, const double *partials, int G);
 template <class T>
 void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
-           const T *invd, T *x, T *r, T *z, double *partials, int *grid);
+           const T *invd, T *x, T *r, T *z, int scaled, double *partials, int *grid);
 template <class T>
 void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *z, T *p);
 

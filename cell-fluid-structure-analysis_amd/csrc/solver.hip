@@ -142,6 +142,20 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   d_Prp.upload(PP.rowptr.data(), PP.rowptr.size());
   d_Pcol.upload(PP.col.data(), PP.col.size());
   d_Pdiag.upload(PP.diag.data(), PP.diag.size());
+  {
+    // the P1 pattern is symmetric: map every slot (i, j) to the slot (j, i)
+    std::vector<int32_t> tslot((size_t)PP.nnz());
+    for (int i = 0; i < nv; ++i)
+      for (int32_t k = PP.rowptr[(size_t)i]; k < PP.rowptr[(size_t)i + 1]; ++k) {
+        const int32_t j = PP.col[(size_t)k];
+        const int32_t *lo = PP.col.data() + PP.rowptr[(size_t)j];
+        const int32_t *hi = PP.col.data() + PP.rowptr[(size_t)j + 1];
+        const int32_t *it = std::lower_bound(lo, hi, i);
+        if (it == hi || *it != i) throw std::runtime_error("P1 pattern is not symmetric");
+        tslot[(size_t)k] = (int32_t)(it - PP.col.data());
+      }
+    d_Ptslot.upload(tslot.data(), tslot.size());
+  }
   nnzA = PA.nnz();
   nnzP = PP.nnz();
   A_ = DevCsr{nu, nnzA, d_Arp.p, d_Acol.p};
@@ -228,6 +242,7 @@ void NSSolver::assemble() {
   }
   lumped_mass_inverse(stream_, A_, d_M.p, d_ilm.p);
   lhs_dirty = lbc_dirty = true;
+  g3t_dirty = true;
 }
 
 void NSSolver::build_sparse_matrices() {
@@ -358,7 +373,8 @@ SolveResult NSSolver::bicgstab(const DevCsr &A, const T *val, const T *invd, con
   const bool time_spmv = (K == 3) && opt.timing >= 2;
   set_state_params(tol, maxit, K);
   int Gv = 0, Gb = 0, Gc = 0, Gr = 0;
-  bicg_init<T, K>(s, n, b, x, w.r.p, w.r0.p, w.p.p, w.v.p, d_part_a.p, &Gv);
+  const T *sc = opt.stop_scaled ? invd : nullptr;
+  bicg_init<T, K>(s, n, b, x, w.r.p, w.r0.p, w.p.p, w.v.p, sc, d_part_a.p, &Gv);
   bicg_s_init<K>(s, st, d_part_a.p, Gv);
   const int chunk = std::max(1, opt.check_every);
   const long max_chunks = (2L * maxit + 16) / chunk + 16;
@@ -382,7 +398,8 @@ SolveResult NSSolver::bicgstab(const DevCsr &A, const T *val, const T *invd, con
         spmv1_dot2<T>(s, A, val, w.z.p, w.t.p, w.s.p, act, d_part_c.p, &Gc);
       if (time_spmv) spmv_event_end();
       bicg_s_omega<K>(s, st, d_part_c.p, Gc);
-      bicg_v3<T, K>(s, n, st, w.y.p, w.z.p, w.s.p, w.t.p, w.r0.p, x, w.r.p, d_part_a.p, &Gv);
+      bicg_v3<T, K>(s, n, st, w.y.p, w.z.p, w.s.p, w.t.p, w.r0.p, x, w.r.p, sc, d_part_a.p,
+                    &Gv);
     }
     read_state();
     bool all_done = true;
@@ -421,7 +438,7 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
   const int *act = st->active;
   set_state_params(tol, maxit, 1);
   int Gv = 0, Gb = 0;
-  cg_init<T>(s, n, b, invd, x, w.r.p, w.p.p, d_part_a.p, &Gv);
+  cg_init<T>(s, n, b, invd, x, w.r.p, w.p.p, opt.stop_scaled, d_part_a.p, &Gv);
   cg_s_init(s, st, d_part_a.p, Gv);
   const int chunk = std::max(1, opt.check_every);
   const long max_chunks = (long)maxit / chunk + 16;
@@ -432,7 +449,7 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
     for (int j = 0; j < chunk; ++j) {
       spmv1_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
       cg_s_alpha(s, st, d_part_b.p, Gb);
-      cg_v1<T>(s, n, st, w.p.p, w.v.p, invd, x, w.r.p, w.z.p, d_part_a.p, &Gv);
+      cg_v1<T>(s, n, st, w.p.p, w.v.p, invd, x, w.r.p, w.z.p, opt.stop_scaled, d_part_a.p, &Gv);
       cg_s_beta(s, st, d_part_a.p, Gv);
       cg_v2<T>(s, n, st, w.z.p, w.p.p);
     }

@@ -29,6 +29,10 @@ struct DBuf {
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
   ~DBuf() { release(); }
   void release() {
     if (p) (void)hipFree(p);
@@ -63,6 +67,7 @@ struct SolverOptions {
   int max_refine = 40;         // outer refinement sweeps (mixed)
   int check_every = 8;         // Krylov iterations between host status reads
   int timing = 0;              // 1: per-phase events; 2: also per-SpMV events
+  int stop_scaled = 0;         // 1: stop on |D^-1 r| <= tol |D^-1 b| (Jacobi-scaled)
 };
 
 struct SolveResult {
@@ -166,6 +171,13 @@ class NSSolver {
   // Momentum-matrix SpMV launches repeated `reps` times between two events.
   double time_momentum_spmv(int reps, int precision);
 
+  // ---- discrete adjoint (adjoint.hip) ----
+  void set_viscosity(double mu);  // re-assembles K
+  int adjoint_forward(int nsteps, const double *g, double *u_final);
+  int adjoint_backward(const double *lambda_u, double *dLdg, double *dLdmu);
+  int adjoint_gradient(int nsteps, int nk, const double *basis, const double *coef, double *J,
+                       double *dJdc, double *dJdmu);
+
  private:
   hipStream_t stream_ = nullptr;
   // mesh on device
@@ -197,6 +209,14 @@ class NSSolver {
   KrylovState *h_state = nullptr;
   std::vector<hipEvent_t> ev_pool_;
   int ev_used_ = 0;
+  // adjoint state
+  DBuf<int> d_Ptslot;  // slot of (j, i) for each slot (i, j) of the P1 pattern
+  DBuf<double> d_G3T, d_lu, d_lus, d_lb, d_lg, d_lgp, d_lpc, d_lrp, d_lp, d_lp_new, d_contrib,
+      d_tmp3;
+  std::vector<DBuf<double>> ckpt_u, ckpt_p;
+  bool g3t_dirty = true;
+  int adj_steps_ = 0;
+  void ensure_adjoint_buffers(int nsteps);
 
   void assemble();
   void prepare_operators();

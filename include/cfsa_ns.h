@@ -59,6 +59,9 @@ typedef struct cfsa_solver_options {
   int max_refine;        /* refinement sweeps, 40 */
   int check_every;       /* Krylov iterations between host convergence reads, 8 */
   int timing;            /* 0 off, 1 per-phase events, 2 also per-SpMV events */
+  int stop_rule;         /* 0 |r| <= tol |b| (reference), 1 |D^-1 r| <= tol |D^-1 b| with
+                            D the Jacobi diagonal (for adjoint / finite-difference runs,
+                            where the 1e10 penalty rows dominate |b|) */
 } cfsa_solver_options;
 
 typedef struct cfsa_step_stats {
@@ -214,6 +217,76 @@ int cfsa_ns_krylov_solve(cfsa_ns *h, int which, int method, int nrhs, const doub
 int cfsa_ns_time_momentum_spmv(cfsa_ns *h, int reps, int precision, double *ms);
 /* the hipStream_t all solver work is issued on */
 int cfsa_ns_get_stream(const cfsa_ns *h, void **stream);
+
+/* ------------------------------------------------------------------------
+ * Discrete adjoint of the PISO step (extension; no counterpart in the
+ * reference).  Dirichlet outlet and n_corrections >= 1.  Linear solves use the
+ * solver options (use stop_rule 1 and tight tolerances for gradients that
+ * must match finite differences).
+ * ---------------------------------------------------------------------- */
+/* set the viscosity and re-assemble K (the reference takes it from Fluid at
+ * initialize only) */
+int cfsa_ns_set_viscosity(cfsa_ns *h, double viscosity);
+/* Run nsteps PISO steps from rest (u = 0 with Inlet nodes at g, p = 0) with
+ * per-node inlet values g (nu x 3), keeping a device checkpoint of (u, p) per
+ * step; u_final (nu x 3) may be NULL. */
+int cfsa_ns_adjoint_forward(cfsa_ns *h, int nsteps, const double *g, double *u_final);
+/* Reverse sweep of the last adjoint_forward: given lambda = dL/du_final
+ * (nu x 3; NULL means the outlet kinetic energy J = 1/2 sum_Outlet |u|^2),
+ * returns dL/dg (nu x 3) and dL/dviscosity.  Each step's u* and p' are
+ * recomputed from its checkpoint. */
+int cfsa_ns_adjoint_backward(cfsa_ns *h, const double *lambda_u, double *dLdg,
+                             double *dLdviscosity);
+/* J = 1/2 sum over Outlet velocity nodes of |u|^2 after nsteps from rest with
+ * g = sum_k coef[k] basis[k] (basis nk x nu x 3); returns J, dJ/dcoef[nk],
+ * dJ/dviscosity. */
+int cfsa_ns_adjoint_gradient(cfsa_ns *h, int nsteps, int nk, const double *basis,
+                             const double *coef, double *J, double *dJdcoef,
+                             double *dJdviscosity);
+
+/* ------------------------------------------------------------------------
+ * Configuration: config::loadConfig (include/IO/YamlConfig.hpp:8-157,
+ * src/IO/YamlConfig.cpp:30-96,180-232) for the mesh, navier_stokes and output
+ * sections; solid / dpm / fsi sections report only `enabled`.
+ * ---------------------------------------------------------------------- */
+typedef struct cfsa_sim_config {
+  /* mesh (MeshConfig) */
+  int mesh_type; /* 0 obj, 1 cylinder, 2 icosphere, 3 box (extension), 4 msh (extension) */
+  char mesh_file[1024];
+  double obj_scale;
+  double cylinder_length, cylinder_radius;
+  int cylinder_divisions;
+  double icosphere_radius;
+  int icosphere_recursion;
+  double max_element_size;
+  double flow_direction[3];
+  double inlet_outlet_coverage;
+  double box_size[3];
+  int box_cells[3];
+  /* navier_stokes (NavierStokesConfig) */
+  int ns_enabled;
+  double density, viscosity;
+  int is_newtonian, is_laminar;
+  double inlet_velocity[3];
+  double outlet_pressure;
+  int use_dirichlet_pressure;
+  double dt, relax_p, relax_u;
+  int n_corrections;   /* extension, default 1 */
+  int pressure_solver; /* extension: 0 bicgstab, 1 cg */
+  int precision;       /* extension: 0 fp64, 1 mixed */
+  double momentum_tol, pressure_tol; /* extension, 1e-6 / 1e-8 */
+  int inlet_profile;   /* extension: 0 uniform, 1 parabolic */
+  /* output (OutputConfig) */
+  char output_prefix[512];
+  int total_steps, output_frequency;
+  double vtk_scale;
+  int output_fluid, output_solid, output_dpm;
+  /* other sections */
+  int solid_enabled, dpm_enabled, fsi_enabled;
+} cfsa_sim_config;
+
+int cfsa_config_load(const char *path, cfsa_sim_config *out);
+int cfsa_config_parse(const char *yaml_text, cfsa_sim_config *out);
 
 #ifdef __cplusplus
 }

@@ -367,8 +367,17 @@ std::vector<double> inv_diag(const Csr &A) {
 
 // Eigen::BiCGSTAB<SparseMatrix, DiagonalPreconditioner>::solve with x0 = 0
 // (Eigen/src/IterativeLinearSolvers/BiCGSTAB.h, internal::bicgstab)
+// Squared norm used by the stopping test: |v|^2 (reference) or, with the
+// scaled rule, |D^-1 v|^2 where D^-1 is the Jacobi scaling.
+double stop_norm2(const std::vector<double> &v, const std::vector<double> &D, bool scaled) {
+  if (!scaled) return edot(v, v);
+  std::vector<double> w(v.size());
+  for (size_t k = 0; k < v.size(); ++k) w[k] = D[k] * v[k];
+  return edot(w, w);
+}
+
 KResult bicgstab(const Csr &A, const std::vector<double> &rhs, std::vector<double> &x,
-                 double tol, int maxit) {
+                 double tol, int maxit, bool scaled = false) {
   const int n = A.n;
   KResult res;
   x.assign(n, 0.0);
@@ -387,11 +396,12 @@ KResult bicgstab(const Csr &A, const std::vector<double> &rhs, std::vector<doubl
   }
   double rho = 1, alpha = 1, w = 1;
   std::vector<double> v(n, 0.0), p(n, 0.0), y(n), z(n), s(n), t(n);
-  const double tol2 = tol * tol * rhs_sqnorm;
+  const double b_stop = scaled ? stop_norm2(rhs, D, true) : rhs_sqnorm;
+  const double tol2 = tol * tol * b_stop;
   const double eps = std::numeric_limits<double>::epsilon();
   const double eps2 = eps * eps;
   int i = 0, restarts = 0;
-  while (edot(r, r) > tol2 && i < maxit) {
+  while (stop_norm2(r, D, scaled) > tol2 && i < maxit) {
     double rho_old = rho;
     rho = edot(r0, r);
     if (std::abs(rho) < eps2 * r0_sqnorm) {
@@ -418,7 +428,7 @@ KResult bicgstab(const Csr &A, const std::vector<double> &rhs, std::vector<doubl
     for (int k = 0; k < n; ++k) r[k] = s[k] - w * t[k];
     ++i;
   }
-  res.err = std::sqrt(edot(r, r) / rhs_sqnorm);
+  res.err = std::sqrt(stop_norm2(r, D, scaled) / b_stop);
   res.iters = i;
   res.ok = res.err <= tol;
   return res;
@@ -427,20 +437,20 @@ KResult bicgstab(const Csr &A, const std::vector<double> &rhs, std::vector<doubl
 // Jacobi PCG with the loop and stopping rule of Eigen's conjugate_gradient
 // (full matrix product).
 KResult pcg(const Csr &A, const std::vector<double> &rhs, std::vector<double> &x, double tol,
-            int maxit) {
+            int maxit, bool scaled = false) {
   const int n = A.n;
   KResult res;
   x.assign(n, 0.0);
   const std::vector<double> D = inv_diag(A);
   std::vector<double> r = rhs;
-  const double rhsNorm2 = edot(rhs, rhs);
+  const double rhsNorm2 = stop_norm2(rhs, D, scaled);
   if (rhsNorm2 == 0) {
     res.iters = 0;
     res.err = 0;
     return res;
   }
   const double threshold = std::max(tol * tol * rhsNorm2, std::numeric_limits<double>::min());
-  double rn2 = edot(r, r);
+  double rn2 = stop_norm2(r, D, scaled);
   if (rn2 < threshold) {
     res.iters = 0;
     res.err = std::sqrt(rn2 / rhsNorm2);
@@ -455,7 +465,7 @@ KResult pcg(const Csr &A, const std::vector<double> &rhs, std::vector<double> &x
     double alpha = absNew / edot(p, q);
     for (int k = 0; k < n; ++k) x[k] = x[k] + alpha * p[k];
     for (int k = 0; k < n; ++k) r[k] = r[k] - alpha * q[k];
-    rn2 = edot(r, r);
+    rn2 = stop_norm2(r, D, scaled);
     if (rn2 < threshold) break;
     for (int k = 0; k < n; ++k) z[k] = D[k] * r[k];
     double absOld = absNew;
@@ -487,6 +497,7 @@ struct orc_ns {
   int outlet_type = 0;
   double momentum_tol = 1e-6, pressure_tol = 1e-8;
   int max_iter = 1000, pressure_solver = 0;
+  bool stop_scaled = false;  // extension: Jacobi-scaled residual stopping rule
   Csr M, K, Gx, Gy, Gz, L;
   int it_mom[3] = {0, 0, 0};
   double err_mom[3] = {0, 0, 0};
@@ -694,7 +705,7 @@ struct orc_ns {
         }
       }
       std::vector<double> sol;
-      KResult r = bicgstab(lhs, rhs, sol, momentum_tol, max_iter);
+      KResult r = bicgstab(lhs, rhs, sol, momentum_tol, max_iter, stop_scaled);
       it_mom[comp] = r.iters;
       err_mom[comp] = r.err;
       if (!r.ok) return 2;
@@ -717,8 +728,8 @@ struct orc_ns {
         if (bc1(i) == OUTLET) rhs[i] = rhs[i] + 1e12 * outlet_pressure;
     }
     std::vector<double> x;
-    KResult r = pressure_solver == 1 ? pcg(A, rhs, x, pressure_tol, max_iter)
-                                     : bicgstab(A, rhs, x, pressure_tol, max_iter);
+    KResult r = pressure_solver == 1 ? pcg(A, rhs, x, pressure_tol, max_iter, stop_scaled)
+                                     : bicgstab(A, rhs, x, pressure_tol, max_iter, stop_scaled);
     const int k = std::min(n_p, 7);
     it_p[k] = r.iters;
     err_p[k] = r.err;
@@ -808,6 +819,175 @@ struct orc_ns {
     reenforceVelocityBCs();
     time += dt;
     return hasNans() ? 4 : 0;
+  }
+
+  // ---------------------------------------------------------------------
+  // Discrete adjoint of pisoStep (build extension; Dirichlet outlet, at
+  // least one corrector).  The step is
+  //   b_c  = (M u_c)/dt - E G_c p + conv_c(u) + 1e10 g_c (Inlet rows)
+  //   u*_c = A^-1 b_c,   A = M/dt + K + penalties,  K = K1 / viscosity
+  //   p'   = Lbc^-1 (-(sum_c G_c u*_c)/dt + penalties)
+  //   u'   = g (Inlet), 0 (Wall), u* - relax_u dt ilm G p' (free P1), u* (free P2)
+  //   p_new = p + ncorr relax_p p' (non-outlet rows), p_out (outlet rows)
+  // and this routine maps (lambda_u', lambda_p') to (lambda_u, lambda_p),
+  // accumulating dJ/dg and dJ/dviscosity.  A, M, K, Lbc are symmetric, so
+  // the adjoint solves reuse the forward operators.
+  static std::vector<double> spmv_t(const Csr &A, const std::vector<double> &x) {
+    std::vector<double> y(A.n, 0.0);
+    for (int i = 0; i < A.n; ++i)
+      for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) y[A.col[k]] += A.val[k] * x[i];
+    return y;
+  }
+  KResult solve(const Csr &A, const std::vector<double> &b, std::vector<double> &x, double tol,
+                int method) {
+    return method == 1 ? pcg(A, b, x, tol, max_iter, stop_scaled)
+                       : bicgstab(A, b, x, tol, max_iter, stop_scaled);
+  }
+
+  int adjointStep(const std::vector<V3> &u_in, const std::vector<double> &p_in,
+                  std::vector<V3> &lu, std::vector<double> &lp, std::vector<V3> &lg,
+                  double &dnu) {
+    velocity = u_in;
+    pressure = p_in;
+    if (int s = solveMomentumPredictor()) return s;
+    n_p = 0;
+    if (int s = solvePressurePoisson()) return s;
+    const std::vector<V3> us = velocity_star;
+    const std::vector<double> pc = pressure_correction;
+    const Csr *G[3] = {&Gx, &Gy, &Gz};
+    // velocity outputs
+    std::vector<V3> lus(nu, V3{});
+    std::vector<double> lgp[3] = {std::vector<double>(nv, 0.0), std::vector<double>(nv, 0.0),
+                                  std::vector<double>(nv, 0.0)};
+    for (int i = 0; i < nu; ++i) {
+      const uint8_t b = bc(i);
+      if (b == INLET) {
+        lg[i] = add(lg[i], lu[i]);
+      } else if (b == WALL) {
+      } else {
+        lus[i] = add(lus[i], lu[i]);
+        if (i < nv && inv_lumped_mass[i] > 1e-14) {
+          const double s = -relax_u * dt * inv_lumped_mass[i];
+          for (int c = 0; c < 3; ++c) lgp[c][i] = s * lu[i][c];
+        }
+      }
+    }
+    std::vector<double> lpc(nv, 0.0);
+    for (int c = 0; c < 3; ++c) {
+      std::vector<double> t = spmv_t(*G[c], lgp[c]);
+      for (int i = 0; i < nv; ++i) lpc[i] += t[i];
+    }
+    // pressure outputs
+    std::vector<double> lp_new(nv, 0.0);
+    for (int i = 0; i < nv; ++i) {
+      if (outlet_type == 0 && bc1(i) == OUTLET) continue;
+      lp_new[i] = lp[i];
+      lpc[i] += n_corrections * relax_p * lp[i];
+    }
+    // pressure solve and divergence
+    std::vector<double> lrp;
+    solve(lbc_matrix(), lpc, lrp, pressure_tol, pressure_solver);
+    std::vector<double> ld(nv);
+    for (int i = 0; i < nv; ++i) ld[i] = -lrp[i] / dt;
+    for (int c = 0; c < 3; ++c) {
+      std::vector<double> t = spmv_t(*G[c], ld);
+      for (int i = 0; i < nv; ++i) lus[i][c] += t[i];
+    }
+    // momentum solves, viscosity dependence of K
+    const Csr lhs = lhs_matrix();
+    std::vector<V3> lu_new(nu, V3{});
+    std::vector<V3> lb(nu);
+    for (int c = 0; c < 3; ++c) {
+      std::vector<double> rhs(nu), x, usc(nu);
+      for (int i = 0; i < nu; ++i) {
+        rhs[i] = lus[i][c];
+        usc[i] = us[i][c];
+      }
+      solve(lhs, rhs, x, momentum_tol, 0);
+      std::vector<double> Kus = spmv(K, usc);
+      double s = 0.0;
+      for (int i = 0; i < nu; ++i) s += x[i] * Kus[i];
+      dnu += s / mu;
+      std::vector<double> Mx = spmv(M, x);
+      for (int i = 0; i < nu; ++i) {
+        lb[i][c] = x[i];
+        lu_new[i][c] += Mx[i] / dt;
+        if (bc(i) == INLET) lg[i][c] += 1e10 * x[i];
+      }
+      std::vector<double> xp(x.begin(), x.begin() + nv);
+      std::vector<double> t = spmv_t(*G[c], xp);
+      for (int i = 0; i < nv; ++i) lp_new[i] -= t[i];
+    }
+    // convection: conv_i = -sum_T w_iT c_T(u), c_T = (grad u_T) u_bar_T
+    for (int t = 0; t < nt; ++t) {
+      const auto &tet = mesh->tets[t];
+      const auto &gr = mesh->grads[t];
+      const auto &en = mesh->tet_edge_nodes[t];
+      const double vol = tet.volume;
+      V3 lc{};
+      for (int i = 0; i < 4; ++i) lc = sub(lc, mul(vol / 4.0, lb[tet.vertids[i]]));
+      for (int e = 0; e < 6; ++e) lc = sub(lc, mul(vol / 24.0, lb[nv + en[e]]));
+      V3 ue{};
+      for (int i = 0; i < 4; ++i) ue = add(ue, mul(0.25, u_in[tet.vertids[i]]));
+      double Gu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int j = 0; j < 4; ++j)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) Gu[a][b] += u_in[tet.vertids[j]][a] * gr[j][b];
+      V3 gt{};  // Gu^T lc / 4
+      for (int b = 0; b < 3; ++b)
+        gt[b] = 0.25 * (Gu[0][b] * lc.x + Gu[1][b] * lc.y + Gu[2][b] * lc.z);
+      for (int j = 0; j < 4; ++j) {
+        const double s = dot(gr[j], ue);
+        lu_new[tet.vertids[j]] = add(lu_new[tet.vertids[j]], add(mul(s, lc), gt));
+      }
+    }
+    lu = lu_new;
+    lp = lp_new;
+    return 0;
+  }
+
+  // J = 1/2 sum_{Outlet velocity nodes} |u|^2 after nsteps from u0 = 0 with
+  // inlet nodes at g, p0 = 0; g = sum_k coef_k basis_k.
+  int adjointGradient(int nsteps, int nk, const double *basis, const double *coef, double *J,
+                      double *dJdc, double *dJdmu) {
+    g.assign(nu, V3{});
+    for (int k = 0; k < nk; ++k)
+      for (int i = 0; i < nu; ++i)
+        for (int c = 0; c < 3; ++c) g[i][c] += coef[k] * basis[((size_t)k * nu + i) * 3 + c];
+    velocity.assign(nu, V3{});
+    for (int i = 0; i < nu; ++i)
+      if (bc(i) == INLET) velocity[i] = g[i];
+    pressure.assign(nv, 0.0);
+    std::vector<std::vector<V3>> cu;
+    std::vector<std::vector<double>> cp;
+    for (int n = 0; n < nsteps; ++n) {
+      cu.push_back(velocity);
+      cp.push_back(pressure);
+      if (int s = pisoStep()) return s;
+    }
+    double j = 0.0;
+    std::vector<V3> lu(nu, V3{});
+    for (int i = 0; i < nu; ++i)
+      if (bc(i) == OUTLET) {
+        j += 0.5 * dot(velocity[i], velocity[i]);
+        lu[i] = velocity[i];
+      }
+    std::vector<double> lp(nv, 0.0);
+    std::vector<V3> lg(nu, V3{});
+    double dnu = 0.0;
+    for (int n = nsteps - 1; n >= 0; --n)
+      if (int s = adjointStep(cu[n], cp[n], lu, lp, lg, dnu)) return s;
+    for (int i = 0; i < nu; ++i)
+      if (bc(i) == INLET) lg[i] = add(lg[i], lu[i]);
+    *J = j;
+    for (int k = 0; k < nk; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < nu; ++i)
+        for (int c = 0; c < 3; ++c) s += lg[i][c] * basis[((size_t)k * nu + i) * 3 + c];
+      dJdc[k] = s;
+    }
+    *dJdmu = dnu;
+    return 0;
   }
 
   // :968-1009
@@ -1034,6 +1214,13 @@ void orc_ns_set_solver(orc_ns *h, double mt, double pt, int maxit, int ps) {
   h->pressure_tol = pt;
   h->max_iter = maxit;
   h->pressure_solver = ps;
+}
+void orc_ns_set_stop_scaled(orc_ns *h, int on) { h->stop_scaled = on != 0; }
+int orc_adjoint_gradient(orc_ns *h, int nsteps, int nk, const double *basis, const double *coef,
+                         double *J, double *dJdc, double *dJdmu) {
+  return guard([&]() -> int {
+    return h->adjointGradient(nsteps, nk, basis, coef, J, dJdc, dJdmu);
+  });
 }
 int orc_ns_piso_step(orc_ns *h) { return h->pisoStep(); }
 int orc_ns_simple_step(orc_ns *h) { return h->simpleStep(); }

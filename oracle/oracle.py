@@ -66,6 +66,10 @@ _sig("orc_ns_get_stats", None, _p, _i32p, _f64p)
 _sig("orc_ns_krylov_solve", C.c_int, _p, C.c_int, C.c_int, _f64p, _f64p, C.c_double, C.c_int,
      C.POINTER(C.c_int32), C.POINTER(C.c_double))
 
+_sig("orc_ns_set_stop_scaled", None, _p, C.c_int)
+_sig("orc_adjoint_gradient", C.c_int, _p, C.c_int, C.c_int, _f64p, _f64p,
+     C.POINTER(C.c_double), _f64p, C.POINTER(C.c_double))
+
 MATRIX = {"M": 0, "K": 1, "lhs": 2, "L": 3, "Lbc": 4, "Gx": 5, "Gy": 6, "Gz": 7}
 
 
@@ -221,6 +225,22 @@ class OracleNS:
 
     def setSolver(self, momentum_tol=1e-6, pressure_tol=1e-8, max_iter=1000, pressure_solver=0):
         _lib.orc_ns_set_solver(self._h, momentum_tol, pressure_tol, max_iter, pressure_solver)
+
+    def setStopScaled(self, on=True):
+        _lib.orc_ns_set_stop_scaled(self._h, int(bool(on)))
+
+    def adjointGradient(self, nsteps, basis, coef):
+        """J, dJ/dcoef, dJ/dviscosity of J = 1/2 sum_outlet |u|^2 after nsteps."""
+        basis = np.ascontiguousarray(basis, np.float64)
+        coef = np.ascontiguousarray(coef, np.float64)
+        J = C.c_double()
+        dmu = C.c_double()
+        dc = np.empty(len(coef), np.float64)
+        st = _lib.orc_adjoint_gradient(self._h, int(nsteps), len(coef), basis, coef, C.byref(J),
+                                       dc, C.byref(dmu))
+        if st:
+            raise RuntimeError(f"adjoint failed with status {st}")
+        return J.value, dc, dmu.value
 
     def pisoStep(self) -> int:
         return _lib.orc_ns_piso_step(self._h)
