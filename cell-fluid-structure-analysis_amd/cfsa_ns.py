@@ -169,6 +169,11 @@ _sig("cfsa_ns_krylov_solve", C.c_int, _p, C.c_int, C.c_int, C.c_int, _f64p, _f64
      C.c_int, C.c_int, C.POINTER(SolveResult))
 _sig("cfsa_ns_time_momentum_spmv", C.c_int, _p, C.c_int, C.c_int, C.POINTER(C.c_double))
 _sig("cfsa_ns_get_stream", C.c_int, _p, C.POINTER(_p))
+for _n in ["pressure_forces", "shear_stress", "total_fluid_forces"]:
+    _sig(f"cfsa_ns_compute_{_n}", C.c_int, _p, _f64p)
+_sig("cfsa_ns_num_faces", C.c_int, _p, C.POINTER(C.c_int64))
+_sig("cfsa_ns_get_mean_face_pressure", C.c_int, _p, C.c_int64, C.POINTER(C.c_double))
+_sig("cfsa_ns_get_pressure_force_at_face", C.c_int, _p, C.c_int64, _f64p)
 _sig("cfsa_ns_set_viscosity", C.c_int, _p, C.c_double)
 _sig("cfsa_ns_adjoint_forward", C.c_int, _p, C.c_int, _f64p, _p)
 _sig("cfsa_ns_adjoint_backward", C.c_int, _p, _p, _f64p, C.POINTER(C.c_double))
@@ -603,6 +608,36 @@ class NavierStokesSolver:
         out = _p()
         _check(_lib.cfsa_ns_get_stream(self._h, C.byref(out)))
         return out.value
+
+    # --- boundary forces (NavierStokes.cpp:357-372, 841-965) ---
+    def _forces(self, name):
+        out = np.empty((self.sizes()["nv"], 3), np.float64)
+        _check(getattr(_lib, f"cfsa_ns_compute_{name}")(self._h, out))
+        return out
+
+    def computePressureForces(self):
+        return self._forces("pressure_forces")
+
+    def computeShearStress(self):
+        return self._forces("shear_stress")
+
+    def computeTotalFluidForces(self):
+        return self._forces("total_fluid_forces")
+
+    def nFaces(self):
+        out = C.c_int64()
+        _check(_lib.cfsa_ns_num_faces(self._h, C.byref(out)))
+        return out.value
+
+    def getMeanFacePressure(self, fi):
+        out = C.c_double()
+        _check(_lib.cfsa_ns_get_mean_face_pressure(self._h, int(fi), C.byref(out)))
+        return out.value
+
+    def getPressureForceAtFace(self, fi):
+        out = np.empty(3, np.float64)
+        _check(_lib.cfsa_ns_get_pressure_force_at_face(self._h, int(fi), out))
+        return out
 
     # --- adjoint (extension) ---
     def setViscosity(self, mu):

@@ -670,6 +670,50 @@ int cfsa_ns_get_stream(const cfsa_ns *h, void **stream) {
 }
 
 // ---------------------------------------------------------------------------
+// Boundary forces
+// ---------------------------------------------------------------------------
+static int forces(cfsa_ns *h, int which, double *out) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    sync_mesh(h);
+    h->s.compute_forces(which, out);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_compute_pressure_forces(cfsa_ns *h, double *out) { return forces(h, 0, out); }
+int cfsa_ns_compute_shear_stress(cfsa_ns *h, double *out) { return forces(h, 1, out); }
+int cfsa_ns_compute_total_fluid_forces(cfsa_ns *h, double *out) { return forces(h, 2, out); }
+int cfsa_ns_num_faces(cfsa_ns *h, int64_t *out) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    *out = h->s.n_faces();
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_get_mean_face_pressure(cfsa_ns *h, int64_t face, double *out) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.face_pressure(face, out, nullptr);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_get_pressure_force_at_face(cfsa_ns *h, int64_t face, double out[3]) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.face_pressure(face, nullptr, out);
+    return CFSA_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint
 // ---------------------------------------------------------------------------
 int cfsa_ns_set_viscosity(cfsa_ns *h, double viscosity) {

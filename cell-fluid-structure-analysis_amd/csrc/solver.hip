@@ -224,6 +224,7 @@ void NSSolver::update_mesh(const double *coords, const uint8_t *p1_bc, const uin
   if (p1_bc) std::memcpy(h_bc.data(), p1_bc, (size_t)nv);
   if (p2_bc && ne) std::memcpy(h_bc.data() + nv, p2_bc, (size_t)ne);
   if (p1_bc || p2_bc) d_bc.upload(h_bc.data(), h_bc.size());
+  if (coords) faces_ready_ = false;
   lhs_dirty = lbc_dirty = true;
 }
 

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "dev.hpp"
+#include "mesh.hpp"
 #include "ops.hpp"
 
 namespace cfsa {
@@ -171,6 +172,14 @@ class NSSolver {
   // Momentum-matrix SpMV launches repeated `reps` times between two events.
   double time_momentum_spmv(int reps, int precision);
 
+  // ---- boundary forces (forces.hip); which: 0 pressure, 1 shear, 2 total ----
+  void compute_forces(int which, double *out);
+  void face_pressure(int64_t fi, double *mean, double force[3]);
+  int64_t n_faces() {
+    ensure_faces();
+    return (int64_t)all_faces_.size();
+  }
+
   // ---- discrete adjoint (adjoint.hip) ----
   void set_viscosity(double mu);  // re-assembles K
   int adjoint_forward(int nsteps, const double *g, double *u_final);
@@ -217,6 +226,13 @@ class NSSolver {
   bool g3t_dirty = true;
   int adj_steps_ = 0;
   void ensure_adjoint_buffers(int nsteps);
+  // boundary faces for the force kernels
+  DBuf<int> d_bfv, d_bft, d_vfp, d_vff;
+  DBuf<double> d_bfg, d_fp, d_fs, d_forces;
+  std::vector<Face> all_faces_;
+  int nbf_ = 0;
+  bool faces_ready_ = false;
+  void ensure_faces();
 
   void assemble();
   void prepare_operators();

@@ -194,6 +194,17 @@ int cfsa_ns_get_step_stats(const cfsa_ns *h, cfsa_step_stats *out);
 /* io::exportToVtk(filename, ns_solver, scale) (VtkExport.cpp:238-268) */
 int cfsa_ns_export_vtk(cfsa_ns *h, const char *path, double scale);
 
+/* boundary forces per P1 vertex (nv x 3): computePressureForces,
+ * computeShearStress, computeTotalFluidForces (NavierStokes.cpp:841-965) */
+int cfsa_ns_compute_pressure_forces(cfsa_ns *h, double *out);
+int cfsa_ns_compute_shear_stress(cfsa_ns *h, double *out);
+int cfsa_ns_compute_total_fluid_forces(cfsa_ns *h, double *out);
+/* faces in sorted-triple order: getMeanFacePressure / getPressureForceAtFace
+ * (NavierStokes.cpp:357-372); cfsa_ns_num_faces gives their count */
+int cfsa_ns_num_faces(cfsa_ns *h, int64_t *out);
+int cfsa_ns_get_mean_face_pressure(cfsa_ns *h, int64_t face, double *out);
+int cfsa_ns_get_pressure_force_at_face(cfsa_ns *h, int64_t face, double out[3]);
+
 /* ---- parity / diagnostics (operators the reference builds internally) ---- */
 /* inv_lumped_mass_ (NavierStokes.cpp:290-306), nu entries */
 int cfsa_ns_get_inv_lumped_mass(cfsa_ns *h, double *out);

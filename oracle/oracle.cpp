@@ -821,6 +821,44 @@ struct orc_ns {
     return hasNans() ? 4 : 0;
   }
 
+  // :841-883
+  std::vector<V3> computePressureForces() const {
+    std::vector<V3> out(nv, V3{});
+    for (const auto &face : mesh->faces) {
+      if (face.tet_b != -1) continue;
+      double fp = 0.0;
+      for (int v : face.vertids) fp += pressure[v];
+      fp /= 3.0;
+      V3 ff = mul(face.area, mul(-fp, face.normal));
+      const double w = 1.0 / 3.0;
+      for (int v : face.vertids) out[v] = add(out[v], mul(w, ff));
+    }
+    return out;
+  }
+  // :885-944
+  std::vector<V3> computeShearStress() const {
+    std::vector<V3> out(nv, V3{});
+    for (const auto &face : mesh->faces) {
+      if (face.tet_b != -1) continue;
+      const auto &tet = mesh->tets[face.tet_a];
+      const auto &gr = mesh->grads[face.tet_a];
+      double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int i = 0; i < 4; ++i)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) G[a][b] = G[a][b] + velocity[tet.vertids[i]][a] * gr[i][b];
+      V3 tr;
+      for (int a = 0; a < 3; ++a) {
+        double tau[3];
+        for (int b = 0; b < 3; ++b) tau[b] = (2.0 * mu) * (0.5 * (G[a][b] + G[b][a]));
+        tr[a] = (tau[0] * face.normal.x + tau[1] * face.normal.y) + tau[2] * face.normal.z;
+      }
+      V3 ff = mul(face.area, tr);
+      const double w = 1.0 / 3.0;
+      for (int v : face.vertids) out[v] = add(out[v], mul(w, ff));
+    }
+    return out;
+  }
+
   // ---------------------------------------------------------------------
   // Discrete adjoint of pisoStep (build extension; Dirichlet outlet, at
   // least one corrector).  The step is
@@ -1216,6 +1254,14 @@ void orc_ns_set_solver(orc_ns *h, double mt, double pt, int maxit, int ps) {
   h->pressure_solver = ps;
 }
 void orc_ns_set_stop_scaled(orc_ns *h, int on) { h->stop_scaled = on != 0; }
+// which: 0 pressure, 1 shear, 2 total (nv x 3)
+void orc_ns_forces(orc_ns *h, int which, double *out) {
+  std::vector<V3> p = h->computePressureForces(), s = h->computeShearStress();
+  for (int i = 0; i < h->nv; ++i) {
+    V3 v = which == 0 ? p[i] : (which == 1 ? s[i] : add(p[i], s[i]));
+    for (int c = 0; c < 3; ++c) out[i * 3 + c] = v[c];
+  }
+}
 int orc_adjoint_gradient(orc_ns *h, int nsteps, int nk, const double *basis, const double *coef,
                          double *J, double *dJdc, double *dJdmu) {
   return guard([&]() -> int {

@@ -66,6 +66,9 @@ void orc_ns_set_relax(orc_ns *h, double relax_u, double relax_p);
 void orc_ns_set_reference_node(orc_ns *h, int node);
 void orc_ns_set_solver(orc_ns *h, double momentum_tol, double pressure_tol, int max_iter,
                        int pressure_solver);
+/* boundary forces per vertex (nv x 3): 0 pressure, 1 shear, 2 total
+ * (NavierStokes.cpp:841-965) */
+void orc_ns_forces(orc_ns *h, int which, double *out);
 /* extension: stop Krylov solves on |D^-1 r| <= tol |D^-1 b| (Jacobi-scaled) */
 void orc_ns_set_stop_scaled(orc_ns *h, int on);
 int orc_ns_piso_step(orc_ns *h);   /* 0 ok, 2 momentum, 3 pressure, 4 NaN */

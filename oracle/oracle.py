@@ -67,6 +67,7 @@ _sig("orc_ns_krylov_solve", C.c_int, _p, C.c_int, C.c_int, _f64p, _f64p, C.c_dou
      C.POINTER(C.c_int32), C.POINTER(C.c_double))
 
 _sig("orc_ns_set_stop_scaled", None, _p, C.c_int)
+_sig("orc_ns_forces", None, _p, C.c_int, _f64p)
 _sig("orc_adjoint_gradient", C.c_int, _p, C.c_int, C.c_int, _f64p, _f64p,
      C.POINTER(C.c_double), _f64p, C.POINTER(C.c_double))
 
@@ -225,6 +226,12 @@ class OracleNS:
 
     def setSolver(self, momentum_tol=1e-6, pressure_tol=1e-8, max_iter=1000, pressure_solver=0):
         _lib.orc_ns_set_solver(self._h, momentum_tol, pressure_tol, max_iter, pressure_solver)
+
+    def forces(self, which):
+        """0 computePressureForces, 1 computeShearStress, 2 computeTotalFluidForces."""
+        out = np.empty((self.sizes()["nv"], 3), np.float64)
+        _lib.orc_ns_forces(self._h, int(which), out)
+        return out
 
     def setStopScaled(self, on=True):
         _lib.orc_ns_set_stop_scaled(self._h, int(bool(on)))

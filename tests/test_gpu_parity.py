@@ -247,3 +247,23 @@ def test_vtk_export_matches_fields(P, O, tmp_path):
     j = txt.index("VECTORS velocity double")
     u = np.array([[float(v) for v in r.split()] for r in txt[j + 1:j + 1 + nv]])
     np.testing.assert_allclose(u, g.getVelocity()[:nv], rtol=1e-5, atol=1e-12)
+
+
+def test_boundary_forces_bit_exact(P, O):
+    g, o, pm, om = make_case(P, O, "kuhn", 4)
+    for _ in range(2):
+        assert g.pisoStep() and o.pisoStep() == 0
+    u = np.random.default_rng(3).uniform(-1, 1, (g.sizes()["nu"], 3))
+    p = np.random.default_rng(4).uniform(-1, 1, g.sizes()["nv"])
+    for obj in (g, o):
+        obj.setVelocity(u)
+        obj.setPressure(p)
+    assert np.array_equal(g.computePressureForces(), o.forces(0))
+    assert np.array_equal(g.computeShearStress(), o.forces(1))
+    assert np.array_equal(g.computeTotalFluidForces(), o.forces(2))
+    v, t, geo = pm.faces()
+    assert g.nFaces() == len(v)
+    for fi in [0, len(v) // 2, len(v) - 1]:
+        mp = p[v[fi]].sum() / 3.0
+        assert g.getMeanFacePressure(fi) == mp
+        np.testing.assert_array_equal(g.getPressureForceAtFace(fi), (-geo[fi, :3] * geo[fi, 3]) * mp)

@@ -288,3 +288,33 @@ def test_step_enforces_boundary_values(O):
     assert not u[lab == 0].any()
     assert not p[m.p1BC() == 2].any()
     assert np.isfinite(u).all() and np.abs(u).max() < 10
+
+
+# ------------------------------------------------------- boundary forces (f1)
+def test_forces_closed_forms(O):
+    """Linear velocity u = A x + b: every tet has grad u = A, so each boundary
+    face carries traction 2 mu sym(A) n area, one third to each vertex.  A
+    uniform pressure p0 gives -p0 n area / 3 per face vertex."""
+    m = _prism(O, 2, 1.0, 1.5, 2.0)
+    mu = 0.004
+    ns = O.OracleNS(m, 1060.0, mu)
+    sz = ns.sizes()
+    X = np.vstack([m.vertices(), m.edgeNodes()])
+    A = np.array([[0.2, -0.4, 0.1], [0.5, 0.3, -0.2], [-0.1, 0.6, 0.4]])
+    ns.setVelocity(X @ A.T + 0.3)
+    ns.setPressure(np.full(sz["nv"], 7.0))
+    v, t, g = m.faces()
+    S = 2 * mu * 0.5 * (A + A.T)
+    want_s = np.zeros((sz["nv"], 3))
+    want_p = np.zeros((sz["nv"], 3))
+    for f in np.nonzero(t[:, 1] == -1)[0]:
+        n, area = g[f, :3], g[f, 3]
+        for vi in v[f]:
+            want_s[vi] += S @ n * area / 3
+            want_p[vi] += -7.0 * n * area / 3
+    np.testing.assert_allclose(ns.forces(1), want_s, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(ns.forces(0), want_p, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(ns.forces(2), want_p + want_s, rtol=1e-12, atol=1e-15)
+    # boundary normals are unit vectors, areas sum to the box surface
+    assert np.allclose(np.linalg.norm(g[t[:, 1] == -1, :3], axis=1), 1.0)
+    assert abs(g[t[:, 1] == -1, 3].sum() - 2 * (1.5 + 2.0 + 3.0)) < 1e-12
