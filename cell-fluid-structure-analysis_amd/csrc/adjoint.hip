@@ -17,6 +17,8 @@
 
 #include "solver.hpp"
 
+#include "dd.hpp"
+
 namespace cfsa {
 
 namespace {
@@ -139,25 +141,6 @@ __global__ void k_gather_slots3(int64_t nnz, const int *__restrict__ tslot,
     for (int c = 0; c < 3; ++c) gt[k * 3 + c] = g[(size_t)tslot[k] * 3 + c];
 }
 
-template <int NV>
-__global__ void k_reduce3(const double *__restrict__ partials, int G, double *out) {
-  __shared__ double red[NV][kBlock];
-  const int tid = threadIdx.x;
-  for (int i = 0; i < NV; ++i) {
-    double s = 0.0;
-    for (int g = tid; g < G; g += kBlock) s += partials[(size_t)i * G + g];
-    red[i][tid] = s;
-  }
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (tid < s)
-      for (int i = 0; i < NV; ++i) red[i][tid] += red[i][tid + s];
-    __syncthreads();
-  }
-  if (tid == 0)
-    for (int i = 0; i < NV; ++i) out[i] = red[i][0];
-}
-
 }  // namespace
 
 void NSSolver::ensure_adjoint_buffers(int nsteps) {
@@ -274,7 +257,7 @@ int NSSolver::adjoint_backward(const double *lambda_u, double *dLdg, double *dLd
     if (!rm.ok) return ST_MOMENTUM_NOCONV;
     // dJ/dviscosity += sum_c lambda_b_c . (K u*_c) / viscosity
     spmv3_dot<double>(s, A_, d_K.p, d_us.p, d_tmp3.p, d_lb.p, nullptr, d_part_r.p, &Gd);
-    hipLaunchKernelGGL((k_reduce3<3>), dim3(1), dim3(kBlock), 0, s, d_part_r.p, Gd, d_scalar.p);
+    hipLaunchKernelGGL((k_reduce_pairs<3>), dim3(1), dim3(kBlock), 0, s, d_part_r.p, Gd, d_scalar.p);
     CFSA_LAUNCH_CHECK();
     double dots[3];
     CFSA_HIP(hipMemcpyAsync(dots, d_scalar.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -65,6 +65,7 @@ cfsa::SolverOptions to_opts(const cfsa_solver_options &o) {
   s.check_every = o.check_every;
   s.timing = o.timing;
   s.stop_scaled = o.stop_rule;
+  s.assembly = o.assembly;
   return s;
 }
 
@@ -448,10 +449,10 @@ int cfsa_ns_set_time(cfsa_ns *h, double t) {
 int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt) {
   CHECK_PTR(h);
   CHECK_PTR(opt);
-  if (opt->max_iter < 0 || opt->check_every < 1 || opt->precision < 0 || opt->precision > 1 ||
+  if (opt->max_iter < 0 || opt->check_every < 1 || opt->precision < 0 || opt->precision > 2 ||
       opt->pressure_solver < 0 || opt->pressure_solver > 1 || !(opt->momentum_tol >= 0) ||
       !(opt->pressure_tol >= 0) || !(opt->inner_tol > 0) || opt->max_refine < 0 ||
-      opt->stop_rule < 0 || opt->stop_rule > 1)
+      opt->stop_rule < 0 || opt->stop_rule > 1 || opt->assembly < 0 || opt->assembly > 1)
     return fail(CFSA_ERR_INVALID, "invalid solver options");
   h->s.opt = to_opts(*opt);
   return CFSA_OK;
@@ -470,6 +471,7 @@ int cfsa_ns_get_options(const cfsa_ns *h, cfsa_solver_options *o) {
   o->check_every = s.check_every;
   o->timing = s.timing;
   o->stop_rule = s.stop_scaled;
+  o->assembly = s.assembly;
   return CFSA_OK;
 }
 

@@ -34,7 +34,7 @@ namespace cfsa {
 constexpr int kBlock = 256;         // threads per block, 4 wave64
 constexpr int kVecGrid = 1024;      // fixed grid of vector/reduction kernels
 constexpr int kSpmvGrid = 2048;     // fixed grid of CSR-stream SpMV kernels
-constexpr int kMaxPartials = 4096;  // >= max(kVecGrid, kSpmvGrid)
+constexpr int kMaxPartials = 4096;  // >= max(kVecGrid, kSpmvGrid); partials hold pairs
 
 // Fluid boundary labels (FluidBCType, NavierStokes.hpp:13-19).
 constexpr uint8_t kWall = 0, kInlet = 1, kOutlet = 2, kInternal = 3;

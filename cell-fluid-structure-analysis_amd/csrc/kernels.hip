@@ -11,6 +11,9 @@
 
 #include <cfloat>
 
+#include "dd.hpp"
+#include "pattern.hpp"
+
 namespace cfsa {
 
 // ===========================================================================
@@ -42,12 +45,13 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(int n,
                                                        double *__restrict__ partials) {
   constexpr int R = StreamCfg<T, KO>::R, B = StreamCfg<T, KO>::B;
   constexpr int NDs = ND > 0 ? ND : 1;
+  // staged products; reused for the block's inner-product pairs at the end
   __shared__ T lds[B * KO];
-  __shared__ double red[NDs][kBlock];
+  static_assert(sizeof(T) * B * KO >= sizeof(double) * 2 * NDs * kBlock, "LDS reuse");
   if (!op.any()) return;
-  double dots[NDs];
+  DD dots[NDs];
 #pragma unroll
-  for (int d = 0; d < NDs; ++d) dots[d] = 0.0;
+  for (int d = 0; d < NDs; ++d) dd_zero(dots[d]);
   const int tid = threadIdx.x;
   const int myc = tid % KO;
   const bool act = op.active(myc);
@@ -82,14 +86,24 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(int n,
     if (owner && act) op.epi(row, myc, acc, dots);
   }
   if constexpr (ND > 0) {
+    double *rh = reinterpret_cast<double *>(lds);  // [ND][kBlock]
+    double *rl = rh + ND * kBlock;                 // [ND][kBlock]
+    __syncthreads();
 #pragma unroll
-    for (int d = 0; d < ND; ++d) red[d][tid] = dots[d];
+    for (int d = 0; d < ND; ++d) {
+      rh[d * kBlock + tid] = dots[d].hi;
+      rl[d * kBlock + tid] = dots[d].lo;
+    }
     __syncthreads();
     if (tid < KO * ND) {
+      // per component c: threads c, c+KO, ... in ascending order
       const int c = tid / ND, d = tid % ND;
-      double s = 0.0;
-      for (int t = c; t < kBlock; t += KO) s += red[d][t];
-      partials[(size_t)(c * ND + d) * gridDim.x + blockIdx.x] = s;
+      DD s;
+      dd_zero(s);
+      for (int t = c; t < kBlock; t += KO) dd_merge(s, rh[d * kBlock + t], rl[d * kBlock + t]);
+      const size_t o = 2 * ((size_t)(c * ND + d) * gridDim.x + blockIdx.x);
+      partials[o] = s.hi;
+      partials[o + 1] = s.lo;
     }
   }
 }
@@ -98,6 +112,115 @@ template <class T, int KO>
 static int stream_grid(int n) {
   const int nrb = div_up(n, StreamCfg<T, KO>::R);
   return std::max(1, std::min(nrb, kSpmvGrid));
+}
+
+// ===========================================================================
+// Sliced-ELL core (the Krylov operators lhs and L_bc): one wavefront per slice
+// of 64 rows, one row per lane.  Matrix values and columns of entry j of the
+// 64 rows are contiguous, so every load of them is a full 64-lane coalesced
+// access, and no LDS staging or block barrier is needed; a lane sums its row
+// in CSR order (bit-identical to the CSR-stream kernel and to the reference).
+// ===========================================================================
+constexpr int kSellGrid = 2048;  // max blocks; always a multiple of 8
+constexpr int kSellUnroll = 8;   // entries in flight per lane
+
+static int sell_grid(int nslices) {
+  constexpr int wpb = kBlock / kSellSlice;
+  const int share = div_up(nslices, 8);
+  const int per_xcd = std::max(1, std::min(div_up(share, wpb), kSellGrid / 8));
+  return 8 * per_xcd;
+}
+
+template <class T, int KO, int ND, int U, class Op>
+__global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices,
+                                                 const int *__restrict__ sptr,
+                                                 const int *__restrict__ rlen,
+                                                 const int *__restrict__ perm, Op op,
+                                                 double *__restrict__ partials) {
+  constexpr int NDs = ND > 0 ? ND : 1;
+  constexpr int wpb = kBlock / kSellSlice;
+  if (!op.any()) return;
+  DD dots[KO * NDs];
+#pragma unroll
+  for (int i = 0; i < KO * NDs; ++i) dd_zero(dots[i]);
+  bool act[KO];
+#pragma unroll
+  for (int c = 0; c < KO; ++c) act[c] = op.active(c);
+  const int lane = threadIdx.x % kSellSlice;
+  const int wv = threadIdx.x / kSellSlice;
+  // Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8):
+  // each XCD gets one contiguous share of the slices, so the x entries its
+  // rows gather stay in that XCD's L2.
+  const int xcd = blockIdx.x & 7, bx = blockIdx.x >> 3, nbx = gridDim.x >> 3;
+  const int share = (nslices + 7) >> 3;
+  const int s0 = xcd * share, s1 = min(nslices, s0 + share);
+  for (int s = s0 + bx * wpb + wv; s < s1; s += nbx * wpb) {
+    const int p = s * kSellSlice + lane;
+    const int len = rlen[p];
+    const int base = sptr[s] + lane;
+    T acc[KO];
+#pragma unroll
+    for (int c = 0; c < KO; ++c) acc[c] = T(0);
+    int j = 0;
+    for (; j + U <= len; j += U) {
+      T pr[U][KO];
+#pragma unroll
+      for (int u = 0; u < U; ++u) op.prod(base + (j + u) * kSellSlice, pr[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int c = 0; c < KO; ++c) acc[c] += pr[u][c];
+    }
+    for (; j < len; ++j) {
+      T pr[KO];
+      op.prod(base + j * kSellSlice, pr);
+#pragma unroll
+      for (int c = 0; c < KO; ++c) acc[c] += pr[c];
+    }
+    if (p < n) {
+      const int row = perm[p];
+#pragma unroll
+      for (int c = 0; c < KO; ++c)
+        if (act[c]) op.epi(row, c, acc[c], &dots[c * NDs]);
+    }
+  }
+  if constexpr (ND > 0) block_partials<KO * ND>(dots, partials);
+}
+
+template <class T>
+__global__ void k_sell_fill(int64_t stored, const int *__restrict__ src,
+                            const double *__restrict__ in, T *__restrict__ out) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < stored;
+       e += (int64_t)gridDim.x * kBlock) {
+    const int k = src[e];
+    out[e] = k >= 0 ? (T)in[k] : T(0);
+  }
+}
+template <class T>
+void sell_fill(hipStream_t s, int64_t stored, const int *src, const double *in, T *out) {
+  if (stored <= 0) return;
+  hipLaunchKernelGGL((k_sell_fill<T>), dim3(4096), dim3(kBlock), 0, s, stored, src, in, out);
+  CFSA_LAUNCH_CHECK();
+}
+template void sell_fill<double>(hipStream_t, int64_t, const int *, const double *, double *);
+template void sell_fill<float>(hipStream_t, int64_t, const int *, const double *, float *);
+
+// Launches the sliced-ELL kernel when A carries that layout, else CSR-stream;
+// returns the grid size (the number of partial pairs per value).
+template <class T, int KO, int ND, class Op>
+static int launch_spmv(hipStream_t s, const DevCsr &A, const Op &op, double *partials) {
+  if (A.sptr) {
+    const int G = sell_grid(A.nslices);
+    hipLaunchKernelGGL((k_sell<T, KO, ND, kSellUnroll, Op>), dim3(G), dim3(kBlock), 0, s, A.n,
+                       A.nslices, A.sptr, A.rlen, A.perm, op, partials);
+    CFSA_LAUNCH_CHECK();
+    return G;
+  }
+  const int G = stream_grid<T, KO>(A.n);
+  hipLaunchKernelGGL((k_csr_stream<T, KO, ND, Op>), dim3(G), dim3(kBlock), 0, s, A.n,
+                     A.rowptr, op, partials);
+  CFSA_LAUNCH_CHECK();
+  return G;
 }
 
 // ---- operator functors -----------------------------------------------------
@@ -126,7 +249,7 @@ struct OpAx : Base {
 #pragma unroll
     for (int c = 0; c < KO; ++c) p[c] = a * x[(size_t)j * KO + c];
   }
-  __device__ void epi(int row, int c, T acc, double *) const {
+  __device__ void epi(int row, int c, T acc, DD *) const {
     y[(size_t)row * KO + c] = acc;
   }
 };
@@ -134,22 +257,22 @@ struct OpAx : Base {
 template <class T, int KO, class Base>
 struct OpAxDot : OpAx<T, KO, Base> {
   const T *__restrict__ r0;
-  __device__ void epi(int row, int c, T acc, double *d) const {
+  __device__ void epi(int row, int c, T acc, DD *d) const {
     const size_t i = (size_t)row * KO + c;
     this->y[i] = acc;
-    d[0] += (double)r0[i] * (double)acc;
+    dd_fma(d[0], (double)r0[i], (double)acc);
   }
 };
 
 template <class T, int KO, class Base>
 struct OpAxDot2 : OpAx<T, KO, Base> {
   const T *__restrict__ sv;
-  __device__ void epi(int row, int c, T acc, double *d) const {
+  __device__ void epi(int row, int c, T acc, DD *d) const {
     const size_t i = (size_t)row * KO + c;
     this->y[i] = acc;
     const double t = (double)acc;
-    d[0] += t * t;
-    d[1] += t * (double)sv[i];
+    dd_fma(d[0], t, t);
+    dd_fma(d[1], t, (double)sv[i]);
   }
 };
 
@@ -157,12 +280,12 @@ template <class T, int KO, class Base>
 struct OpResidual : OpAx<T, KO, Base> {
   const T *__restrict__ b;
   T *__restrict__ r0;
-  __device__ void epi(int row, int c, T acc, double *d) const {
+  __device__ void epi(int row, int c, T acc, DD *d) const {
     const size_t i = (size_t)row * KO + c;
     const T r = b[i] - acc;
     this->y[i] = r;
     r0[i] = r;
-    d[0] += (double)r * (double)r;
+    dd_fma(d[0], (double)r, (double)r);
   }
 };
 
@@ -176,7 +299,7 @@ struct OpGrad3 : MaskBase {
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = g[(size_t)k * 3 + c] * xj;
   }
-  __device__ void epi(int row, int c, double acc, double *) const {
+  __device__ void epi(int row, int c, double acc, DD *) const {
     y[(size_t)row * 3 + c] = acc;
   }
 };
@@ -198,7 +321,7 @@ struct OpMomRhs : MaskBase {
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = a * u[(size_t)j * 3 + c];
   }
-  __device__ void epi(int row, int c, double acc, double *) const {
+  __device__ void epi(int row, int c, double acc, DD *) const {
     const size_t i = (size_t)row * 3 + c;
     // rhs = 0; rhs += (M u)/dt; rhs(P1) -= G p; rhs += conv; rhs += pen*u_bc
     double v = 0.0 + acc / dt;
@@ -223,10 +346,7 @@ void spmv3(hipStream_t s, const DevCsr &A, const T *val, const T *x, T *y,
   op.val = val;
   op.x = x;
   op.y = y;
-  const int G = stream_grid<T, 3>(A.n);
-  hipLaunchKernelGGL((k_csr_stream<T, 3, 0, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
-                     A.n, A.rowptr, op, nullptr);
-  CFSA_LAUNCH_CHECK();
+  (void)launch_spmv<T, 3, 0>(s, A, op, nullptr);
 }
 
 template <class T>
@@ -239,11 +359,8 @@ void spmv3_dot(hipStream_t s, const DevCsr &A, const T *val, const T *y, T *v,
   op.x = y;
   op.y = v;
   op.r0 = r0;
-  const int G = stream_grid<T, 3>(A.n);
+  const int G = launch_spmv<T, 3, 1>(s, A, op, partials);
   *grid = G;
-  hipLaunchKernelGGL((k_csr_stream<T, 3, 1, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
-                     A.n, A.rowptr, op, partials);
-  CFSA_LAUNCH_CHECK();
 }
 
 template <class T>
@@ -256,11 +373,8 @@ void spmv3_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
   op.x = z;
   op.y = t;
   op.sv = sv;
-  const int G = stream_grid<T, 3>(A.n);
+  const int G = launch_spmv<T, 3, 2>(s, A, op, partials);
   *grid = G;
-  hipLaunchKernelGGL((k_csr_stream<T, 3, 2, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
-                     A.n, A.rowptr, op, partials);
-  CFSA_LAUNCH_CHECK();
 }
 
 template <class T>
@@ -275,11 +389,8 @@ void spmv3_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
   op.y = r;
   op.b = b;
   op.r0 = r0;
-  const int G = stream_grid<T, 3>(A.n);
+  const int G = launch_spmv<T, 3, 1>(s, A, op, partials);
   *grid = G;
-  hipLaunchKernelGGL((k_csr_stream<T, 3, 1, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
-                     A.n, A.rowptr, op, partials);
-  CFSA_LAUNCH_CHECK();
 }
 
 template <class T>
@@ -290,10 +401,7 @@ void spmv1(hipStream_t s, const DevCsr &A, const T *val, const T *x, T *y) {
   op.val = val;
   op.x = x;
   op.y = y;
-  const int G = stream_grid<T, 1>(A.n);
-  hipLaunchKernelGGL((k_csr_stream<T, 1, 0, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
-                     A.n, A.rowptr, op, nullptr);
-  CFSA_LAUNCH_CHECK();
+  (void)launch_spmv<T, 1, 0>(s, A, op, nullptr);
 }
 
 template <class T>
@@ -306,11 +414,8 @@ void spmv1_dot(hipStream_t s, const DevCsr &A, const T *val, const T *y, T *v,
   op.x = y;
   op.y = v;
   op.r0 = r0;
-  const int G = stream_grid<T, 1>(A.n);
+  const int G = launch_spmv<T, 1, 1>(s, A, op, partials);
   *grid = G;
-  hipLaunchKernelGGL((k_csr_stream<T, 1, 1, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
-                     A.n, A.rowptr, op, partials);
-  CFSA_LAUNCH_CHECK();
 }
 
 template <class T>
@@ -323,11 +428,8 @@ void spmv1_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
   op.x = z;
   op.y = t;
   op.sv = sv;
-  const int G = stream_grid<T, 1>(A.n);
+  const int G = launch_spmv<T, 1, 2>(s, A, op, partials);
   *grid = G;
-  hipLaunchKernelGGL((k_csr_stream<T, 1, 2, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
-                     A.n, A.rowptr, op, partials);
-  CFSA_LAUNCH_CHECK();
 }
 
 template <class T>
@@ -342,11 +444,8 @@ void spmv1_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
   op.y = r;
   op.b = b;
   op.r0 = r0;
-  const int G = stream_grid<T, 1>(A.n);
+  const int G = launch_spmv<T, 1, 1>(s, A, op, partials);
   *grid = G;
-  hipLaunchKernelGGL((k_csr_stream<T, 1, 1, decltype(op)>), dim3(G), dim3(kBlock), 0, s,
-                     A.n, A.rowptr, op, partials);
-  CFSA_LAUNCH_CHECK();
 }
 
 void spmv_grad3(hipStream_t s, const DevCsr &P, const double *gval, const double *x,
@@ -377,7 +476,7 @@ struct OpGradT3 : Mask1Base {
     const double *gk = g + (size_t)k * 3;
     p[0] = gk[0] * xj[0] + gk[1] * xj[1] + gk[2] * xj[2];
   }
-  __device__ void epi(int row, int, double acc, double *) const {
+  __device__ void epi(int row, int, double acc, DD *) const {
     y[row] = (beta != 0.0 ? beta * y[row] : 0.0) + alpha * acc;
   }
 };
@@ -401,7 +500,7 @@ void spmv_gradT_in3(hipStream_t s, const DevCsr &P, const double *gT, const doub
 
 // y[i][c] += sum_j G_c[i,j] x[j] over the P1 rows of a [*][3] array
 struct OpGrad3Acc : OpGrad3 {
-  __device__ void epi(int row, int c, double acc, double *) const {
+  __device__ void epi(int row, int c, double acc, DD *) const {
     y[(size_t)row * 3 + c] += acc;
   }
 };
@@ -434,7 +533,7 @@ struct OpMomAdj : MaskBase {
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = a * lb[(size_t)j * 3 + c];
   }
-  __device__ void epi(int row, int c, double acc, double *) const {
+  __device__ void epi(int row, int c, double acc, DD *) const {
     const size_t i = (size_t)row * 3 + c;
     lu[i] = acc / dt;
     if (bc[row] == kInlet) lg[i] += pen * lb[i];
@@ -505,50 +604,6 @@ CFSA_INST_SPMV(float)
 // ===========================================================================
 int vec_grid(int n) { return std::max(1, std::min(div_up(n, kBlock), kVecGrid)); }
 
-// Fixed-shape tree over one block; writes partials[v * gridDim.x + block].
-template <int NV>
-__device__ void block_partials(double (&v)[NV], double *__restrict__ partials) {
-  __shared__ double red[NV][kBlock];
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) red[i][tid] = v[i];
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (tid < s) {
-#pragma unroll
-      for (int i = 0; i < NV; ++i) red[i][tid] += red[i][tid + s];
-    }
-    __syncthreads();
-  }
-  if (tid < NV) partials[(size_t)tid * gridDim.x + blockIdx.x] = red[tid][0];
-}
-
-// Single-block final reduction of NV partial arrays of length G into out[].
-template <int NV>
-__device__ void reduce_final(const double *__restrict__ partials, int G, double *out) {
-  __shared__ double red[NV][kBlock];
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double s = 0.0;
-    for (int g = tid; g < G; g += kBlock) s += partials[(size_t)i * G + g];
-    red[i][tid] = s;
-  }
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (tid < s) {
-#pragma unroll
-      for (int i = 0; i < NV; ++i) red[i][tid] += red[i][tid + s];
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) out[i] = red[i][0];
-  }
-  __syncthreads();
-}
-
 // ===========================================================================
 // BiCGSTAB (Eigen's right-preconditioned van der Vorst form)
 // ===========================================================================
@@ -561,9 +616,9 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(int n, const T *__restrict
                                                       double *__restrict__ partials) {
   // partials: [2c] = stopping norm of b (|b|^2, or |D^-1 b|^2 when sc != null),
   //           [2c+1] = |b|^2 = r0.r of the first iteration
-  double acc[2 * K];
+  DD acc[2 * K];
 #pragma unroll
-  for (int i = 0; i < 2 * K; ++i) acc[i] = 0.0;
+  for (int i = 0; i < 2 * K; ++i) dd_zero(acc[i]);
   for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
        e += (size_t)gridDim.x * kBlock) {
     const double de = sc ? (double)sc[e] : 1.0;
@@ -576,10 +631,9 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(int n, const T *__restrict
       r0[i] = bi;
       p[i] = T(0);
       v[i] = T(0);
-      const double d = (double)bi * (double)bi;
-      const double ds = de * (double)bi;
-      acc[2 * c] += sc ? ds * ds : d;
-      acc[2 * c + 1] += d;
+      const double ds = sc ? de * (double)bi : (double)bi;
+      dd_fma(acc[2 * c], ds, ds);
+      dd_fma(acc[2 * c + 1], (double)bi, (double)bi);
     }
   }
   block_partials<2 * K>(acc, partials);
@@ -774,14 +828,14 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v3(
   if (!st->any_active) return;
   int act[K];
   double al[K], om[K];
-  double acc[2 * K];
+  DD acc[2 * K];
 #pragma unroll
   for (int c = 0; c < K; ++c) {
     act[c] = st->active[c];
     al[c] = st->alpha[c];
     om[c] = st->omega[c];
-    acc[2 * c] = 0.0;
-    acc[2 * c + 1] = 0.0;
+    dd_zero(acc[2 * c]);
+    dd_zero(acc[2 * c + 1]);
   }
   for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
        e += (size_t)gridDim.x * kBlock) {
@@ -793,9 +847,9 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v3(
       x[i] = (T)((double)x[i] + (al[c] * (double)y[i] + om[c] * (double)z[i]));
       const T ri = (T)((double)sv[i] - om[c] * (double)t[i]);
       r[i] = ri;
-      const double rs = de * (double)ri;
-      acc[2 * c] += sc ? rs * rs : (double)ri * (double)ri;
-      acc[2 * c + 1] += (double)r0[i] * (double)ri;
+      const double rs = sc ? de * (double)ri : (double)ri;
+      dd_fma(acc[2 * c], rs, rs);
+      dd_fma(acc[2 * c + 1], (double)r0[i], (double)ri);
     }
   }
   block_partials<2 * K>(acc, partials);
@@ -896,7 +950,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int n, const T *__restrict__
                                                     T *__restrict__ x, T *__restrict__ r,
                                                     T *__restrict__ p, int scaled,
                                                     double *__restrict__ partials) {
-  double acc[2] = {0.0, 0.0};
+  DD acc[2];
+  dd_zero(acc[0]);
+  dd_zero(acc[1]);
   for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n;
        i += (size_t)gridDim.x * kBlock) {
     const T bi = b[i];
@@ -904,8 +960,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int n, const T *__restrict__
     r[i] = bi;
     const T pi = (T)((double)invd[i] * (double)bi);
     p[i] = pi;
-    acc[0] += scaled ? (double)pi * (double)pi : (double)bi * (double)bi;
-    acc[1] += (double)bi * (double)pi;
+    const double sn = scaled ? (double)pi : (double)bi;
+    dd_fma(acc[0], sn, sn);
+    dd_fma(acc[1], (double)bi, (double)pi);
   }
   block_partials<2>(acc, partials);
 }
@@ -986,7 +1043,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__re
                                                   double *__restrict__ partials) {
   if (!st->active[0]) return;
   const double al = st->alpha[0];
-  double acc[2] = {0.0, 0.0};
+  DD acc[2];
+  dd_zero(acc[0]);
+  dd_zero(acc[1]);
   for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n;
        i += (size_t)gridDim.x * kBlock) {
     x[i] = (T)((double)x[i] + al * (double)p[i]);
@@ -994,8 +1053,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__re
     r[i] = ri;
     const T zi = (T)((double)invd[i] * (double)ri);
     z[i] = zi;
-    acc[0] += scaled ? (double)zi * (double)zi : (double)ri * (double)ri;
-    acc[1] += (double)ri * (double)zi;
+    const double sn = scaled ? (double)zi : (double)ri;
+    dd_fma(acc[0], sn, sn);
+    dd_fma(acc[1], (double)ri, (double)zi);
   }
   block_partials<2>(acc, partials);
 }
@@ -1088,26 +1148,15 @@ __device__ __forceinline__ double dot3(const double *a, const double *b) {
 
 constexpr int kEdgeV[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
 
-// One thread per (tet, local node a): it owns global row node[a] for this tet.
-// Tets of one colour share no vertex, so no two threads touch the same row.
+// Row node[a] of tet t: adds the tet's entries (a, 0..9) of M and K and, for a
+// vertex (a < 4), its entries (a, 0..3) of L and G_x, G_y, G_z.
 // reference: buildMassMatrix (NavierStokes.cpp:227-310), buildStiffnessMatrix
 // (:135-225), buildGradientMatrices (:70-133), buildPoissonMatrix (:316-356).
-__global__ __launch_bounds__(kBlock) void k_assemble(
-    int ntc, const int *__restrict__ order, const int *__restrict__ tets,
-    const int *__restrict__ ten, int nv, const double *__restrict__ vol,
-    const double *__restrict__ grad, double rho, double inv_mu,
-    const int *__restrict__ Arp, const int *__restrict__ Acol, double *__restrict__ M,
-    double *__restrict__ K, const int *__restrict__ Prp, const int *__restrict__ Pcol,
-    double *__restrict__ L, double *__restrict__ G3) {
-  const int idx = blockIdx.x * kBlock + threadIdx.x;
-  const int tl = idx / 10, a = idx % 10;
-  if (tl >= ntc) return;
-  const int t = order[tl];
-  int node[10];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) node[i] = tets[(size_t)t * 4 + i];
-#pragma unroll
-  for (int e = 0; e < 6; ++e) node[4 + e] = nv + ten[(size_t)t * 6 + e];
+__device__ __forceinline__ void tet_row_entries(
+    int t, int a, const int (&node)[10], const double *__restrict__ vol, const double *__restrict__ grad, double rho,
+    double inv_mu, const int *__restrict__ Arp, const int *__restrict__ Acol,
+    double *__restrict__ M, double *__restrict__ K, const int *__restrict__ Prp,
+    const int *__restrict__ Pcol, double *__restrict__ L, double *__restrict__ G3) {
   double g[10][3];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -1146,6 +1195,72 @@ __global__ __launch_bounds__(kBlock) void k_assemble(
       for (int c = 0; c < 3; ++c) G3[(size_t)slot * 3 + c] += contrib * g[b][c];
     }
   }
+}
+
+__device__ __forceinline__ void tet_nodes(int t, const int *__restrict__ tets,
+                                          const int *__restrict__ ten, int nv, int (&node)[10]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) node[i] = tets[(size_t)t * 4 + i];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) node[4 + e] = nv + ten[(size_t)t * 6 + e];
+}
+
+// Row gather: one thread per velocity node r walks the tets containing r in
+// ascending tet order, so every matrix entry is summed in the order Eigen's
+// setFromTriplets sums the reference's duplicates (triplets are inserted tet
+// by tet), and the assembled values equal the reference's bit for bit.
+// Atomic-free: each thread writes only its own row's slots.
+__global__ __launch_bounds__(kBlock) void k_assemble_rows(
+    int nu, const int *__restrict__ incp, const int *__restrict__ inct,
+    const int *__restrict__ tets, const int *__restrict__ ten, int nv,
+    const double *__restrict__ vol, const double *__restrict__ grad, double rho,
+    double inv_mu, const int *__restrict__ Arp, const int *__restrict__ Acol,
+    double *__restrict__ M, double *__restrict__ K, const int *__restrict__ Prp,
+    const int *__restrict__ Pcol, double *__restrict__ L, double *__restrict__ G3) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nu) return;
+  for (int k = incp[r]; k < incp[r + 1]; ++k) {
+    const int t = inct[k];
+    int node[10];
+    tet_nodes(t, tets, ten, nv, node);
+    int a = 0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+      if (node[i] == r) a = i;
+    tet_row_entries(t, a, node, vol, grad, rho, inv_mu, Arp, Acol, M, K, Prp, Pcol, L,
+                    G3);
+  }
+}
+
+void assemble_rows(hipStream_t s, int nu, const int *incp, const int *inct, const int *tets,
+                   const int *ten, int nv, const double *vol, const double *grad, double rho,
+                   double inv_mu, const DevCsr &A, double *M, double *K, const DevCsr &P,
+                   double *L, double *G3) {
+  if (nu <= 0) return;
+  hipLaunchKernelGGL(k_assemble_rows, dim3(div_up(nu, kBlock)), dim3(kBlock), 0, s, nu, incp,
+                     inct, tets, ten, nv, vol, grad, rho, inv_mu, A.rowptr, A.col, M, K,
+                     P.rowptr, P.col, L, G3);
+  CFSA_LAUNCH_CHECK();
+}
+
+// Coloured scatter: one thread per (tet, local node a) of one colour; tets of
+// one colour share no vertex, so no two threads touch the same row.  Entries
+// are summed in colour order, which differs from the reference's tet order in
+// the last bits.
+__global__ __launch_bounds__(kBlock) void k_assemble(
+    int ntc, const int *__restrict__ order, const int *__restrict__ tets,
+    const int *__restrict__ ten, int nv, const double *__restrict__ vol,
+    const double *__restrict__ grad, double rho, double inv_mu,
+    const int *__restrict__ Arp, const int *__restrict__ Acol, double *__restrict__ M,
+    double *__restrict__ K, const int *__restrict__ Prp, const int *__restrict__ Pcol,
+    double *__restrict__ L, double *__restrict__ G3) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  const int tl = idx / 10, a = idx % 10;
+  if (tl >= ntc) return;
+  const int t = order[tl];
+  int node[10];
+  tet_nodes(t, tets, ten, nv, node);
+  tet_row_entries(t, a, node, vol, grad, rho, inv_mu, Arp, Acol, M, K, Prp, Pcol, L, G3);
 }
 
 void assemble_colour(hipStream_t s, int ntc, const int *order, const int *tets,

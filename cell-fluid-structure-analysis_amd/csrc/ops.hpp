@@ -6,12 +6,24 @@
 
 namespace cfsa {
 
+// A device matrix pattern.  With sptr == nullptr it is CSR (rowptr, col) and
+// value arrays are in CSR slot order.  With sptr set it is the sliced-ELL copy
+// (col = per stored entry, sptr/rlen/perm as in SellPattern) and value arrays
+// are in stored-entry order; the SpMV launchers pick the kernel from that.
 struct DevCsr {
   int n = 0;
   int64_t nnz = 0;
   const int *rowptr = nullptr;
   const int *col = nullptr;
+  const int *sptr = nullptr;
+  const int *rlen = nullptr;
+  const int *perm = nullptr;
+  int nslices = 0;
 };
+
+// out[e] = src[e] >= 0 ? (T)in[src[e]] : 0 over the stored entries
+template <class T>
+void sell_fill(hipStream_t s, int64_t stored, const int *src, const double *in, T *out);
 
 // ---- CSR-stream SpMV family (3 interleaved right-hand sides unless noted) ----
 template <class T>
@@ -108,6 +120,10 @@ void assemble_colour(hipStream_t s, int ntc, const int *order, const int *tets,
                      const int *ten, int nv, const double *vol, const double *grad,
                      double rho, double inv_mu, const DevCsr &A, double *M, double *K,
                      const DevCsr &P, double *L, double *G3);
+void assemble_rows(hipStream_t s, int nu, const int *incp, const int *inct, const int *tets,
+                   const int *ten, int nv, const double *vol, const double *grad, double rho,
+                   double inv_mu, const DevCsr &A, double *M, double *K, const DevCsr &P,
+                   double *L, double *G3);
 void lumped_mass_inverse(hipStream_t s, const DevCsr &A, const double *M, double *ilm);
 void build_lhs(hipStream_t s, int64_t nnz, const double *M, const double *K, double dt,
                double *lhs);

@@ -31,6 +31,60 @@ void parallel_chunks(int64_t n, F &&f) {
 
 }  // namespace
 
+SellPattern build_sell(const CsrPattern &P) {
+  SellPattern S;
+  const int64_t n = P.n;
+  S.n = n;
+  const int64_t ns = (n + kSellSlice - 1) / kSellSlice;
+  if (ns > INT32_MAX) throw std::runtime_error("sliced-ELL: too many slices");
+  S.nslices = (int)ns;
+  const int64_t npos = ns * kSellSlice;
+  S.rlen.assign((size_t)npos, 0);
+  S.perm.assign((size_t)npos, 0);
+  std::vector<int32_t> idx;
+  for (int64_t w0 = 0; w0 < n; w0 += kSellWindow) {
+    const int64_t w1 = std::min(n, w0 + kSellWindow);
+    idx.resize((size_t)(w1 - w0));
+    for (int64_t i = w0; i < w1; ++i) idx[(size_t)(i - w0)] = (int32_t)i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
+      return P.rowptr[(size_t)a + 1] - P.rowptr[(size_t)a] >
+             P.rowptr[(size_t)b + 1] - P.rowptr[(size_t)b];
+    });
+    for (int64_t k = 0; k < w1 - w0; ++k) {
+      const int32_t r = idx[(size_t)k];
+      S.perm[(size_t)(w0 + k)] = r;
+      S.rlen[(size_t)(w0 + k)] = P.rowptr[(size_t)r + 1] - P.rowptr[(size_t)r];
+    }
+  }
+  S.sptr.assign((size_t)ns + 1, 0);
+  int64_t total = 0;
+  for (int64_t s = 0; s < ns; ++s) {
+    int32_t mx = 0;
+    for (int l = 0; l < kSellSlice; ++l)
+      mx = std::max(mx, S.rlen[(size_t)(s * kSellSlice + l)]);
+    total += (int64_t)mx * kSellSlice;
+    if (total > INT32_MAX) throw std::runtime_error("sliced-ELL exceeds int32 offsets");
+    S.sptr[(size_t)s + 1] = (int32_t)total;
+  }
+  S.col.assign((size_t)total, 0);
+  S.src.assign((size_t)total, -1);
+  parallel_chunks(ns, [&](int, int64_t a, int64_t b) {
+    for (int64_t s = a; s < b; ++s)
+      for (int l = 0; l < kSellSlice; ++l) {
+        const int64_t p = s * kSellSlice + l;
+        if (p >= n) continue;
+        const int32_t r = S.perm[(size_t)p];
+        const int32_t k0 = P.rowptr[(size_t)r];
+        for (int32_t j = 0; j < S.rlen[(size_t)p]; ++j) {
+          const size_t e = (size_t)S.sptr[(size_t)s] + (size_t)j * kSellSlice + (size_t)l;
+          S.col[e] = P.col[(size_t)(k0 + j)];
+          S.src[e] = k0 + j;
+        }
+      }
+  });
+  return S;
+}
+
 Incidence build_incidence(const std::vector<int32_t> &elem_nodes, int npt,
                           int64_t n_nodes) {
   const int64_t nt = (int64_t)elem_nodes.size() / npt;

@@ -49,4 +49,25 @@ struct Colouring {
 };
 Colouring colour_tets(const int32_t *tets, int64_t nt, int64_t nv);
 
+// Sliced-ELL copy of a CSR pattern for the Krylov SpMV: rows are grouped in
+// slices of kSellSlice (one wavefront, one row per lane); inside a window of
+// kSellWindow consecutive rows, rows are stably ordered by decreasing length
+// so the lanes of a slice have nearly equal lengths.  Entry j of the row at
+// position p = s * kSellSlice + l is stored at sptr[s] + j * kSellSlice + l,
+// in the CSR order of that row (ascending columns), so a lane that walks
+// j = 0, 1, ... sums its row in the same order as the CSR product.
+constexpr int kSellSlice = 64;
+constexpr int kSellWindow = 4096;  // multiple of kSellSlice
+struct SellPattern {
+  int64_t n = 0;
+  int nslices = 0;
+  std::vector<int32_t> sptr;  // nslices + 1 offsets of the slices' entries
+  std::vector<int32_t> rlen;  // per position (nslices * kSellSlice), 0 past n
+  std::vector<int32_t> perm;  // per position: CSR row (0 past n)
+  std::vector<int32_t> col;   // per stored entry (0 in padding)
+  std::vector<int32_t> src;   // per stored entry: CSR slot, -1 in padding
+  int64_t stored() const { return sptr.empty() ? 0 : sptr.back(); }
+};
+SellPattern build_sell(const CsrPattern &P);
+
 }  // namespace cfsa

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <iostream>
 
+#include "dd.hpp"
 #include "mesh.hpp"
 #include "pattern.hpp"
 
@@ -38,25 +39,6 @@ __global__ void k_accumulate(int64_t N, const float *__restrict__ d, double *__r
        i += (int64_t)gridDim.x * kBlock)
     x[i] = x[i] + (double)d[i];
 }
-template <int NV>
-__global__ void k_reduce_out(const double *__restrict__ partials, int G, double *out) {
-  __shared__ double red[NV][kBlock];
-  const int tid = threadIdx.x;
-  for (int i = 0; i < NV; ++i) {
-    double s = 0.0;
-    for (int g = tid; g < G; g += kBlock) s += partials[(size_t)i * G + g];
-    red[i][tid] = s;
-  }
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (tid < s)
-      for (int i = 0; i < NV; ++i) red[i][tid] += red[i][tid + s];
-    __syncthreads();
-  }
-  if (tid == 0)
-    for (int i = 0; i < NV; ++i) out[i] = red[i][0];
-}
-
 constexpr double kMomentumPenalty = 1e10;  // NavierStokes.cpp:463
 constexpr double kPressurePenalty = 1e12;  // NavierStokes.cpp:611,619
 
@@ -67,10 +49,10 @@ NSSolver::NSSolver() {
   CFSA_HIP(hipHostMalloc((void **)&h_state, sizeof(KrylovState), hipHostMallocDefault));
   std::memset(h_state, 0, sizeof(KrylovState));
   d_state.alloc(1);
-  d_part_a.alloc((size_t)6 * kMaxPartials);
-  d_part_b.alloc((size_t)6 * kMaxPartials);
-  d_part_c.alloc((size_t)6 * kMaxPartials);
-  d_part_r.alloc((size_t)6 * kMaxPartials);
+  d_part_a.alloc((size_t)12 * kMaxPartials);
+  d_part_b.alloc((size_t)12 * kMaxPartials);
+  d_part_c.alloc((size_t)12 * kMaxPartials);
+  d_part_r.alloc((size_t)12 * kMaxPartials);
   d_mask.alloc(3);
   d_flag.alloc(1);
   d_scalar.alloc(8);
@@ -160,10 +142,32 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   nnzP = PP.nnz();
   A_ = DevCsr{nu, nnzA, d_Arp.p, d_Acol.p};
   P_ = DevCsr{nv, nnzP, d_Prp.p, d_Pcol.p};
+  {
+    SellPattern SA = build_sell(PA);
+    d_Asptr.upload(SA.sptr.data(), SA.sptr.size());
+    d_Arlen.upload(SA.rlen.data(), SA.rlen.size());
+    d_Aperm.upload(SA.perm.data(), SA.perm.size());
+    d_Ascol.upload(SA.col.data(), SA.col.size());
+    d_Asrc.upload(SA.src.data(), SA.src.size());
+    storedA = SA.stored();
+    As_ = DevCsr{nu, nnzA, nullptr, d_Ascol.p, d_Asptr.p, d_Arlen.p, d_Aperm.p, SA.nslices};
+  }
+  {
+    SellPattern SP = build_sell(PP);
+    d_Psptr.upload(SP.sptr.data(), SP.sptr.size());
+    d_Prlen.upload(SP.rlen.data(), SP.rlen.size());
+    d_Pperm.upload(SP.perm.data(), SP.perm.size());
+    d_Pscol.upload(SP.col.data(), SP.col.size());
+    d_Psrc.upload(SP.src.data(), SP.src.size());
+    storedP = SP.stored();
+    Ps_ = DevCsr{nv, nnzP, nullptr, d_Pscol.p, d_Psptr.p, d_Prlen.p, d_Pperm.p, SP.nslices};
+  }
 
   d_M.alloc((size_t)nnzA);
   d_K.alloc((size_t)nnzA);
   d_lhs.alloc((size_t)nnzA);
+  d_lhsS.alloc((size_t)storedA);
+  d_LbcS.alloc((size_t)storedP);
   d_L.alloc((size_t)nnzP);
   d_Lbc.alloc((size_t)nnzP);
   d_G3.alloc((size_t)nnzP * 3);
@@ -236,10 +240,15 @@ void NSSolver::assemble() {
   d_L.zero(stream_);
   d_G3.zero(stream_);
   const double inv_mu = 1.0 / viscosity;  // NavierStokes.cpp:153
-  for (int c = 0; c < ncolours; ++c) {
-    const int a = h_colour_offsets[(size_t)c], b = h_colour_offsets[(size_t)c + 1];
-    assemble_colour(stream_, b - a, d_order.p + a, d_tets.p, d_ten.p, nv, d_vol.p, d_grad.p,
-                    density, inv_mu, A_, d_M.p, d_K.p, P_, d_L.p, d_G3.p);
+  if (opt.assembly == 0) {
+    assemble_rows(stream_, nu, d_incp.p, d_inct.p, d_tets.p, d_ten.p, nv, d_vol.p, d_grad.p,
+                  density, inv_mu, A_, d_M.p, d_K.p, P_, d_L.p, d_G3.p);
+  } else {
+    for (int c = 0; c < ncolours; ++c) {
+      const int a = h_colour_offsets[(size_t)c], b = h_colour_offsets[(size_t)c + 1];
+      assemble_colour(stream_, b - a, d_order.p + a, d_tets.p, d_ten.p, nv, d_vol.p,
+                      d_grad.p, density, inv_mu, A_, d_M.p, d_K.p, P_, d_L.p, d_G3.p);
+    }
   }
   lumped_mass_inverse(stream_, A_, d_M.p, d_ilm.p);
   lhs_dirty = lbc_dirty = true;
@@ -253,9 +262,9 @@ void NSSolver::build_sparse_matrices() {
 }
 
 void NSSolver::ensure_mixed_buffers() {
-  if (d_lhs32.n == (size_t)nnzA && d_Lbc32.n == (size_t)nnzP) return;
-  d_lhs32.alloc((size_t)nnzA);
-  d_Lbc32.alloc((size_t)nnzP);
+  if (d_lhs32.n == (size_t)storedA && d_Lbc32.n == (size_t)storedP) return;
+  d_lhs32.alloc((size_t)storedA);
+  d_Lbc32.alloc((size_t)storedP);
   d_invdA32.alloc((size_t)nu);
   d_invdL32.alloc((size_t)nv);
   f3_.alloc((size_t)nu * 3);
@@ -268,13 +277,14 @@ void NSSolver::ensure_mixed_buffers() {
 // lhs = M/dt + K with 1e10 on Inlet/Wall diagonals (NavierStokes.cpp:460-495);
 // L_bc = L with 1e12 on Outlet (or reference-node) diagonals (:603-626).
 void NSSolver::prepare_operators() {
-  if (opt.precision == 1) ensure_mixed_buffers();
+  if (opt.precision != 0) ensure_mixed_buffers();
   if (lhs_dirty || lhs_dt != dt) {
     build_lhs(stream_, nnzA, d_M.p, d_K.p, dt, d_lhs.p);
     add_diag_penalty(stream_, nu, d_Adiag.p, d_bc.p, 0, -1, kMomentumPenalty, d_lhs.p);
     inv_diag(stream_, nu, d_Adiag.p, d_lhs.p, d_invdA.p);
-    if (d_lhs32.n == (size_t)nnzA) {
-      convert<float, double>(stream_, nnzA, d_lhs.p, d_lhs32.p);
+    sell_fill<double>(stream_, storedA, d_Asrc.p, d_lhs.p, d_lhsS.p);
+    if (d_lhs32.n == (size_t)storedA) {
+      sell_fill<float>(stream_, storedA, d_Asrc.p, d_lhs.p, d_lhs32.p);
       convert<float, double>(stream_, nu, d_invdA.p, d_invdA32.p);
     }
     lhs_dirty = false;
@@ -289,8 +299,9 @@ void NSSolver::prepare_operators() {
       add_diag_penalty(stream_, nv, d_Pdiag.p, d_bc.p, 2, reference_node, kPressurePenalty,
                        d_Lbc.p);
     inv_diag(stream_, nv, d_Pdiag.p, d_Lbc.p, d_invdL.p);
-    if (d_Lbc32.n == (size_t)nnzP) {
-      convert<float, double>(stream_, nnzP, d_Lbc.p, d_Lbc32.p);
+    sell_fill<double>(stream_, storedP, d_Psrc.p, d_Lbc.p, d_LbcS.p);
+    if (d_Lbc32.n == (size_t)storedP) {
+      sell_fill<float>(stream_, storedP, d_Psrc.p, d_Lbc.p, d_Lbc32.p);
       convert<float, double>(stream_, nv, d_invdL.p, d_invdL32.p);
     }
     lbc_dirty = false;
@@ -481,7 +492,7 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const float *
       spmv3_residual<double>(s, A, val64, x, b, d_r64.p, w64.r0.p, d_mask.p, d_part_r.p, &Gr);
     else
       spmv1_residual<double>(s, A, val64, x, b, d_r64.p, w64.r0.p, d_mask.p, d_part_r.p, &Gr);
-    hipLaunchKernelGGL((k_reduce_out<K>), dim3(1), dim3(kBlock), 0, s, d_part_r.p, Gr,
+    hipLaunchKernelGGL((k_reduce_pairs<K>), dim3(1), dim3(kBlock), 0, s, d_part_r.p, Gr,
                        d_scalar.p);
     CFSA_LAUNCH_CHECK();
     double out[3] = {0, 0, 0};
@@ -526,20 +537,23 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const float *
   return res;
 }
 
+// precision 1: FP32 inner solves for both systems; 2: for the momentum solve only
 SolveResult NSSolver::solve_momentum(const double *b, double *x) {
-  if (opt.precision == 1)
-    return refine<3>(A_, d_lhs.p, d_lhs32.p, d_invdA32.p, b, x, opt.momentum_tol, opt.max_iter,
-                     0);
-  return bicgstab<double, 3>(A_, d_lhs.p, d_invdA.p, b, x, opt.momentum_tol, opt.max_iter, w3_);
+  if (opt.precision == 1 || opt.precision == 2)
+    return refine<3>(As_, d_lhsS.p, d_lhs32.p, d_invdA32.p, b, x, opt.momentum_tol,
+                     opt.max_iter, 0);
+  return bicgstab<double, 3>(As_, d_lhsS.p, d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
+                             w3_);
 }
 
 SolveResult NSSolver::solve_pressure(const double *b, double *x) {
   if (opt.precision == 1)
-    return refine<1>(P_, d_Lbc.p, d_Lbc32.p, d_invdL32.p, b, x, opt.pressure_tol, opt.max_iter,
-                     opt.pressure_solver);
+    return refine<1>(Ps_, d_LbcS.p, d_Lbc32.p, d_invdL32.p, b, x, opt.pressure_tol,
+                     opt.max_iter, opt.pressure_solver);
   if (opt.pressure_solver == 1)
-    return pcg<double>(P_, d_Lbc.p, d_invdL.p, b, x, opt.pressure_tol, opt.max_iter, w1_);
-  return bicgstab<double, 1>(P_, d_Lbc.p, d_invdL.p, b, x, opt.pressure_tol, opt.max_iter, w1_);
+    return pcg<double>(Ps_, d_LbcS.p, d_invdL.p, b, x, opt.pressure_tol, opt.max_iter, w1_);
+  return bicgstab<double, 1>(Ps_, d_LbcS.p, d_invdL.p, b, x, opt.pressure_tol, opt.max_iter,
+                             w1_);
 }
 
 // ---------------------------------------------------------------------------
@@ -794,10 +808,14 @@ void NSSolver::apply_matrix(int which, const double *x, double *y, int nrhs) {
   }
   dx.upload(x, (size_t)n * nrhs);
   dy.alloc((size_t)n * nrhs);
+  // the Krylov operators run through their sliced-ELL copies
+  const DevCsr &A = which == 2 ? As_ : which == 4 ? Ps_ : p2 ? A_ : P_;
+  if (which == 2) val = d_lhsS.p;
+  if (which == 4) val = d_LbcS.p;
   if (nrhs == 3)
-    spmv3<double>(stream_, p2 ? A_ : P_, val, dx.p, dy.p, nullptr);
+    spmv3<double>(stream_, A, val, dx.p, dy.p, nullptr);
   else
-    spmv1<double>(stream_, p2 ? A_ : P_, val, dx.p, dy.p);
+    spmv1<double>(stream_, A, val, dx.p, dy.p);
   CFSA_HIP(hipStreamSynchronize(stream_));
   dy.download(y, (size_t)n * nrhs);
 }
@@ -814,7 +832,7 @@ SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double
   opt.precision = precision;
   prepare_operators();
   opt.precision = saved;
-  const DevCsr &A = which == 2 ? A_ : P_;
+  const DevCsr &A = which == 2 ? As_ : Ps_;
   const int n = A.n;
   DBuf<double> db, dx;
   db.upload(b, (size_t)n * nrhs);
@@ -822,18 +840,18 @@ SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double
   SolveResult r;
   if (precision == 1) {
     if (nrhs == 3)
-      r = refine<3>(A, d_lhs.p, d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit, 0);
+      r = refine<3>(A, d_lhsS.p, d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit, 0);
     else if (which == 2)
       throw std::invalid_argument("krylov_solve: mixed momentum needs 3 rhs");
     else
-      r = refine<1>(A, d_Lbc.p, d_Lbc32.p, d_invdL32.p, db.p, dx.p, tol, maxit, method);
+      r = refine<1>(A, d_LbcS.p, d_Lbc32.p, d_invdL32.p, db.p, dx.p, tol, maxit, method);
   } else if (method == 1) {
-    r = pcg<double>(A, which == 2 ? d_lhs.p : d_Lbc.p, which == 2 ? d_invdA.p : d_invdL.p,
+    r = pcg<double>(A, which == 2 ? d_lhsS.p : d_LbcS.p, which == 2 ? d_invdA.p : d_invdL.p,
                     db.p, dx.p, tol, maxit, which == 2 ? w3_ : w1_);
   } else if (nrhs == 3) {
-    r = bicgstab<double, 3>(A, d_lhs.p, d_invdA.p, db.p, dx.p, tol, maxit, w3_);
+    r = bicgstab<double, 3>(A, d_lhsS.p, d_invdA.p, db.p, dx.p, tol, maxit, w3_);
   } else {
-    r = bicgstab<double, 1>(A, which == 2 ? d_lhs.p : d_Lbc.p,
+    r = bicgstab<double, 1>(A, which == 2 ? d_lhsS.p : d_LbcS.p,
                             which == 2 ? d_invdA.p : d_invdL.p, db.p, dx.p, tol, maxit,
                             which == 2 ? w3_ : w1_);
   }
@@ -854,9 +872,9 @@ double NSSolver::time_momentum_spmv(int reps, int precision) {
   CFSA_HIP(hipEventRecord(a, stream_));
   for (int i = 0; i < reps; ++i) {
     if (precision == 1)
-      spmv3<float>(stream_, A_, d_lhs32.p, f3_.y.p, f3_.v.p, nullptr);
+      spmv3<float>(stream_, As_, d_lhs32.p, f3_.y.p, f3_.v.p, nullptr);
     else
-      spmv3<double>(stream_, A_, d_lhs.p, w3_.y.p, w3_.v.p, nullptr);
+      spmv3<double>(stream_, As_, d_lhsS.p, w3_.y.p, w3_.v.p, nullptr);
   }
   CFSA_HIP(hipEventRecord(b, stream_));
   CFSA_HIP(hipEventSynchronize(b));

@@ -63,12 +63,13 @@ struct SolverOptions {
   double pressure_tol = 1e-8;  // NavierStokes.cpp:635
   int max_iter = 1000;         // NavierStokes.cpp:499,634
   int pressure_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG
-  int precision = 0;           // 0 FP64, 1 FP32 inner solves + FP64 refinement
+  int precision = 0;           // 0 FP64, 1 FP32 inner + FP64 refinement, 2 momentum only
   double inner_tol = 1e-4;     // relative tolerance of each FP32 inner solve
   int max_refine = 40;         // outer refinement sweeps (mixed)
   int check_every = 8;         // Krylov iterations between host status reads
   int timing = 0;              // 1: per-phase events; 2: also per-SpMV events
   int stop_scaled = 0;         // 1: stop on |D^-1 r| <= tol |D^-1 b| (Jacobi-scaled)
+  int assembly = 0;            // 0 row gather (reference summation order), 1 coloured
 };
 
 struct SolveResult {
@@ -200,8 +201,14 @@ class NSSolver {
   // patterns
   DBuf<int> d_Arp, d_Acol, d_Adiag, d_Prp, d_Pcol, d_Pdiag;
   DevCsr A_{}, P_{};
-  // operators
+  // sliced-ELL copies of the two Krylov operators' patterns (lhs, L_bc)
+  DBuf<int> d_Asptr, d_Arlen, d_Aperm, d_Ascol, d_Asrc;
+  DBuf<int> d_Psptr, d_Prlen, d_Pperm, d_Pscol, d_Psrc;
+  DevCsr As_{}, Ps_{};
+  int64_t storedA = 0, storedP = 0;
+  // operators (CSR slot order; the *S and *32 arrays are in sliced-ELL order)
   DBuf<double> d_M, d_K, d_lhs, d_L, d_Lbc, d_G3, d_ilm, d_invdA, d_invdL;
+  DBuf<double> d_lhsS, d_LbcS;
   DBuf<float> d_lhs32, d_Lbc32, d_invdA32, d_invdL32;
   bool lhs_dirty = true, lbc_dirty = true;
   double lhs_dt = 0.0;

@@ -54,7 +54,8 @@ typedef struct cfsa_solver_options {
   double pressure_tol;   /* 1e-8 */
   int max_iter;          /* 1000 */
   int pressure_solver;   /* 0 BiCGSTAB (reference), 1 Jacobi PCG */
-  int precision;         /* 0 FP64, 1 FP32 inner solves in FP64 iterative refinement */
+  int precision;         /* 0 FP64, 1 FP32 inner solves in FP64 iterative refinement for the
+                            momentum and pressure solves, 2 for the momentum solve only */
   double inner_tol;      /* relative tolerance of each FP32 inner solve, 1e-4 */
   int max_refine;        /* refinement sweeps, 40 */
   int check_every;       /* Krylov iterations between host convergence reads, 8 */
@@ -62,6 +63,9 @@ typedef struct cfsa_solver_options {
   int stop_rule;         /* 0 |r| <= tol |b| (reference), 1 |D^-1 r| <= tol |D^-1 b| with
                             D the Jacobi diagonal (for adjoint / finite-difference runs,
                             where the 1e10 penalty rows dominate |b|) */
+  int assembly;          /* 0 row gather in ascending tet order (sums every entry in the
+                            reference's order: bit-exact matrices), 1 graph-coloured
+                            element scatter (same values up to rounding order) */
 } cfsa_solver_options;
 
 typedef struct cfsa_step_stats {

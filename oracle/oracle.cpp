@@ -346,8 +346,32 @@ double eigen_dot(const double *a, const double *b, size_t n) {
   for (size_t i = as; i < n; ++i) res = res + a[i] * b[i];
   return res;
 }
+// Dot2 of Ogita, Rump & Oishi ("Accurate Sum and Dot Product", SIAM J. Sci.
+// Comput. 26(6), 2005): exact products (TwoProduct via fma) and error-free
+// additions (TwoSum) accumulated in (p, s); the result p + s is as accurate as
+// a sum in twice the working precision, so its rounding does not depend on
+// the order of the terms (up to values within ~n*u^2*sum|a_i b_i| of a
+// rounding boundary).  Used, sequentially, for every Krylov inner product.
+double dot2(const double *a, const double *b, size_t n) {
+  double p = 0.0, s = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double h = a[i] * b[i];
+    const double r = std::fma(a[i], b[i], -h);
+    const double t = p + h;
+    const double z = t - p;
+    const double q = (p - (t - z)) + (h - z);
+    p = t;
+    s = s + (q + r);
+  }
+  return p + s;
+}
+
+// 0: Dot2 (default; what the GPU library evaluates), 1: Eigen's packet order
+int g_dot_mode = 0;
+
 double edot(const std::vector<double> &a, const std::vector<double> &b) {
-  return eigen_dot(a.data(), b.data(), a.size());
+  return g_dot_mode == 1 ? eigen_dot(a.data(), b.data(), a.size())
+                         : dot2(a.data(), b.data(), a.size());
 }
 
 struct KResult {
@@ -1065,6 +1089,9 @@ extern "C" {
 
 const char *orc_last_error(void) { return g_err.c_str(); }
 
+double orc_dot2(const double *a, const double *b, int64_t n) { return dot2(a, b, (size_t)n); }
+void orc_set_dot_mode(int mode) { g_dot_mode = mode == 1 ? 1 : 0; }
+int orc_get_dot_mode(void) { return g_dot_mode; }
 double orc_eigen_dot(const double *a, const double *b, int64_t n) {
   return eigen_dot(a, b, (size_t)n);
 }

@@ -98,6 +98,12 @@ int orc_csr_krylov(int64_t n, const int32_t *rp, const int32_t *col, const doubl
 
 /* Eigen SSE2 reduction order, exposed for the known-answer tests */
 double orc_eigen_dot(const double *a, const double *b, int64_t n);
+/* Order-robust inner product (Dot2) used by the Krylov restatements. */
+double orc_dot2(const double *a, const double *b, int64_t n);
+/* Inner products of the Krylov restatements: 0 Dot2 (default), 1 Eigen's
+ * 2-wide packet order (a build-dependent order of the reference's Eigen). */
+void orc_set_dot_mode(int mode);
+int orc_get_dot_mode(void);
 /* y = A x in Eigen's sparse*dense accumulation order (nrhs interleaved) */
 void orc_csr_spmv(int64_t n, const int32_t *rp, const int32_t *col, const double *val,
                   const double *x, double *y, int nrhs);

@@ -28,6 +28,9 @@ def _sig(name, res, *args):
 
 _sig("orc_last_error", C.c_char_p)
 _sig("orc_eigen_dot", C.c_double, _f64p, _f64p, C.c_int64)
+_sig("orc_dot2", C.c_double, _f64p, _f64p, C.c_int64)
+_sig("orc_set_dot_mode", None, C.c_int)
+_sig("orc_get_dot_mode", C.c_int)
 _sig("orc_csr_spmv", None, C.c_int64, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int)
 _sig("orc_mesh_structured_prism", C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
      C.c_int, C.POINTER(_p))
@@ -78,6 +81,22 @@ def eigen_dot(a, b) -> float:
     a = np.ascontiguousarray(a, np.float64)
     b = np.ascontiguousarray(b, np.float64)
     return _lib.orc_eigen_dot(a, b, a.size)
+
+
+def dot2(a, b) -> float:
+    """Order-robust inner product (Ogita-Rump-Oishi Dot2), sequential."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return _lib.orc_dot2(a, b, a.size)
+
+
+def set_dot_mode(mode: int) -> None:
+    """Inner products of the Krylov restatements: 0 Dot2, 1 Eigen packet order."""
+    _lib.orc_set_dot_mode(int(mode))
+
+
+def get_dot_mode() -> int:
+    return int(_lib.orc_get_dot_mode())
 
 
 def csr_spmv(rp, col, val, x):

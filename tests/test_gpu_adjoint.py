@@ -21,12 +21,12 @@ def inlet_basis(X, ax_lo=(0.5, 0.5), scale=0.5):
     return B
 
 
-def gpu_solver(P, pm, mu, ncorr=1):
+def gpu_solver(P, pm, mu, ncorr=1, tol=TOL):
     g = P.NavierStokesSolver()
     g.initialize(pm, 1060.0, mu)
     g.setDt(0.01)
     g.setNumberOfCorrections(ncorr)
-    g.setOptions(momentum_tol=TOL, pressure_tol=TOL, stop_rule=1, max_iter=5000)
+    g.setOptions(momentum_tol=tol, pressure_tol=tol, stop_rule=1, max_iter=5000)
     return g
 
 
@@ -60,9 +60,12 @@ def test_adjoint_matches_oracle_adjoint(P, O, setup, ncorr):
 
 
 def test_adjoint_matches_central_differences(P, setup):
+    """Normwise: |fd_k - dJ/dc_k| <= 1e-5 max_j |dJ/dc_j|.  The FD quotient
+    carries an absolute noise of about tol * |J| / h from the forward solves,
+    so a component far below the largest one is not resolved relatively."""
     pm, om, B, c = setup
     steps = 4
-    g = gpu_solver(P, pm, 0.004)
+    g = gpu_solver(P, pm, 0.004, tol=1e-14)
     J, dc, dmu = g.adjointGradient(steps, B, c)
     h = 1e-6
     for k in range(4):
@@ -70,10 +73,10 @@ def test_adjoint_matches_central_differences(P, setup):
         cp[k] += h
         cm[k] -= h
         fd = (g.adjointGradient(steps, B, cp)[0] - g.adjointGradient(steps, B, cm)[0]) / (2 * h)
-        assert abs(fd - dc[k]) <= 1e-5 * max(abs(dc[k]), 1e-3 * np.abs(dc).max()), (k, fd, dc[k])
-    mu, hm = 0.004, 1e-6 * 0.004
-    Jp = gpu_solver(P, pm, mu + hm).adjointGradient(steps, B, c)[0]
-    Jm = gpu_solver(P, pm, mu - hm).adjointGradient(steps, B, c)[0]
+        assert abs(fd - dc[k]) <= 1e-5 * np.abs(dc).max(), (k, fd, dc[k])
+    mu, hm = 0.004, 1e-6  # h = 1e-6 for every parameter (2.5e-4 relative for mu)
+    Jp = gpu_solver(P, pm, mu + hm, tol=1e-14).adjointGradient(steps, B, c)[0]
+    Jm = gpu_solver(P, pm, mu - hm, tol=1e-14).adjointGradient(steps, B, c)[0]
     fdmu = (Jp - Jm) / (2 * hm)
     assert abs(fdmu - dmu) <= 1e-5 * abs(dmu), (fdmu, dmu)
 

@@ -2,12 +2,14 @@
 against the CPU oracle on the same mesh and inputs.
 
 Bars: integer work (patterns, colouring, incidence) bit-exact; kernels whose
-floating-point order reproduces the reference (SpMV, convection, divergence)
-bit-exact; assembled matrix values within 1e-13 relative (the coloured
-scatter sums element contributions in colour order instead of tet order);
-Krylov solves and whole steps: equal iteration counts and velocity /
-pressure relative L2 error <= 1e-8 per step (the FP64 tolerance of the
-step parity requirement).
+floating-point order reproduces the reference (row-gather assembly, SpMV,
+convection, divergence) bit-exact; the optional coloured scatter assembly
+within 1e-13 relative (it sums element contributions in colour order instead
+of tet order); Krylov solves and whole steps: equal iteration counts and
+velocity / pressure relative L2 error <= 1e-8 per step (the FP64 tolerance of
+the step parity requirement).  Inner products are order-robust (Dot2) on both
+sides, so in practice the FP64 iterates agree bit for bit; the tests report
+the largest difference seen.
 """
 import numpy as np
 import pytest
@@ -87,9 +89,8 @@ def test_patterns_values_colouring(P, O, case, request):
         rg, cg, vg = g.getMatrix(name)
         ro, co, vo = o.getMatrix(name)
         assert np.array_equal(rg, ro) and np.array_equal(cg, co), name
-        scale = np.abs(vo).max()
-        np.testing.assert_allclose(vg, vo, rtol=1e-13, atol=1e-15 * scale, err_msg=name)
-    np.testing.assert_allclose(g.getInvLumpedMass(), o.getInvLumpedMass(), rtol=1e-13)
+        assert np.array_equal(vg, vo), name
+    assert np.array_equal(g.getInvLumpedMass(), o.getInvLumpedMass())
     col, nc = O.colour_tets(pm.tets(), pm.nVertices())
     assert np.array_equal(g.colouring(), col) and g.numColours() == nc
     ptr, tet = g.incidence()
@@ -97,6 +98,23 @@ def test_patterns_values_colouring(P, O, case, request):
     for node in np.random.default_rng(0).integers(0, len(ptr) - 1, 50):
         want = np.nonzero((T == node).any(axis=1))[0]
         assert np.array_equal(tet[ptr[node]:ptr[node + 1]], want)
+
+
+@pytest.mark.parametrize("case", ["kuhn4", "cyl"])
+def test_coloured_assembly_close(P, O, case, request):
+    g, o, pm, om = request.getfixturevalue(case)
+    g.setOptions(assembly=1)
+    try:
+        g.build_sparse_matrices()
+        for name in ["M", "K", "L", "Gx", "Gy", "Gz"]:
+            vg = g.getMatrix(name)[2]
+            vo = o.getMatrix(name)[2]
+            np.testing.assert_allclose(vg, vo, rtol=1e-13, atol=1e-15 * np.abs(vo).max(),
+                                       err_msg=name)
+    finally:
+        g.setOptions(assembly=0)
+        g.build_sparse_matrices()
+    assert np.array_equal(g.getMatrix("K")[2], o.getMatrix("K")[2])
 
 
 @pytest.mark.parametrize("case", ["kuhn4", "cyl"])
@@ -136,13 +154,13 @@ def test_krylov_parity(P, O, case, request):
     for c in range(3):
         xo, io = o.krylovSolve("lhs", b[:, c], method=0, tol=1e-10)
         assert info["iters"][c] == io["iters"], (c, info, io)
-        assert relerr(x[:, c], xo) < 1e-12
+        assert relerr(x[:, c], xo) <= 1e-12, (c, relerr(x[:, c], xo))
     bp = rng.uniform(-1, 1, sz["nv"])
     for method in (0, 1):
         xp, ip = g.krylovSolve("Lbc", bp, method=method, tol=1e-9)
         xo, io = o.krylovSolve("Lbc", bp, method=method, tol=1e-9)
         assert ip["iters"][0] == io["iters"], (method, ip, io)
-        assert relerr(xp, xo) < 1e-11
+        assert relerr(xp, xo) <= 1e-12, (method, relerr(xp, xo))
     x0, i0 = g.krylovSolve("Lbc", np.zeros(sz["nv"]))
     assert i0["iters"][0] == 0 and not x0.any()
 
@@ -218,7 +236,10 @@ def test_step_before_initialize(P):
 
 
 def test_momentum_noconvergence_reported(P, O):
+    # step 1 from rest: the y/z right-hand sides are zero and x is penalty
+    # dominated, so momentum converges; step 2 needs more than one iteration
     g, o, pm, om = make_case(P, O, "kuhn", 4)
+    assert g.pisoStep()
     g.setOptions(max_iter=1)
     assert g.pisoStepStatus() == 2
     assert "Momentum" in P.last_error()
@@ -227,7 +248,8 @@ def test_momentum_noconvergence_reported(P, O):
 def test_nan_reported(P, O):
     g, o, pm, om = make_case(P, O, "kuhn", 4)
     u = g.getVelocity()
-    u[pm.p1BC() == 3] = np.nan
+    lab = np.r_[pm.p1BC(), pm.p2BC()]
+    u[lab == 3] = np.nan
     g.setVelocity(u)
     st = g.pisoStepStatus()
     assert st in (2, 4)

@@ -223,6 +223,51 @@ def test_eigen_dot_reduction_order(O):
         assert abs(O.eigen_dot(a, b) - np.dot(a, b)) < 1e-12 * max(1, abs(np.dot(a, b)))
 
 
+def test_dot2_correctly_rounded_and_order_free(O):
+    """Dot2 against the exactly rounded value of the exact inner product
+    (rational arithmetic); a permutation of the terms changes nothing; and a
+    cancellation that defeats the plain loop is resolved."""
+    from fractions import Fraction
+    rng = np.random.default_rng(11)
+    for trial in range(30):
+        n = int(rng.integers(1, 600))
+        a = rng.standard_normal(n) * 10.0 ** rng.integers(-4, 5, n)
+        b = rng.standard_normal(n) * 10.0 ** rng.integers(-4, 5, n)
+        exact = sum((Fraction(x) * Fraction(y) for x, y in zip(a, b)), Fraction(0))
+        want = float(exact)
+        assert O.dot2(a, b) == want, trial
+        perm = rng.permutation(n)
+        assert O.dot2(a[perm], b[perm]) == want, trial
+    a = np.array([1e16, 1.0, -1e16])
+    b = np.ones(3)
+    assert O.dot2(a, b) == 1.0 and O.eigen_dot(a, b) == 0.0
+    assert O.dot2(np.zeros(0), np.zeros(0)) == 0.0
+
+
+def test_step_with_eigen_dot_order_agrees(O):
+    """The Krylov restatements with Eigen's packet-order inner products and with
+    Dot2 converge to the same step at tight tolerances (the inner-product
+    rounding only moves the iterates at the level of the tolerance)."""
+    out = []
+    try:
+        for mode in (1, 0):
+            O.set_dot_mode(mode)
+            m = _prism(O, 3, 1.0, 1.0, 1.0)
+            m.setupBoundaryConditions()
+            ns = O.OracleNS(m, 1060.0, 0.004)
+            ns.setDt(0.01)
+            ns.setMeanInletVelocity((0.5, 0.0, 0.0))
+            ns.setSolver(1e-13, 1e-13, 5000, 0)
+            ns.setStopScaled(True)
+            for _ in range(3):
+                assert ns.pisoStep() == 0
+            out.append((ns.getVelocity(), ns.getPressure()))
+    finally:
+        O.set_dot_mode(0)
+    for a, b in zip(out[0], out[1]):
+        assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b)
+
+
 def test_krylov_matches_dense_solve(O, ns3):
     m, ns = ns3
     sz = ns.sizes()
