@@ -183,6 +183,17 @@ def main():
         res["spmv_fp32"] = {"avg_ms": spmv32_ms, "GB/s": b32 / spmv32_ms / 1e6}
         primary = "mixed"
     res["spmv_fp64"] = {"avg_ms": spmv64_ms, "GB/s": b64 / spmv64_ms / 1e6}
+    # both Krylov SpMV layouts on the momentum operator (3 right-hand sides)
+    lay = {}
+    for layout, name in ((1, "csr_stream"), (2, "sliced_ell")):
+        g64.setOptions(spmv_layout=layout)
+        t64 = g64.timeMomentumSpmv(20, 0)
+        t32 = g64.timeMomentumSpmv(20, 1)
+        lay[name] = {"fp64_ms": t64, "fp64_GB/s": b64 / t64 / 1e6, "fp32_ms": t32,
+                     "fp32_GB/s": spmv_bytes(sz["nu"], sz["nnzA"], 4) / t32 / 1e6}
+    g64.setOptions(spmv_layout=0)
+    res["spmv_layouts"] = lay
+    log(f"spmv layouts: {lay}")
 
     if primary == "mixed":
         ach = b32 / spmv32_ms / 1e6

@@ -19,32 +19,28 @@ namespace cfsa {
 // ===========================================================================
 // CSR-stream core
 // ===========================================================================
-template <class T, int KO>
-struct StreamCfg;
-template <>
-struct StreamCfg<double, 3> {
-  static constexpr int R = 85, B = 2048;  // 48 KiB of staged products
-};
-template <>
-struct StreamCfg<double, 1> {
-  static constexpr int R = 256, B = 4096;  // 32 KiB
-};
-template <>
-struct StreamCfg<float, 3> {
-  static constexpr int R = 85, B = 4096;  // 48 KiB
-};
-template <>
-struct StreamCfg<float, 1> {
-  static constexpr int R = 256, B = 8192;  // 32 KiB
-};
+// Rows are cut into windows (host-built, see stream_windows) of at most
+// kStreamB<KO> nonzeros, one block per window.  The block first loads the
+// window's columns and values (clamped indices, so every load is issued
+// before any is used), then gathers x and stages the products in LDS, then
+// sums them per (row, component), kBlock / KO rows at a time.
+//
+// Inner products: each block reduces its rows' pairs and stores them in
+// wpart[v][window]; k_window_reduce then merges each group of kStreamGroup
+// consecutive windows in a fixed tree order into partials[v][group].  (A
+// last-block-done merge inside this kernel would need a device-scope release
+// per block, which writes back the XCD's whole L2 and costs far more.)
+constexpr int kStreamGroup = 256;
 
 template <class T, int KO, int ND, class Op>
-__global__ __launch_bounds__(kBlock) void k_csr_stream(int n,
-                                                       const int *__restrict__ rowptr,
-                                                       Op op,
-                                                       double *__restrict__ partials) {
-  constexpr int R = StreamCfg<T, KO>::R, B = StreamCfg<T, KO>::B;
+__global__ __launch_bounds__(kBlock) void k_csr_stream(const int *__restrict__ rowptr,
+                                                       const int *__restrict__ win, int nwin,
+                                                       Op op, double *__restrict__ wpart) {
+  constexpr int B = kStreamB<KO>;
+  constexpr int R = kBlock / KO;   // rows summed per pass
+  constexpr int PER = B / kBlock;  // staged entries per thread
   constexpr int NDs = ND > 0 ? ND : 1;
+  constexpr int NV = KO * ND;
   // staged products; reused for the block's inner-product pairs at the end
   __shared__ T lds[B * KO];
   static_assert(sizeof(T) * B * KO >= sizeof(double) * 2 * NDs * kBlock, "LDS reuse");
@@ -54,36 +50,48 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(int n,
   for (int d = 0; d < NDs; ++d) dd_zero(dots[d]);
   const int tid = threadIdx.x;
   const int myc = tid % KO;
-  const bool act = op.active(myc);
-  const int nrb = (n + R - 1) / R;
-  for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
-    const int r0 = rb * R, r1 = min(n, r0 + R);
-    const int k0 = rowptr[r0], k1 = rowptr[r1];
-    const bool owner = tid < KO * (r1 - r0);
-    const int row = r0 + tid / KO;
-    int rs = 0, re = 0;
-    if (owner) {
-      rs = rowptr[row];
-      re = rowptr[row + 1];
-    }
-    T acc = T(0);
-    for (int kb = k0; kb < k1; kb += B) {
-      const int len = min(B, k1 - kb);
-      __syncthreads();
-#pragma unroll 4
-      for (int i = tid; i < len; i += kBlock) {
-        T p[KO];
-        op.prod(kb + i, p);
+  const bool act = op.active(myc) && tid < KO * R;
+  const int w = blockIdx.x;
+  const int r0 = win[w], r1 = win[w + 1];
+  const int k0 = rowptr[r0];
+  const int len = rowptr[r1] - k0;  // <= B by construction of the windows
+  const int row0 = r0 + tid / KO;
+  typename Op::Pre pre0{};
+  if (act && row0 < r1) pre0 = op.pre(row0, myc);
+  if (len > 0) {
+    typename Op::Frag f[PER];
 #pragma unroll
-        for (int c = 0; c < KO; ++c) lds[i * KO + c] = p[c];
-      }
-      __syncthreads();
-      if (owner && act) {
-        const int a = max(rs, kb), b = min(re, kb + len);
-        for (int k = a; k < b; ++k) acc += lds[(k - kb) * KO + myc];
+    for (int u = 0; u < PER; ++u) f[u] = op.fetch(k0 + min(tid + u * kBlock, len - 1));
+    T p[PER][KO];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) op.prod(f[u], p[u]);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = tid + u * kBlock;
+      if (i < len) {
+#pragma unroll
+        for (int c = 0; c < KO; ++c) lds[i * KO + c] = p[u][c];
       }
     }
-    if (owner && act) op.epi(row, myc, acc, dots);
+  }
+  __syncthreads();
+  for (int row = row0; row < r1; row += R) {
+    if (act) {
+      const typename Op::Pre pr = row == row0 ? pre0 : op.pre(row, myc);
+      const int ra = rowptr[row] - k0, rb = rowptr[row + 1] - k0;
+      // sequential sum in ascending column order; LDS reads are batched
+      T acc = T(0);
+      int k = ra;
+      for (; k + 8 <= rb; k += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = lds[(k + u) * KO + myc];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u];
+      }
+      for (; k < rb; ++k) acc += lds[k * KO + myc];
+      op.epi(row, myc, acc, pr, dots);
+    }
   }
   if constexpr (ND > 0) {
     double *rh = reinterpret_cast<double *>(lds);  // [ND][kBlock]
@@ -95,23 +103,47 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(int n,
       rl[d * kBlock + tid] = dots[d].lo;
     }
     __syncthreads();
-    if (tid < KO * ND) {
-      // per component c: threads c, c+KO, ... in ascending order
-      const int c = tid / ND, d = tid % ND;
+    // value v = (component c, product d) owns 32 lanes (half a wavefront):
+    // lane l merges the threads c + KO * (l + 32 q), then a fixed shuffle tree
+    static_assert(NV * 32 <= kBlock, "lanes per value");
+    if (tid < NV * 32) {
+      const int v = tid / 32, l = tid % 32;
+      const int c = v / ND, d = v % ND;
       DD s;
       dd_zero(s);
-      for (int t = c; t < kBlock; t += KO) dd_merge(s, rh[d * kBlock + t], rl[d * kBlock + t]);
-      const size_t o = 2 * ((size_t)(c * ND + d) * gridDim.x + blockIdx.x);
-      partials[o] = s.hi;
-      partials[o + 1] = s.lo;
+      for (int t = c + KO * l; t < kBlock; t += KO * 32)
+        dd_merge(s, rh[d * kBlock + t], rl[d * kBlock + t]);
+#pragma unroll
+      for (int st = 16; st > 0; st >>= 1) {
+        const double oh = __shfl_down(s.hi, st, 32);
+        const double ol = __shfl_down(s.lo, st, 32);
+        dd_merge(s, oh, ol);
+      }
+      if (l == 0) {
+        wpart[2 * ((size_t)v * nwin + w)] = s.hi;
+        wpart[2 * ((size_t)v * nwin + w) + 1] = s.lo;
+      }
     }
   }
 }
 
-template <class T, int KO>
-static int stream_grid(int n) {
-  const int nrb = div_up(n, StreamCfg<T, KO>::R);
-  return std::max(1, std::min(nrb, kSpmvGrid));
+// partials[v][g] = merge of wpart[v][g * kStreamGroup + 0 .. 255] (tree order)
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_window_reduce(int nwin,
+                                                          const double *__restrict__ wpart,
+                                                          double *__restrict__ partials) {
+  static_assert(kStreamGroup == kBlock, "one window per thread");
+  DD v[NV];
+  const int w = blockIdx.x * kStreamGroup + threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    dd_zero(v[i]);
+    if (w < nwin) {
+      v[i].hi = wpart[2 * ((size_t)i * nwin + w)];
+      v[i].lo = wpart[2 * ((size_t)i * nwin + w) + 1];
+    }
+  }
+  block_partials<NV>(v, partials);
 }
 
 // ===========================================================================
@@ -165,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices,
     for (; j + U <= len; j += U) {
       T pr[U][KO];
 #pragma unroll
-      for (int u = 0; u < U; ++u) op.prod(base + (j + u) * kSellSlice, pr[u]);
+      for (int u = 0; u < U; ++u) op.prod(op.fetch(base + (j + u) * kSellSlice), pr[u]);
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -173,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices,
     }
     for (; j < len; ++j) {
       T pr[KO];
-      op.prod(base + j * kSellSlice, pr);
+      op.prod(op.fetch(base + j * kSellSlice), pr);
 #pragma unroll
       for (int c = 0; c < KO; ++c) acc[c] += pr[c];
     }
@@ -181,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices,
       const int row = perm[p];
 #pragma unroll
       for (int c = 0; c < KO; ++c)
-        if (act[c]) op.epi(row, c, acc[c], &dots[c * NDs]);
+        if (act[c]) op.epi(row, c, acc[c], op.pre(row, c), &dots[c * NDs]);
     }
   }
   if constexpr (ND > 0) block_partials<KO * ND>(dots, partials);
@@ -216,15 +248,32 @@ static int launch_spmv(hipStream_t s, const DevCsr &A, const Op &op, double *par
     CFSA_LAUNCH_CHECK();
     return G;
   }
-  const int G = stream_grid<T, KO>(A.n);
-  hipLaunchKernelGGL((k_csr_stream<T, KO, ND, Op>), dim3(G), dim3(kBlock), 0, s, A.n,
-                     A.rowptr, op, partials);
+  const int *win = KO == 3 ? A.win3 : A.win1;
+  const int nwin = KO == 3 ? A.nwin3 : A.nwin1;
+  if (!win || nwin < 1) throw std::logic_error("CSR pattern without stream windows");
+  const int G = (nwin + kStreamGroup - 1) / kStreamGroup;
+  if (ND > 0 && (!A.wpart || (int64_t)KO * ND * nwin > A.wpart_cap))
+    throw std::logic_error("CSR-stream inner products need window scratch");
+  if (G > kMaxPartials)
+    throw std::runtime_error("CSR-stream: matrix too large for the partial buffers");
+  hipLaunchKernelGGL((k_csr_stream<T, KO, ND, Op>), dim3(nwin), dim3(kBlock), 0, s, A.rowptr,
+                     win, nwin, op, A.wpart);
   CFSA_LAUNCH_CHECK();
+  if constexpr (ND > 0) {
+    // (a fully masked launch leaves stale pairs; its consumers skip them)
+    hipLaunchKernelGGL((k_window_reduce<KO * ND>), dim3(G), dim3(kBlock), 0, s, nwin, A.wpart,
+                       partials);
+    CFSA_LAUNCH_CHECK();
+  }
   return G;
 }
 
 // ---- operator functors -----------------------------------------------------
+// Pre: epilogue operands of a (row, component), loaded before the row's sum
+// so their latency overlaps the staging of the products.
 struct MaskBase {
+  struct Pre {};
+  __device__ Pre pre(int, int) const { return {}; }
   const int *mask;
   __device__ bool active(int c) const { return mask == nullptr || mask[c] != 0; }
   __device__ bool any() const {
@@ -232,6 +281,8 @@ struct MaskBase {
   }
 };
 struct Mask1Base {
+  struct Pre {};
+  __device__ Pre pre(int, int) const { return {}; }
   const int *mask;
   __device__ bool active(int) const { return mask == nullptr || mask[0] != 0; }
   __device__ bool any() const { return mask == nullptr || mask[0] != 0; }
@@ -243,13 +294,16 @@ struct OpAx : Base {
   const T *__restrict__ val;
   const T *__restrict__ x;
   T *__restrict__ y;
-  __device__ void prod(int k, T *p) const {
-    const int j = col[k];
-    const T a = val[k];
+  struct Frag {
+    int j;
+    T a;
+  };
+  __device__ Frag fetch(int k) const { return Frag{col[k], val[k]}; }
+  __device__ void prod(const Frag &f, T *p) const {
 #pragma unroll
-    for (int c = 0; c < KO; ++c) p[c] = a * x[(size_t)j * KO + c];
+    for (int c = 0; c < KO; ++c) p[c] = f.a * x[(size_t)f.j * KO + c];
   }
-  __device__ void epi(int row, int c, T acc, DD *) const {
+  __device__ void epi(int row, int c, T acc, const typename Base::Pre &, DD *) const {
     y[(size_t)row * KO + c] = acc;
   }
 };
@@ -257,22 +311,28 @@ struct OpAx : Base {
 template <class T, int KO, class Base>
 struct OpAxDot : OpAx<T, KO, Base> {
   const T *__restrict__ r0;
-  __device__ void epi(int row, int c, T acc, DD *d) const {
-    const size_t i = (size_t)row * KO + c;
-    this->y[i] = acc;
-    dd_fma(d[0], (double)r0[i], (double)acc);
+  struct Pre {
+    T r0;
+  };
+  __device__ Pre pre(int row, int c) const { return Pre{r0[(size_t)row * KO + c]}; }
+  __device__ void epi(int row, int c, T acc, const Pre &pr, DD *d) const {
+    this->y[(size_t)row * KO + c] = acc;
+    dd_fma(d[0], (double)pr.r0, (double)acc);
   }
 };
 
 template <class T, int KO, class Base>
 struct OpAxDot2 : OpAx<T, KO, Base> {
   const T *__restrict__ sv;
-  __device__ void epi(int row, int c, T acc, DD *d) const {
-    const size_t i = (size_t)row * KO + c;
-    this->y[i] = acc;
+  struct Pre {
+    T sv;
+  };
+  __device__ Pre pre(int row, int c) const { return Pre{sv[(size_t)row * KO + c]}; }
+  __device__ void epi(int row, int c, T acc, const Pre &pr, DD *d) const {
+    this->y[(size_t)row * KO + c] = acc;
     const double t = (double)acc;
     dd_fma(d[0], t, t);
-    dd_fma(d[1], t, (double)sv[i]);
+    dd_fma(d[1], t, (double)pr.sv);
   }
 };
 
@@ -280,9 +340,13 @@ template <class T, int KO, class Base>
 struct OpResidual : OpAx<T, KO, Base> {
   const T *__restrict__ b;
   T *__restrict__ r0;
-  __device__ void epi(int row, int c, T acc, DD *d) const {
+  struct Pre {
+    T b;
+  };
+  __device__ Pre pre(int row, int c) const { return Pre{b[(size_t)row * KO + c]}; }
+  __device__ void epi(int row, int c, T acc, const Pre &pr, DD *d) const {
     const size_t i = (size_t)row * KO + c;
-    const T r = b[i] - acc;
+    const T r = pr.b - acc;
     this->y[i] = r;
     r0[i] = r;
     dd_fma(d[0], (double)r, (double)r);
@@ -294,12 +358,19 @@ struct OpGrad3 : MaskBase {
   const double *__restrict__ g;
   const double *__restrict__ x;
   double *__restrict__ y;
-  __device__ void prod(int k, double *p) const {
-    const double xj = x[col[k]];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) p[c] = g[(size_t)k * 3 + c] * xj;
+  struct Frag {
+    int j;
+    double g[3];
+  };
+  __device__ Frag fetch(int k) const {
+    return Frag{col[k], {g[(size_t)k * 3], g[(size_t)k * 3 + 1], g[(size_t)k * 3 + 2]}};
   }
-  __device__ void epi(int row, int c, double acc, DD *) const {
+  __device__ void prod(const Frag &f, double *p) const {
+    const double xj = x[f.j];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p[c] = f.g[c] * xj;
+  }
+  __device__ void epi(int row, int c, double acc, const Pre &, DD *) const {
     y[(size_t)row * 3 + c] = acc;
   }
 };
@@ -315,13 +386,16 @@ struct OpMomRhs : MaskBase {
   double *__restrict__ rhs;
   double dt, penalty;
   int nv;
-  __device__ void prod(int k, double *p) const {
-    const int j = col[k];
-    const double a = M[k];
+  struct Frag {
+    int j;
+    double a;
+  };
+  __device__ Frag fetch(int k) const { return Frag{col[k], M[k]}; }
+  __device__ void prod(const Frag &f, double *p) const {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) p[c] = a * u[(size_t)j * 3 + c];
+    for (int c = 0; c < 3; ++c) p[c] = f.a * u[(size_t)f.j * 3 + c];
   }
-  __device__ void epi(int row, int c, double acc, DD *) const {
+  __device__ void epi(int row, int c, double acc, const Pre &, DD *) const {
     const size_t i = (size_t)row * 3 + c;
     // rhs = 0; rhs += (M u)/dt; rhs(P1) -= G p; rhs += conv; rhs += pen*u_bc
     double v = 0.0 + acc / dt;
@@ -456,10 +530,7 @@ void spmv_grad3(hipStream_t s, const DevCsr &P, const double *gval, const double
   op.g = gval;
   op.x = x;
   op.y = y;
-  const int G = stream_grid<double, 3>(P.n);
-  hipLaunchKernelGGL((k_csr_stream<double, 3, 0, OpGrad3>), dim3(G), dim3(kBlock), 0, s,
-                     P.n, P.rowptr, op, nullptr);
-  CFSA_LAUNCH_CHECK();
+  (void)launch_spmv<double, 3, 0>(s, P, op, nullptr);
 }
 
 // y = beta y + alpha sum_c G_c^T x_c with transposed values gT and x of
@@ -471,12 +542,18 @@ struct OpGradT3 : Mask1Base {
   double *__restrict__ y;
   int xs;
   double alpha, beta;
-  __device__ void prod(int k, double *p) const {
-    const double *xj = x + (size_t)col[k] * xs;
-    const double *gk = g + (size_t)k * 3;
-    p[0] = gk[0] * xj[0] + gk[1] * xj[1] + gk[2] * xj[2];
+  struct Frag {
+    int j;
+    double g[3];
+  };
+  __device__ Frag fetch(int k) const {
+    return Frag{col[k], {g[(size_t)k * 3], g[(size_t)k * 3 + 1], g[(size_t)k * 3 + 2]}};
   }
-  __device__ void epi(int row, int, double acc, DD *) const {
+  __device__ void prod(const Frag &f, double *p) const {
+    const double *xj = x + (size_t)f.j * xs;
+    p[0] = f.g[0] * xj[0] + f.g[1] * xj[1] + f.g[2] * xj[2];
+  }
+  __device__ void epi(int row, int, double acc, const Pre &, DD *) const {
     y[row] = (beta != 0.0 ? beta * y[row] : 0.0) + alpha * acc;
   }
 };
@@ -492,15 +569,12 @@ void spmv_gradT_in3(hipStream_t s, const DevCsr &P, const double *gT, const doub
   op.xs = xs;
   op.alpha = alpha;
   op.beta = beta;
-  const int G = stream_grid<double, 1>(P.n);
-  hipLaunchKernelGGL((k_csr_stream<double, 1, 0, OpGradT3>), dim3(G), dim3(kBlock), 0, s, P.n,
-                     P.rowptr, op, nullptr);
-  CFSA_LAUNCH_CHECK();
+  (void)launch_spmv<double, 1, 0>(s, P, op, nullptr);
 }
 
 // y[i][c] += sum_j G_c[i,j] x[j] over the P1 rows of a [*][3] array
 struct OpGrad3Acc : OpGrad3 {
-  __device__ void epi(int row, int c, double acc, DD *) const {
+  __device__ void epi(int row, int c, double acc, const Pre &, DD *) const {
     y[(size_t)row * 3 + c] += acc;
   }
 };
@@ -512,10 +586,7 @@ void spmv_grad3_acc(hipStream_t s, const DevCsr &P, const double *g, const doubl
   op.g = g;
   op.x = x;
   op.y = y;
-  const int G = stream_grid<double, 3>(P.n);
-  hipLaunchKernelGGL((k_csr_stream<double, 3, 0, OpGrad3Acc>), dim3(G), dim3(kBlock), 0, s,
-                     P.n, P.rowptr, op, nullptr);
-  CFSA_LAUNCH_CHECK();
+  (void)launch_spmv<double, 3, 0>(s, P, op, nullptr);
 }
 
 // adjoint of the momentum right-hand side: lu = (M lb)/dt, lg += pen lb on Inlet rows
@@ -527,13 +598,16 @@ struct OpMomAdj : MaskBase {
   double *__restrict__ lu;
   double *__restrict__ lg;
   double dt, pen;
-  __device__ void prod(int k, double *p) const {
-    const int j = col[k];
-    const double a = M[k];
+  struct Frag {
+    int j;
+    double a;
+  };
+  __device__ Frag fetch(int k) const { return Frag{col[k], M[k]}; }
+  __device__ void prod(const Frag &f, double *p) const {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) p[c] = a * lb[(size_t)j * 3 + c];
+    for (int c = 0; c < 3; ++c) p[c] = f.a * lb[(size_t)f.j * 3 + c];
   }
-  __device__ void epi(int row, int c, double acc, DD *) const {
+  __device__ void epi(int row, int c, double acc, const Pre &, DD *) const {
     const size_t i = (size_t)row * 3 + c;
     lu[i] = acc / dt;
     if (bc[row] == kInlet) lg[i] += pen * lb[i];
@@ -551,10 +625,7 @@ void momentum_rhs_adjoint(hipStream_t s, const DevCsr &A, const double *M, const
   op.lg = lg;
   op.dt = dt;
   op.pen = pen;
-  const int G = stream_grid<double, 3>(A.n);
-  hipLaunchKernelGGL((k_csr_stream<double, 3, 0, OpMomAdj>), dim3(G), dim3(kBlock), 0, s, A.n,
-                     A.rowptr, op, nullptr);
-  CFSA_LAUNCH_CHECK();
+  (void)launch_spmv<double, 3, 0>(s, A, op, nullptr);
 }
 
 void momentum_rhs(hipStream_t s, const DevCsr &A, const double *M, const double *u,
@@ -574,10 +645,7 @@ void momentum_rhs(hipStream_t s, const DevCsr &A, const double *M, const double 
   op.dt = dt;
   op.penalty = penalty;
   op.nv = nv;
-  const int G = stream_grid<double, 3>(A.n);
-  hipLaunchKernelGGL((k_csr_stream<double, 3, 0, OpMomRhs>), dim3(G), dim3(kBlock), 0, s,
-                     A.n, A.rowptr, op, nullptr);
-  CFSA_LAUNCH_CHECK();
+  (void)launch_spmv<double, 3, 0>(s, A, op, nullptr);
 }
 
 #define CFSA_INST_SPMV(T)                                                              \

@@ -143,6 +143,31 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   A_ = DevCsr{nu, nnzA, d_Arp.p, d_Acol.p};
   P_ = DevCsr{nv, nnzP, d_Prp.p, d_Pcol.p};
   {
+    const std::vector<int32_t> a1 = stream_windows(PA, kStreamB<1>);
+    const std::vector<int32_t> a3 = stream_windows(PA, kStreamB<3>);
+    const std::vector<int32_t> p1 = stream_windows(PP, kStreamB<1>);
+    const std::vector<int32_t> p3 = stream_windows(PP, kStreamB<3>);
+    d_Aw1.upload(a1.data(), a1.size());
+    d_Aw3.upload(a3.data(), a3.size());
+    d_Pw1.upload(p1.data(), p1.size());
+    d_Pw3.upload(p3.data(), p3.size());
+    A_.win1 = d_Aw1.p;
+    A_.win3 = d_Aw3.p;
+    A_.nwin1 = (int)a1.size() - 1;
+    A_.nwin3 = (int)a3.size() - 1;
+    P_.win1 = d_Pw1.p;
+    P_.win3 = d_Pw3.p;
+    P_.nwin1 = (int)p1.size() - 1;
+    P_.nwin3 = (int)p3.size() - 1;
+    const int64_t cap = std::max({(int64_t)2 * A_.nwin1, (int64_t)6 * A_.nwin3,
+                                  (int64_t)2 * P_.nwin1, (int64_t)6 * P_.nwin3});
+    d_wpart.alloc((size_t)(2 * cap));
+    for (DevCsr *X : {&A_, &P_}) {
+      X->wpart = d_wpart.p;
+      X->wpart_cap = cap;
+    }
+  }
+  {
     SellPattern SA = build_sell(PA);
     d_Asptr.upload(SA.sptr.data(), SA.sptr.size());
     d_Arlen.upload(SA.rlen.data(), SA.rlen.size());
@@ -166,8 +191,6 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   d_M.alloc((size_t)nnzA);
   d_K.alloc((size_t)nnzA);
   d_lhs.alloc((size_t)nnzA);
-  d_lhsS.alloc((size_t)storedA);
-  d_LbcS.alloc((size_t)storedP);
   d_L.alloc((size_t)nnzP);
   d_Lbc.alloc((size_t)nnzP);
   d_G3.alloc((size_t)nnzP * 3);
@@ -262,9 +285,11 @@ void NSSolver::build_sparse_matrices() {
 }
 
 void NSSolver::ensure_mixed_buffers() {
-  if (d_lhs32.n == (size_t)storedA && d_Lbc32.n == (size_t)storedP) return;
-  d_lhs32.alloc((size_t)storedA);
-  d_Lbc32.alloc((size_t)storedP);
+  const size_t na = (size_t)std::max<int64_t>(storedA, nnzA);
+  const size_t np = (size_t)std::max<int64_t>(storedP, nnzP);
+  if (d_lhs32.n == na && d_Lbc32.n == np) return;
+  d_lhs32.alloc(na);
+  d_Lbc32.alloc(np);
   d_invdA32.alloc((size_t)nu);
   d_invdL32.alloc((size_t)nv);
   f3_.alloc((size_t)nu * 3);
@@ -278,15 +303,30 @@ void NSSolver::ensure_mixed_buffers() {
 // L_bc = L with 1e12 on Outlet (or reference-node) diagonals (:603-626).
 void NSSolver::prepare_operators() {
   if (opt.precision != 0) ensure_mixed_buffers();
+  // Krylov SpMV layout: the momentum operator streams best as CSR-stream (its
+  // gathers of neighbouring columns share cache lines), the P1 pressure
+  // operator as sliced-ELL; option spmv_layout forces one of them.
+  const int la = opt.spmv_layout == 2 ? 1 : 0;
+  const int lp = opt.spmv_layout == 1 ? 0 : 1;
+  if (la != sellA_) lhs_dirty = true;
+  if (lp != sellP_) lbc_dirty = true;
+  const bool mixed = d_lhs32.n > 0;
   if (lhs_dirty || lhs_dt != dt) {
     build_lhs(stream_, nnzA, d_M.p, d_K.p, dt, d_lhs.p);
     add_diag_penalty(stream_, nu, d_Adiag.p, d_bc.p, 0, -1, kMomentumPenalty, d_lhs.p);
     inv_diag(stream_, nu, d_Adiag.p, d_lhs.p, d_invdA.p);
-    sell_fill<double>(stream_, storedA, d_Asrc.p, d_lhs.p, d_lhsS.p);
-    if (d_lhs32.n == (size_t)storedA) {
-      sell_fill<float>(stream_, storedA, d_Asrc.p, d_lhs.p, d_lhs32.p);
+    if (la) {
+      if (d_lhsS.n != (size_t)storedA) d_lhsS.alloc((size_t)storedA);
+      sell_fill<double>(stream_, storedA, d_Asrc.p, d_lhs.p, d_lhsS.p);
+    }
+    if (mixed) {
+      if (la)
+        sell_fill<float>(stream_, storedA, d_Asrc.p, d_lhs.p, d_lhs32.p);
+      else
+        convert<float, double>(stream_, nnzA, d_lhs.p, d_lhs32.p);
       convert<float, double>(stream_, nu, d_invdA.p, d_invdA32.p);
     }
+    sellA_ = la;
     lhs_dirty = false;
     lhs_dt = dt;
   }
@@ -299,11 +339,18 @@ void NSSolver::prepare_operators() {
       add_diag_penalty(stream_, nv, d_Pdiag.p, d_bc.p, 2, reference_node, kPressurePenalty,
                        d_Lbc.p);
     inv_diag(stream_, nv, d_Pdiag.p, d_Lbc.p, d_invdL.p);
-    sell_fill<double>(stream_, storedP, d_Psrc.p, d_Lbc.p, d_LbcS.p);
-    if (d_Lbc32.n == (size_t)storedP) {
-      sell_fill<float>(stream_, storedP, d_Psrc.p, d_Lbc.p, d_Lbc32.p);
+    if (lp) {
+      if (d_LbcS.n != (size_t)storedP) d_LbcS.alloc((size_t)storedP);
+      sell_fill<double>(stream_, storedP, d_Psrc.p, d_Lbc.p, d_LbcS.p);
+    }
+    if (mixed) {
+      if (lp)
+        sell_fill<float>(stream_, storedP, d_Psrc.p, d_Lbc.p, d_Lbc32.p);
+      else
+        convert<float, double>(stream_, nnzP, d_Lbc.p, d_Lbc32.p);
       convert<float, double>(stream_, nv, d_invdL.p, d_invdL32.p);
     }
+    sellP_ = lp;
     lbc_dirty = false;
   }
 }
@@ -540,19 +587,19 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const float *
 // precision 1: FP32 inner solves for both systems; 2: for the momentum solve only
 SolveResult NSSolver::solve_momentum(const double *b, double *x) {
   if (opt.precision == 1 || opt.precision == 2)
-    return refine<3>(As_, d_lhsS.p, d_lhs32.p, d_invdA32.p, b, x, opt.momentum_tol,
+    return refine<3>(momA(), momV(), d_lhs32.p, d_invdA32.p, b, x, opt.momentum_tol,
                      opt.max_iter, 0);
-  return bicgstab<double, 3>(As_, d_lhsS.p, d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
+  return bicgstab<double, 3>(momA(), momV(), d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
                              w3_);
 }
 
 SolveResult NSSolver::solve_pressure(const double *b, double *x) {
   if (opt.precision == 1)
-    return refine<1>(Ps_, d_LbcS.p, d_Lbc32.p, d_invdL32.p, b, x, opt.pressure_tol,
+    return refine<1>(preA(), preV(), d_Lbc32.p, d_invdL32.p, b, x, opt.pressure_tol,
                      opt.max_iter, opt.pressure_solver);
   if (opt.pressure_solver == 1)
-    return pcg<double>(Ps_, d_LbcS.p, d_invdL.p, b, x, opt.pressure_tol, opt.max_iter, w1_);
-  return bicgstab<double, 1>(Ps_, d_LbcS.p, d_invdL.p, b, x, opt.pressure_tol, opt.max_iter,
+    return pcg<double>(preA(), preV(), d_invdL.p, b, x, opt.pressure_tol, opt.max_iter, w1_);
+  return bicgstab<double, 1>(preA(), preV(), d_invdL.p, b, x, opt.pressure_tol, opt.max_iter,
                              w1_);
 }
 
@@ -809,9 +856,9 @@ void NSSolver::apply_matrix(int which, const double *x, double *y, int nrhs) {
   dx.upload(x, (size_t)n * nrhs);
   dy.alloc((size_t)n * nrhs);
   // the Krylov operators run through their sliced-ELL copies
-  const DevCsr &A = which == 2 ? As_ : which == 4 ? Ps_ : p2 ? A_ : P_;
-  if (which == 2) val = d_lhsS.p;
-  if (which == 4) val = d_LbcS.p;
+  const DevCsr &A = which == 2 ? momA() : which == 4 ? preA() : p2 ? A_ : P_;
+  if (which == 2) val = momV();
+  if (which == 4) val = preV();
   if (nrhs == 3)
     spmv3<double>(stream_, A, val, dx.p, dy.p, nullptr);
   else
@@ -832,7 +879,7 @@ SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double
   opt.precision = precision;
   prepare_operators();
   opt.precision = saved;
-  const DevCsr &A = which == 2 ? As_ : Ps_;
+  const DevCsr &A = which == 2 ? momA() : preA();
   const int n = A.n;
   DBuf<double> db, dx;
   db.upload(b, (size_t)n * nrhs);
@@ -840,18 +887,18 @@ SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double
   SolveResult r;
   if (precision == 1) {
     if (nrhs == 3)
-      r = refine<3>(A, d_lhsS.p, d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit, 0);
+      r = refine<3>(A, momV(), d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit, 0);
     else if (which == 2)
       throw std::invalid_argument("krylov_solve: mixed momentum needs 3 rhs");
     else
-      r = refine<1>(A, d_LbcS.p, d_Lbc32.p, d_invdL32.p, db.p, dx.p, tol, maxit, method);
+      r = refine<1>(A, preV(), d_Lbc32.p, d_invdL32.p, db.p, dx.p, tol, maxit, method);
   } else if (method == 1) {
-    r = pcg<double>(A, which == 2 ? d_lhsS.p : d_LbcS.p, which == 2 ? d_invdA.p : d_invdL.p,
+    r = pcg<double>(A, which == 2 ? momV() : preV(), which == 2 ? d_invdA.p : d_invdL.p,
                     db.p, dx.p, tol, maxit, which == 2 ? w3_ : w1_);
   } else if (nrhs == 3) {
-    r = bicgstab<double, 3>(A, d_lhsS.p, d_invdA.p, db.p, dx.p, tol, maxit, w3_);
+    r = bicgstab<double, 3>(A, momV(), d_invdA.p, db.p, dx.p, tol, maxit, w3_);
   } else {
-    r = bicgstab<double, 1>(A, which == 2 ? d_lhsS.p : d_LbcS.p,
+    r = bicgstab<double, 1>(A, which == 2 ? momV() : preV(),
                             which == 2 ? d_invdA.p : d_invdL.p, db.p, dx.p, tol, maxit,
                             which == 2 ? w3_ : w1_);
   }
@@ -872,9 +919,9 @@ double NSSolver::time_momentum_spmv(int reps, int precision) {
   CFSA_HIP(hipEventRecord(a, stream_));
   for (int i = 0; i < reps; ++i) {
     if (precision == 1)
-      spmv3<float>(stream_, As_, d_lhs32.p, f3_.y.p, f3_.v.p, nullptr);
+      spmv3<float>(stream_, momA(), d_lhs32.p, f3_.y.p, f3_.v.p, nullptr);
     else
-      spmv3<double>(stream_, As_, d_lhsS.p, w3_.y.p, w3_.v.p, nullptr);
+      spmv3<double>(stream_, momA(), momV(), w3_.y.p, w3_.v.p, nullptr);
   }
   CFSA_HIP(hipEventRecord(b, stream_));
   CFSA_HIP(hipEventSynchronize(b));

@@ -70,6 +70,7 @@ struct SolverOptions {
   int timing = 0;              // 1: per-phase events; 2: also per-SpMV events
   int stop_scaled = 0;         // 1: stop on |D^-1 r| <= tol |D^-1 b| (Jacobi-scaled)
   int assembly = 0;            // 0 row gather (reference summation order), 1 coloured
+  int spmv_layout = 0;         // Krylov SpMV: 0 auto, 1 CSR-stream, 2 sliced-ELL
 };
 
 struct SolveResult {
@@ -200,6 +201,8 @@ class NSSolver {
   std::vector<int32_t> h_tets, h_ten;
   // patterns
   DBuf<int> d_Arp, d_Acol, d_Adiag, d_Prp, d_Pcol, d_Pdiag;
+  DBuf<int> d_Aw1, d_Aw3, d_Pw1, d_Pw3;  // CSR-stream windows
+  DBuf<double> d_wpart;                  // CSR-stream inner-product scratch
   DevCsr A_{}, P_{};
   // sliced-ELL copies of the two Krylov operators' patterns (lhs, L_bc)
   DBuf<int> d_Asptr, d_Arlen, d_Aperm, d_Ascol, d_Asrc;
@@ -211,6 +214,12 @@ class NSSolver {
   DBuf<double> d_lhsS, d_LbcS;
   DBuf<float> d_lhs32, d_Lbc32, d_invdA32, d_invdL32;
   bool lhs_dirty = true, lbc_dirty = true;
+  int sellA_ = -1, sellP_ = -1;  // layout the Krylov operator arrays hold (1 sliced-ELL)
+  // the Krylov operators in their current layout (valid after prepare_operators)
+  const DevCsr &momA() const { return sellA_ == 1 ? As_ : A_; }
+  const double *momV() const { return sellA_ == 1 ? d_lhsS.p : d_lhs.p; }
+  const DevCsr &preA() const { return sellP_ == 1 ? Ps_ : P_; }
+  const double *preV() const { return sellP_ == 1 ? d_LbcS.p : d_Lbc.p; }
   double lhs_dt = 0.0;
   // fields
   DBuf<double> d_u, d_us, d_p, d_pc, d_ginlet, d_conv, d_cT, d_wT, d_div, d_rhs,

@@ -66,6 +66,8 @@ typedef struct cfsa_solver_options {
   int assembly;          /* 0 row gather in ascending tet order (sums every entry in the
                             reference's order: bit-exact matrices), 1 graph-coloured
                             element scatter (same values up to rounding order) */
+  int spmv_layout;       /* Krylov SpMV: 0 auto (momentum CSR-stream, pressure sliced-ELL),
+                            1 CSR-stream, 2 sliced-ELL; every layout gives the same bits */
 } cfsa_solver_options;
 
 typedef struct cfsa_step_stats {

@@ -118,15 +118,20 @@ def test_coloured_assembly_close(P, O, case, request):
 
 
 @pytest.mark.parametrize("case", ["kuhn4", "cyl"])
-def test_spmv_bit_exact(P, O, case, request):
+@pytest.mark.parametrize("layout", [1, 2])
+def test_spmv_bit_exact(P, O, case, layout, request):
+    """Both Krylov SpMV layouts (CSR-stream, sliced-ELL) and the CSR-stream
+    products of M, K, L, G reproduce the sequential CSR sums bit for bit."""
     g, o, pm, om = request.getfixturevalue(case)
+    g.setOptions(spmv_layout=layout)
     rng = np.random.default_rng(0)
     sz = g.sizes()
-    for name, nrhs in [("lhs", 3), ("M", 3), ("K", 1), ("L", 1), ("Lbc", 1)]:
+    for name, nrhs in [("lhs", 3), ("lhs", 1), ("M", 3), ("K", 1), ("L", 1), ("Lbc", 1)]:
         n = sz["nu"] if name in ("M", "K", "lhs") else sz["nv"]
         x = rng.uniform(-1, 1, (n, 3) if nrhs == 3 else n)
         rp, col, val = g.getMatrix(name)
         assert np.array_equal(g.applyMatrix(name, x, nrhs), O.csr_spmv(rp, col, val, x)), name
+    g.setOptions(spmv_layout=0)
     p = rng.uniform(-1, 1, sz["nv"])
     y = g.applyMatrix("Gx", p)
     for c, name in enumerate(["Gx", "Gy", "Gz"]):
@@ -145,8 +150,10 @@ def test_advection_divergence_bit_exact(P, O, case, request):
 
 
 @pytest.mark.parametrize("case", ["kuhn4", "cyl"])
-def test_krylov_parity(P, O, case, request):
+@pytest.mark.parametrize("layout", [1, 2])
+def test_krylov_parity(P, O, case, layout, request):
     g, o, pm, om = request.getfixturevalue(case)
+    g.setOptions(spmv_layout=layout)
     sz = g.sizes()
     rng = np.random.default_rng(2)
     b = rng.uniform(-1, 1, (sz["nu"], 3))
@@ -163,6 +170,7 @@ def test_krylov_parity(P, O, case, request):
         assert relerr(xp, xo) <= 1e-12, (method, relerr(xp, xo))
     x0, i0 = g.krylovSolve("Lbc", np.zeros(sz["nv"]))
     assert i0["iters"][0] == 0 and not x0.any()
+    g.setOptions(spmv_layout=0)
 
 
 # ------------------------------------------------------------------ steps
