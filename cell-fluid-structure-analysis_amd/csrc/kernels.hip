@@ -340,16 +340,20 @@ template <class T, int KO, class Base>
 struct OpResidual : OpAx<T, KO, Base> {
   const T *__restrict__ b;
   T *__restrict__ r0;
+  const T *__restrict__ sc;  // per-row scale of the norm (nullptr: |r|^2)
   struct Pre {
-    T b;
+    T b, sc;
   };
-  __device__ Pre pre(int row, int c) const { return Pre{b[(size_t)row * KO + c]}; }
+  __device__ Pre pre(int row, int c) const {
+    return Pre{b[(size_t)row * KO + c], sc ? sc[row] : T(1)};
+  }
   __device__ void epi(int row, int c, T acc, const Pre &pr, DD *d) const {
     const size_t i = (size_t)row * KO + c;
     const T r = pr.b - acc;
     this->y[i] = r;
     r0[i] = r;
-    dd_fma(d[0], (double)r, (double)r);
+    const double rs = sc ? (double)pr.sc * (double)r : (double)r;
+    dd_fma(d[0], rs, rs);
   }
 };
 
@@ -453,7 +457,7 @@ void spmv3_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
 
 template <class T>
 void spmv3_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
-                    const T *b, T *r, T *r0, const int *active, double *partials,
+                    const T *b, T *r, T *r0, const T *sc, const int *active, double *partials,
                     int *grid) {
   OpResidual<T, 3, MaskBase> op{};
   op.mask = active;
@@ -463,6 +467,7 @@ void spmv3_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
   op.y = r;
   op.b = b;
   op.r0 = r0;
+  op.sc = sc;
   const int G = launch_spmv<T, 3, 1>(s, A, op, partials);
   *grid = G;
 }
@@ -508,7 +513,7 @@ void spmv1_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
 
 template <class T>
 void spmv1_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
-                    const T *b, T *r, T *r0, const int *active, double *partials,
+                    const T *b, T *r, T *r0, const T *sc, const int *active, double *partials,
                     int *grid) {
   OpResidual<T, 1, Mask1Base> op{};
   op.mask = active;
@@ -518,6 +523,7 @@ void spmv1_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
   op.y = r;
   op.b = b;
   op.r0 = r0;
+  op.sc = sc;
   const int G = launch_spmv<T, 1, 1>(s, A, op, partials);
   *grid = G;
 }
@@ -656,14 +662,16 @@ void momentum_rhs(hipStream_t s, const DevCsr &A, const double *M, const double 
   template void spmv3_dot2<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,   \
                               const T *, const int *, double *, int *);                 \
   template void spmv3_residual<T>(hipStream_t, const DevCsr &, const T *, const T *,    \
-                                  const T *, T *, T *, const int *, double *, int *);   \
+                                  const T *, T *, T *, const T *, const int *, double *, \
+                                  int *);                                               \
   template void spmv1<T>(hipStream_t, const DevCsr &, const T *, const T *, T *);       \
   template void spmv1_dot<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,    \
                              const T *, const int *, double *, int *);                  \
   template void spmv1_dot2<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,   \
                               const T *, const int *, double *, int *);                 \
   template void spmv1_residual<T>(hipStream_t, const DevCsr &, const T *, const T *,    \
-                                  const T *, T *, T *, const int *, double *, int *);
+                                  const T *, T *, T *, const T *, const int *, double *, \
+                                  int *);
 CFSA_INST_SPMV(double)
 CFSA_INST_SPMV(float)
 
@@ -1010,98 +1018,120 @@ CFSA_INST_BICG_S(1)
 CFSA_INST_BICG_S(3)
 
 // ===========================================================================
-// Jacobi PCG (stopping rule of Eigen's conjugate_gradient)
+// Jacobi PCG (stopping rule of Eigen's conjugate_gradient), K interleaved
+// right-hand sides solved independently with per-component scalars.  With
+// K = 1 this is exactly the reference's pressure CG path.
 // ===========================================================================
-template <class T>
+template <class T, int K>
 __global__ __launch_bounds__(kBlock) void k_cg_init(int n, const T *__restrict__ b,
                                                     const T *__restrict__ invd,
                                                     T *__restrict__ x, T *__restrict__ r,
                                                     T *__restrict__ p, int scaled,
                                                     double *__restrict__ partials) {
-  DD acc[2];
-  dd_zero(acc[0]);
-  dd_zero(acc[1]);
-  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n;
-       i += (size_t)gridDim.x * kBlock) {
-    const T bi = b[i];
-    x[i] = T(0);
-    r[i] = bi;
-    const T pi = (T)((double)invd[i] * (double)bi);
-    p[i] = pi;
-    const double sn = scaled ? (double)pi : (double)bi;
-    dd_fma(acc[0], sn, sn);
-    dd_fma(acc[1], (double)bi, (double)pi);
+  // partials: [2c] = stopping norm of b, [2c+1] = b . M^-1 b
+  DD acc[2 * K];
+#pragma unroll
+  for (int i = 0; i < 2 * K; ++i) dd_zero(acc[i]);
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
+       e += (size_t)gridDim.x * kBlock) {
+    const double de = (double)invd[e];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const size_t i = e * K + c;
+      const T bi = b[i];
+      x[i] = T(0);
+      r[i] = bi;
+      const T pi = (T)(de * (double)bi);
+      p[i] = pi;
+      const double sn = scaled ? (double)pi : (double)bi;
+      dd_fma(acc[2 * c], sn, sn);
+      dd_fma(acc[2 * c + 1], (double)bi, (double)pi);
+    }
   }
-  block_partials<2>(acc, partials);
+  block_partials<2 * K>(acc, partials);
 }
 
+template <int K>
 __global__ void k_cg_s_init(KrylovState *st, const double *partials, int G) {
-  __shared__ double tot[2];
-  reduce_final<2>(partials, G, tot);
+  __shared__ double tot[2 * K];
+  reduce_final<2 * K>(partials, G, tot);
   if (threadIdx.x != 0) return;
-  const double bb = tot[0];
-  st->rhs_sq[0] = bb;
-  st->iters[0] = 0;
-  st->restarts[0] = 0;
-  st->active[0] = 0;
-  st->any_active = 0;
-  for (int c = 1; c < 3; ++c) {
+  int any = 0;
+  for (int c = 0; c < K; ++c) {
+    const double bb = tot[2 * c];
+    st->rhs_sq[c] = bb;
+    st->iters[c] = 0;
+    st->restarts[c] = 0;
+    st->active[c] = 0;
+    if (bb == 0.0) {
+      st->status[c] = KS_DONE;
+      st->err[c] = 0.0;
+      continue;
+    }
+    const double thr = fmax(st->tol * st->tol * bb, DBL_MIN);
+    st->tol2[c] = thr;
+    st->rr[c] = bb;
+    if (bb < thr) {
+      st->status[c] = KS_DONE;
+      st->err[c] = sqrt(bb / bb);
+      continue;
+    }
+    st->rho[c] = tot[2 * c + 1];
+    st->status[c] = (st->maxit > 0) ? KS_RUNNING : KS_DONE;
+    st->err[c] = 1.0;
+    st->active[c] = st->maxit > 0;
+    any |= st->active[c];
+  }
+  for (int c = K; c < 3; ++c) {
     st->status[c] = KS_DONE;
     st->active[c] = 0;
   }
-  if (bb == 0.0) {
-    st->status[0] = KS_DONE;
-    st->err[0] = 0.0;
-    return;
-  }
-  const double thr = fmax(st->tol * st->tol * bb, DBL_MIN);
-  st->tol2[0] = thr;
-  st->rr[0] = bb;
-  if (bb < thr) {
-    st->status[0] = KS_DONE;
-    st->err[0] = sqrt(bb / bb);
-    return;
-  }
-  st->rho[0] = tot[1];
-  st->status[0] = (st->maxit > 0) ? KS_RUNNING : KS_DONE;
-  st->err[0] = 1.0;
-  st->active[0] = st->maxit > 0;
-  st->any_active = st->active[0];
+  st->any_active = any;
 }
 
+template <int K>
 __global__ void k_cg_s_alpha(KrylovState *st, const double *partials, int G) {
-  __shared__ double tot[1];
-  reduce_final<1>(partials, G, tot);
-  if (threadIdx.x != 0 || !st->active[0]) return;
-  st->r0v[0] = tot[0];
-  st->alpha[0] = st->rho[0] / tot[0];
+  __shared__ double tot[K];
+  reduce_final<K>(partials, G, tot);
+  if (threadIdx.x != 0) return;
+  for (int c = 0; c < K; ++c) {
+    if (!st->active[c]) continue;
+    st->r0v[c] = tot[c];
+    st->alpha[c] = st->rho[c] / tot[c];
+  }
 }
 
+template <int K>
 __global__ void k_cg_s_beta(KrylovState *st, const double *partials, int G) {
-  __shared__ double tot[2];
-  reduce_final<2>(partials, G, tot);
-  if (threadIdx.x != 0 || !st->active[0]) return;
-  const double rn2 = tot[0];
-  st->rr[0] = rn2;
-  if (rn2 < st->tol2[0]) {
-    st->status[0] = KS_DONE;
-    st->err[0] = sqrt(rn2 / st->rhs_sq[0]);
-    st->active[0] = 0;
-    st->any_active = 0;
-    return;
+  __shared__ double tot[2 * K];
+  reduce_final<2 * K>(partials, G, tot);
+  if (threadIdx.x != 0) return;
+  for (int c = 0; c < K; ++c) {
+    if (!st->active[c]) continue;
+    const double rn2 = tot[2 * c];
+    st->rr[c] = rn2;
+    if (rn2 < st->tol2[c]) {
+      st->status[c] = KS_DONE;
+      st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
+      st->active[c] = 0;
+      continue;
+    }
+    st->rho_old[c] = st->rho[c];
+    st->rho[c] = tot[2 * c + 1];
+    st->beta[c] = st->rho[c] / st->rho_old[c];
+    st->iters[c] += 1;
+    if (st->iters[c] >= st->maxit) {
+      // the direction update below still runs; the loop then ends
+      st->status[c] = KS_DONE;
+      st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
+    }
   }
-  st->rho_old[0] = st->rho[0];
-  st->rho[0] = tot[1];
-  st->beta[0] = st->rho[0] / st->rho_old[0];
-  st->iters[0] += 1;
-  if (st->iters[0] >= st->maxit) {
-    // the direction update below still runs; the loop then ends
-    st->status[0] = KS_DONE;
-    st->err[0] = sqrt(rn2 / st->rhs_sq[0]);
-  }
+  int any = 0;
+  for (int c = 0; c < K; ++c) any |= st->active[c];
+  st->any_active = any;
 }
 
-template <class T>
+template <class T, int K>
 __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__restrict__ st,
                                                   const T *__restrict__ p,
                                                   const T *__restrict__ q,
@@ -1109,90 +1139,128 @@ __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__re
                                                   T *__restrict__ x, T *__restrict__ r,
                                                   T *__restrict__ z, int scaled,
                                                   double *__restrict__ partials) {
-  if (!st->active[0]) return;
-  const double al = st->alpha[0];
-  DD acc[2];
-  dd_zero(acc[0]);
-  dd_zero(acc[1]);
-  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n;
-       i += (size_t)gridDim.x * kBlock) {
-    x[i] = (T)((double)x[i] + al * (double)p[i]);
-    const T ri = (T)((double)r[i] - al * (double)q[i]);
-    r[i] = ri;
-    const T zi = (T)((double)invd[i] * (double)ri);
-    z[i] = zi;
-    const double sn = scaled ? (double)zi : (double)ri;
-    dd_fma(acc[0], sn, sn);
-    dd_fma(acc[1], (double)ri, (double)zi);
+  if (!st->any_active) return;
+  int act[K];
+  double al[K];
+  DD acc[2 * K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    act[c] = st->active[c];
+    al[c] = st->alpha[c];
+    dd_zero(acc[2 * c]);
+    dd_zero(acc[2 * c + 1]);
   }
-  block_partials<2>(acc, partials);
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
+       e += (size_t)gridDim.x * kBlock) {
+    const double de = (double)invd[e];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      if (!act[c]) continue;
+      const size_t i = e * K + c;
+      x[i] = (T)((double)x[i] + al[c] * (double)p[i]);
+      const T ri = (T)((double)r[i] - al[c] * (double)q[i]);
+      r[i] = ri;
+      const T zi = (T)(de * (double)ri);
+      z[i] = zi;
+      const double sn = scaled ? (double)zi : (double)ri;
+      dd_fma(acc[2 * c], sn, sn);
+      dd_fma(acc[2 * c + 1], (double)ri, (double)zi);
+    }
+  }
+  block_partials<2 * K>(acc, partials);
 }
 
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_cg_v2(int n, KrylovState *__restrict__ st,
+template <class T, int K>
+__global__ __launch_bounds__(kBlock) void k_cg_v2(int n, const KrylovState *__restrict__ st,
                                                   const T *__restrict__ z,
                                                   T *__restrict__ p) {
-  if (!st->active[0]) return;
-  const double be = st->beta[0];
-  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n;
-       i += (size_t)gridDim.x * kBlock)
-    p[i] = (T)((double)z[i] + be * (double)p[i]);
-}
-
-// Ends the iteration when the limit was reached in k_cg_s_beta.
-__global__ void k_cg_s_end(KrylovState *st) {
-  if (st->active[0] && st->status[0] == KS_DONE) {
-    st->active[0] = 0;
-    st->any_active = 0;
+  if (!st->any_active) return;
+  int act[K];
+  double be[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    act[c] = st->active[c];
+    be[c] = st->beta[c];
+  }
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
+       e += (size_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      if (!act[c]) continue;
+      const size_t i = e * K + c;
+      p[i] = (T)((double)z[i] + be[c] * (double)p[i]);
+    }
   }
 }
 
-template <class T>
+// Ends the iteration of the right-hand sides that reached the limit in
+// k_cg_s_beta.
+template <int K>
+__global__ void k_cg_s_end(KrylovState *st) {
+  int any = 0;
+  for (int c = 0; c < K; ++c) {
+    if (st->active[c] && st->status[c] == KS_DONE) st->active[c] = 0;
+    any |= st->active[c];
+  }
+  st->any_active = any;
+}
+
+template <class T, int K>
 void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p, int scaled,
              double *partials, int *grid) {
   const int G = vec_grid(n);
   *grid = G;
-  hipLaunchKernelGGL((k_cg_init<T>), dim3(G), dim3(kBlock), 0, s, n, b, invd, x, r, p, scaled,
-                     partials);
+  hipLaunchKernelGGL((k_cg_init<T, K>), dim3(G), dim3(kBlock), 0, s, n, b, invd, x, r, p,
+                     scaled, partials);
   CFSA_LAUNCH_CHECK();
 }
+template <int K>
 void cg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G) {
-  hipLaunchKernelGGL(k_cg_s_init, dim3(1), dim3(kBlock), 0, s, st, partials, G);
+  hipLaunchKernelGGL((k_cg_s_init<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G);
   CFSA_LAUNCH_CHECK();
 }
+template <int K>
 void cg_s_alpha(hipStream_t s, KrylovState *st, const double *partials, int G) {
-  hipLaunchKernelGGL(k_cg_s_alpha, dim3(1), dim3(kBlock), 0, s, st, partials, G);
+  hipLaunchKernelGGL((k_cg_s_alpha<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G);
   CFSA_LAUNCH_CHECK();
 }
+template <int K>
 void cg_s_beta(hipStream_t s, KrylovState *st, const double *partials, int G) {
-  hipLaunchKernelGGL(k_cg_s_beta, dim3(1), dim3(kBlock), 0, s, st, partials, G);
+  hipLaunchKernelGGL((k_cg_s_beta<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G);
   CFSA_LAUNCH_CHECK();
 }
-template <class T>
+template <class T, int K>
 void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
            const T *invd, T *x, T *r, T *z, int scaled, double *partials, int *grid) {
   const int G = vec_grid(n);
   *grid = G;
-  hipLaunchKernelGGL((k_cg_v1<T>), dim3(G), dim3(kBlock), 0, s, n, st, p, q, invd, x, r, z,
+  hipLaunchKernelGGL((k_cg_v1<T, K>), dim3(G), dim3(kBlock), 0, s, n, st, p, q, invd, x, r, z,
                      scaled, partials);
   CFSA_LAUNCH_CHECK();
 }
-template <class T>
+template <class T, int K>
 void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *z, T *p) {
-  hipLaunchKernelGGL((k_cg_v2<T>), dim3(vec_grid(n)), dim3(kBlock), 0, s, n,
-                     const_cast<KrylovState *>(st), z, p);
+  hipLaunchKernelGGL((k_cg_v2<T, K>), dim3(vec_grid(n)), dim3(kBlock), 0, s, n, st, z, p);
   CFSA_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_cg_s_end, dim3(1), dim3(1), 0, s, const_cast<KrylovState *>(st));
+  hipLaunchKernelGGL((k_cg_s_end<K>), dim3(1), dim3(1), 0, s, const_cast<KrylovState *>(st));
   CFSA_LAUNCH_CHECK();
 }
-#define CFSA_INST_CG(T)                                                                   \
-  template void cg_init<T>(hipStream_t, int, const T *, const T *, T *, T *, T *, int,      \
-                           double *, int *);                                              \
-  template void cg_v1<T>(hipStream_t, int, const KrylovState *, const T *, const T *,     \
-                         const T *, T *, T *, T *, int, double *, int *);                 \
-  template void cg_v2<T>(hipStream_t, int, const KrylovState *, const T *, T *);
-CFSA_INST_CG(double)
-CFSA_INST_CG(float)
+#define CFSA_INST_CG(T, K)                                                                \
+  template void cg_init<T, K>(hipStream_t, int, const T *, const T *, T *, T *, T *, int,   \
+                              double *, int *);                                           \
+  template void cg_v1<T, K>(hipStream_t, int, const KrylovState *, const T *, const T *,  \
+                            const T *, T *, T *, T *, int, double *, int *);              \
+  template void cg_v2<T, K>(hipStream_t, int, const KrylovState *, const T *, T *);
+CFSA_INST_CG(double, 1)
+CFSA_INST_CG(double, 3)
+CFSA_INST_CG(float, 1)
+CFSA_INST_CG(float, 3)
+#define CFSA_INST_CG_S(K)                                                        \
+  template void cg_s_init<K>(hipStream_t, KrylovState *, const double *, int);   \
+  template void cg_s_alpha<K>(hipStream_t, KrylovState *, const double *, int);  \
+  template void cg_s_beta<K>(hipStream_t, KrylovState *, const double *, int);
+CFSA_INST_CG_S(1)
+CFSA_INST_CG_S(3)
 
 // ===========================================================================
 // Element assembly into the fixed patterns (graph-coloured, atomic-free)

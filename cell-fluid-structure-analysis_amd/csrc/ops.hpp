@@ -51,7 +51,7 @@ void spmv3_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
 // r = b - A x (rhs of the active mask only), r0 = r, partials r.r [3][G]
 template <class T>
 void spmv3_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
-                    const T *b, T *r, T *r0, const int *active, double *partials,
+                    const T *b, T *r, T *r0, const T *sc, const int *active, double *partials,
                     int *grid);
 // single right-hand side variants
 template <class T>
@@ -64,7 +64,7 @@ void spmv1_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
                 const T *sv, const int *active, double *partials, int *grid);
 template <class T>
 void spmv1_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
-                    const T *b, T *r, T *r0, const int *active, double *partials,
+                    const T *b, T *r, T *r0, const T *sc, const int *active, double *partials,
                     int *grid);
 // y[i][c] = sum_j G_c[i,j] x[j]   (G values interleaved [nnz][3])
 void spmv_grad3(hipStream_t s, const DevCsr &P, const double *gval, const double *x,
@@ -114,16 +114,19 @@ void bicg_v3(hipStream_t s, int n, const KrylovState *st, const T *y, const T *z
              const T *sv, const T *t, const T *r0, T *x, T *r, const T *stop_scale,
              double *partials, int *grid);
 // Jacobi PCG (single right-hand side)
-template <class T>
+template <class T, int K>
 void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p, int scaled,
              double *partials, int *grid);
+template <int K>
 void cg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G);
+template <int K>
 void cg_s_alpha(hipStream_t s, KrylovState *st, const double *partials, int G);
+template <int K>
 void cg_s_beta(hipStream_t s, KrylovState *st, const double *partials, int G);
-template <class T>
+template <class T, int K>
 void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
            const T *invd, T *x, T *r, T *z, int scaled, double *partials, int *grid);
-template <class T>
+template <class T, int K>
 void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *z, T *p);
 
 // ---- FEM element / node kernels ----

@@ -472,9 +472,9 @@ SolveResult NSSolver::bicgstab(const DevCsr &A, const T *val, const T *invd, con
       // r = b - A x; r0 = r; rho = r0_sqnorm = |r|^2 (Eigen restart branch)
       set_mask(rs[0], rs[1], rs[2]);
       if (K == 3)
-        spmv3_residual<T>(s, A, val, x, b, w.r.p, w.r0.p, d_mask.p, d_part_r.p, &Gr);
+        spmv3_residual<T>(s, A, val, x, b, w.r.p, w.r0.p, nullptr, d_mask.p, d_part_r.p, &Gr);
       else
-        spmv1_residual<T>(s, A, val, x, b, w.r.p, w.r0.p, d_mask.p, d_part_r.p, &Gr);
+        spmv1_residual<T>(s, A, val, x, b, w.r.p, w.r0.p, nullptr, d_mask.p, d_part_r.p, &Gr);
       bicg_s_restart<K>(s, st, d_part_r.p, Gr, d_mask.p);
     }
   }
@@ -488,35 +488,47 @@ SolveResult NSSolver::bicgstab(const DevCsr &A, const T *val, const T *invd, con
   return res;
 }
 
-template <class T>
+template <class T, int K>
 SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
                           double tol, int maxit, KrylovWork<T> &w) {
   hipStream_t s = stream_;
   const int n = A.n;
   KrylovState *st = d_state.p;
   const int *act = st->active;
-  set_state_params(tol, maxit, 1);
+  const bool time_spmv = (K == 3) && opt.timing >= 2;
+  set_state_params(tol, maxit, K);
   int Gv = 0, Gb = 0;
-  cg_init<T>(s, n, b, invd, x, w.r.p, w.p.p, opt.stop_scaled, d_part_a.p, &Gv);
-  cg_s_init(s, st, d_part_a.p, Gv);
+  cg_init<T, K>(s, n, b, invd, x, w.r.p, w.p.p, opt.stop_scaled, d_part_a.p, &Gv);
+  cg_s_init<K>(s, st, d_part_a.p, Gv);
   const int chunk = std::max(1, opt.check_every);
   const long max_chunks = (long)maxit / chunk + 16;
   for (long ch = 0;; ++ch) {
     if (ch > max_chunks) throw std::runtime_error("PCG driver did not terminate");
     read_state();
-    if (h_state->status[0] == KS_DONE) break;
+    bool done = true;
+    for (int c = 0; c < K; ++c) done = done && h_state->status[c] == KS_DONE;
+    if (done) break;
     for (int j = 0; j < chunk; ++j) {
-      spmv1_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
-      cg_s_alpha(s, st, d_part_b.p, Gb);
-      cg_v1<T>(s, n, st, w.p.p, w.v.p, invd, x, w.r.p, w.z.p, opt.stop_scaled, d_part_a.p, &Gv);
-      cg_s_beta(s, st, d_part_a.p, Gv);
-      cg_v2<T>(s, n, st, w.z.p, w.p.p);
+      if (time_spmv) spmv_event_begin();
+      if (K == 3)
+        spmv3_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
+      else
+        spmv1_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
+      if (time_spmv) spmv_event_end();
+      cg_s_alpha<K>(s, st, d_part_b.p, Gb);
+      cg_v1<T, K>(s, n, st, w.p.p, w.v.p, invd, x, w.r.p, w.z.p, opt.stop_scaled, d_part_a.p,
+                  &Gv);
+      cg_s_beta<K>(s, st, d_part_a.p, Gv);
+      cg_v2<T, K>(s, n, st, w.z.p, w.p.p);
     }
   }
   SolveResult res;
-  res.iters[0] = h_state->iters[0];
-  res.err[0] = h_state->err[0];
-  res.ok = h_state->err[0] <= tol;
+  res.ok = true;
+  for (int c = 0; c < K; ++c) {
+    res.iters[c] = h_state->iters[c];
+    res.err[c] = h_state->err[c];
+    if (!(h_state->err[c] <= tol)) res.ok = false;
+  }
   return res;
 }
 
@@ -524,9 +536,9 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
 // b - A x, its norm and the stopping test are FP64; matrix values, vectors and
 // SpMV accumulation of the inner solve are FP32 (inner scalars stay FP64).
 template <int K>
-SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const float *val32,
-                             const float *invd32, const double *b, double *x, double tol,
-                             int maxit, int method) {
+SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double *invd64,
+                             const float *val32, const float *invd32, const double *b,
+                             double *x, double tol, int maxit, int method) {
   hipStream_t s = stream_;
   const int n = A.n;
   KrylovWork<double> &w64 = (K == 3) ? w3_ : w1_;
@@ -534,11 +546,15 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const float *
   CFSA_HIP(hipMemsetAsync(x, 0, (size_t)n * K * sizeof(double), s));
   int Gr = 0;
   set_mask(1, K == 3, K == 3);
+  // the outer test uses the same stopping rule as the FP64 solvers
+  const double *sc = opt.stop_scaled ? invd64 : nullptr;
   auto residual = [&]() {
     if (K == 3)
-      spmv3_residual<double>(s, A, val64, x, b, d_r64.p, w64.r0.p, d_mask.p, d_part_r.p, &Gr);
+      spmv3_residual<double>(s, A, val64, x, b, d_r64.p, w64.r0.p, sc, d_mask.p, d_part_r.p,
+                             &Gr);
     else
-      spmv1_residual<double>(s, A, val64, x, b, d_r64.p, w64.r0.p, d_mask.p, d_part_r.p, &Gr);
+      spmv1_residual<double>(s, A, val64, x, b, d_r64.p, w64.r0.p, sc, d_mask.p, d_part_r.p,
+                             &Gr);
     hipLaunchKernelGGL((k_reduce_pairs<K>), dim3(1), dim3(kBlock), 0, s, d_part_r.p, Gr,
                        d_scalar.p);
     CFSA_LAUNCH_CHECK();
@@ -566,8 +582,8 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const float *
                        d_mask.p, d_b32.p);
     CFSA_LAUNCH_CHECK();
     SolveResult in;
-    if (K == 1 && method == 1)
-      in = pcg<float>(A, val32, invd32, d_b32.p, d_x32.p, opt.inner_tol, maxit, w32);
+    if (method == 1)
+      in = pcg<float, K>(A, val32, invd32, d_b32.p, d_x32.p, opt.inner_tol, maxit, w32);
     else
       in = bicgstab<float, K>(A, val32, invd32, d_b32.p, d_x32.p, opt.inner_tol, maxit, w32);
     for (int c = 0; c < K; ++c) res.iters[c] += in.iters[c];
@@ -587,18 +603,21 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const float *
 // precision 1: FP32 inner solves for both systems; 2: for the momentum solve only
 SolveResult NSSolver::solve_momentum(const double *b, double *x) {
   if (opt.precision == 1 || opt.precision == 2)
-    return refine<3>(momA(), momV(), d_lhs32.p, d_invdA32.p, b, x, opt.momentum_tol,
-                     opt.max_iter, 0);
+    return refine<3>(momA(), momV(), d_invdA.p, d_lhs32.p, d_invdA32.p, b, x,
+                     opt.momentum_tol, opt.max_iter, opt.momentum_solver);
+  if (opt.momentum_solver == 1)
+    return pcg<double, 3>(momA(), momV(), d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
+                          w3_);
   return bicgstab<double, 3>(momA(), momV(), d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
                              w3_);
 }
 
 SolveResult NSSolver::solve_pressure(const double *b, double *x) {
   if (opt.precision == 1)
-    return refine<1>(preA(), preV(), d_Lbc32.p, d_invdL32.p, b, x, opt.pressure_tol,
-                     opt.max_iter, opt.pressure_solver);
+    return refine<1>(preA(), preV(), d_invdL.p, d_Lbc32.p, d_invdL32.p, b, x,
+                     opt.pressure_tol, opt.max_iter, opt.pressure_solver);
   if (opt.pressure_solver == 1)
-    return pcg<double>(preA(), preV(), d_invdL.p, b, x, opt.pressure_tol, opt.max_iter, w1_);
+    return pcg<double, 1>(preA(), preV(), d_invdL.p, b, x, opt.pressure_tol, opt.max_iter, w1_);
   return bicgstab<double, 1>(preA(), preV(), d_invdL.p, b, x, opt.pressure_tol, opt.max_iter,
                              w1_);
 }
@@ -874,7 +893,6 @@ SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double
   if (which != 2 && which != 4) throw std::invalid_argument("krylov_solve: matrix 2 or 4");
   if (nrhs != 1 && nrhs != 3) throw std::invalid_argument("krylov_solve: nrhs 1 or 3");
   if (which == 4 && nrhs != 1) throw std::invalid_argument("krylov_solve: L_bc takes 1 rhs");
-  if (method == 1 && nrhs != 1) throw std::invalid_argument("krylov_solve: PCG takes 1 rhs");
   const int saved = opt.precision;
   opt.precision = precision;
   prepare_operators();
@@ -887,14 +905,18 @@ SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double
   SolveResult r;
   if (precision == 1) {
     if (nrhs == 3)
-      r = refine<3>(A, momV(), d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit, 0);
+      r = refine<3>(A, momV(), d_invdA.p, d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit,
+                    method);
     else if (which == 2)
       throw std::invalid_argument("krylov_solve: mixed momentum needs 3 rhs");
     else
-      r = refine<1>(A, preV(), d_Lbc32.p, d_invdL32.p, db.p, dx.p, tol, maxit, method);
+      r = refine<1>(A, preV(), d_invdL.p, d_Lbc32.p, d_invdL32.p, db.p, dx.p, tol, maxit,
+                    method);
+  } else if (method == 1 && nrhs == 3) {
+    r = pcg<double, 3>(A, momV(), d_invdA.p, db.p, dx.p, tol, maxit, w3_);
   } else if (method == 1) {
-    r = pcg<double>(A, which == 2 ? momV() : preV(), which == 2 ? d_invdA.p : d_invdL.p,
-                    db.p, dx.p, tol, maxit, which == 2 ? w3_ : w1_);
+    r = pcg<double, 1>(A, which == 2 ? momV() : preV(), which == 2 ? d_invdA.p : d_invdL.p,
+                       db.p, dx.p, tol, maxit, which == 2 ? w3_ : w1_);
   } else if (nrhs == 3) {
     r = bicgstab<double, 3>(A, momV(), d_invdA.p, db.p, dx.p, tol, maxit, w3_);
   } else {

@@ -63,6 +63,7 @@ struct SolverOptions {
   double pressure_tol = 1e-8;  // NavierStokes.cpp:635
   int max_iter = 1000;         // NavierStokes.cpp:499,634
   int pressure_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG
+  int momentum_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG (lhs is SPD)
   int precision = 0;           // 0 FP64, 1 FP32 inner + FP64 refinement, 2 momentum only
   double inner_tol = 1e-4;     // relative tolerance of each FP32 inner solve
   int max_refine = 40;         // outer refinement sweeps (mixed)
@@ -267,13 +268,13 @@ class NSSolver {
   template <class T, int K>
   SolveResult bicgstab(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
                        double tol, int maxit, KrylovWork<T> &w);
-  template <class T>
+  template <class T, int K>
   SolveResult pcg(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
                   double tol, int maxit, KrylovWork<T> &w);
   template <int K>
-  SolveResult refine(const DevCsr &A, const double *val64, const float *val32,
-                     const float *invd32, const double *b, double *x, double tol,
-                     int maxit, int method);
+  SolveResult refine(const DevCsr &A, const double *val64, const double *invd64,
+                     const float *val32, const float *invd32, const double *b, double *x,
+                     double tol, int maxit, int method);
   SolveResult solve_momentum(const double *b, double *x);
   SolveResult solve_pressure(const double *b, double *x);
 };

@@ -68,6 +68,8 @@ typedef struct cfsa_solver_options {
                             element scatter (same values up to rounding order) */
   int spmv_layout;       /* Krylov SpMV: 0 auto (momentum CSR-stream, pressure sliced-ELL),
                             1 CSR-stream, 2 sliced-ELL; every layout gives the same bits */
+  int momentum_solver;   /* 0 BiCGSTAB (reference), 1 Jacobi PCG (the momentum matrix
+                            M/dt + K + penalty diagonal is symmetric positive definite) */
 } cfsa_solver_options;
 
 typedef struct cfsa_step_stats {

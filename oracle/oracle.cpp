@@ -521,6 +521,7 @@ struct orc_ns {
   int outlet_type = 0;
   double momentum_tol = 1e-6, pressure_tol = 1e-8;
   int max_iter = 1000, pressure_solver = 0;
+  int momentum_solver = 0;  // extension: 1 Jacobi CG on the (SPD) momentum matrix
   bool stop_scaled = false;  // extension: Jacobi-scaled residual stopping rule
   Csr M, K, Gx, Gy, Gz, L;
   int it_mom[3] = {0, 0, 0};
@@ -729,7 +730,8 @@ struct orc_ns {
         }
       }
       std::vector<double> sol;
-      KResult r = bicgstab(lhs, rhs, sol, momentum_tol, max_iter, stop_scaled);
+      KResult r = momentum_solver == 1 ? pcg(lhs, rhs, sol, momentum_tol, max_iter, stop_scaled)
+                                       : bicgstab(lhs, rhs, sol, momentum_tol, max_iter, stop_scaled);
       it_mom[comp] = r.iters;
       err_mom[comp] = r.err;
       if (!r.ok) return 2;
@@ -1281,6 +1283,7 @@ void orc_ns_set_solver(orc_ns *h, double mt, double pt, int maxit, int ps) {
   h->pressure_solver = ps;
 }
 void orc_ns_set_stop_scaled(orc_ns *h, int on) { h->stop_scaled = on != 0; }
+void orc_ns_set_momentum_solver(orc_ns *h, int method) { h->momentum_solver = method; }
 // which: 0 pressure, 1 shear, 2 total (nv x 3)
 void orc_ns_forces(orc_ns *h, int which, double *out) {
   std::vector<V3> p = h->computePressureForces(), s = h->computeShearStress();

@@ -71,6 +71,8 @@ void orc_ns_set_solver(orc_ns *h, double momentum_tol, double pressure_tol, int 
 void orc_ns_forces(orc_ns *h, int which, double *out);
 /* extension: stop Krylov solves on |D^-1 r| <= tol |D^-1 b| (Jacobi-scaled) */
 void orc_ns_set_stop_scaled(orc_ns *h, int on);
+/* extension: momentum solver 0 BiCGSTAB (reference), 1 Jacobi CG */
+void orc_ns_set_momentum_solver(orc_ns *h, int method);
 int orc_ns_piso_step(orc_ns *h);   /* 0 ok, 2 momentum, 3 pressure, 4 NaN */
 int orc_ns_simple_step(orc_ns *h);
 void orc_ns_sizes(const orc_ns *h, int64_t out[6]); /* nv nt ne nu nnzA nnzP */

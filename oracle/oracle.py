@@ -70,6 +70,7 @@ _sig("orc_ns_krylov_solve", C.c_int, _p, C.c_int, C.c_int, _f64p, _f64p, C.c_dou
      C.POINTER(C.c_int32), C.POINTER(C.c_double))
 
 _sig("orc_ns_set_stop_scaled", None, _p, C.c_int)
+_sig("orc_ns_set_momentum_solver", None, _p, C.c_int)
 _sig("orc_ns_forces", None, _p, C.c_int, _f64p)
 _sig("orc_adjoint_gradient", C.c_int, _p, C.c_int, C.c_int, _f64p, _f64p,
      C.POINTER(C.c_double), _f64p, C.POINTER(C.c_double))
@@ -254,6 +255,10 @@ class OracleNS:
 
     def setStopScaled(self, on=True):
         _lib.orc_ns_set_stop_scaled(self._h, int(bool(on)))
+
+    def setMomentumSolver(self, method):
+        """Extension: 0 BiCGSTAB (reference), 1 Jacobi CG."""
+        _lib.orc_ns_set_momentum_solver(self._h, int(method))
 
     def adjointGradient(self, nsteps, basis, coef):
         """J, dJ/dcoef, dJ/dviscosity of J = 1/2 sum_outlet |u|^2 after nsteps."""

@@ -15,8 +15,8 @@ REPO = Path(__file__).resolve().parents[1]
 
 
 @pytest.mark.parametrize("kind,n,steps", [("kuhn", 6, 3), ("cyl", 0, 3)])
-@pytest.mark.parametrize("precision", [1, 2])
-def test_mixed_precision_error_vs_fp64(P, O, kind, n, steps, precision):
+@pytest.mark.parametrize("precision,msolver", [(1, 0), (2, 0), (2, 1)])
+def test_mixed_precision_error_vs_fp64(P, O, kind, n, steps, precision, msolver):
     """FP32 inner solves (inner tol 1e-4) inside FP64 refinement, momentum to
     1e-10 and pressure to 1e-8 (Jacobi-scaled rule, as the FP64 path): the
     velocity stays within 1e-7 and the pressure within 1e-5 of the FP64 path.
@@ -25,7 +25,7 @@ def test_mixed_precision_error_vs_fp64(P, O, kind, n, steps, precision):
     g64, o, pm, om = make_case(P, O, kind, n, **tols)
     g32, _, _, _ = make_case(P, O, kind, n, **tols)
     g64.setOptions(stop_rule=1)
-    g32.setOptions(stop_rule=1, precision=precision)
+    g32.setOptions(stop_rule=1, precision=precision, momentum_solver=msolver)
     for s in range(steps):
         assert g64.pisoStep(), P.last_error()
         assert g32.pisoStep(), P.last_error()

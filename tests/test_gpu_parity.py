@@ -26,7 +26,7 @@ def relerr(a, b):
 
 def make_case(P, O, kind, n=4, rho=1060.0, mu=0.004, dt=0.01, uin=(0.5, 0.0, 0.0),
               ncorr=1, outlet=0, p_out=0.0, relax=(1.0, 1.0), pressure_solver=0,
-              momentum_tol=1e-6, pressure_tol=1e-8, inlet_before_init=False):
+              momentum_tol=1e-6, pressure_tol=1e-8, inlet_before_init=False, momentum_solver=0):
     if kind == "kuhn":
         pm = P.Mesh.structuredRectangularPrism(1.0, 1.0, 1.0, n, n, n)
     elif kind == "box":
@@ -49,7 +49,7 @@ def make_case(P, O, kind, n=4, rho=1060.0, mu=0.004, dt=0.01, uin=(0.5, 0.0, 0.0
     g.setRelaxU(relax[0])
     g.setRelaxP(relax[1])
     g.setOptions(momentum_tol=momentum_tol, pressure_tol=pressure_tol,
-                 pressure_solver=pressure_solver)
+                 pressure_solver=pressure_solver, momentum_solver=momentum_solver)
     o = O.OracleNS(om, rho, mu, inlet0) if outlet == 0 else None
     if o is None:
         # the oracle picks the Neumann reference node at initialize, like the reference
@@ -60,6 +60,7 @@ def make_case(P, O, kind, n=4, rho=1060.0, mu=0.004, dt=0.01, uin=(0.5, 0.0, 0.0
     o.setNumberOfCorrections(ncorr)
     o.setRelax(*relax)
     o.setSolver(momentum_tol, pressure_tol, 1000, pressure_solver)
+    o.setMomentumSolver(momentum_solver)
     return g, o, pm, om
 
 
@@ -170,6 +171,12 @@ def test_krylov_parity(P, O, case, layout, request):
         assert relerr(xp, xo) <= 1e-12, (method, relerr(xp, xo))
     x0, i0 = g.krylovSolve("Lbc", np.zeros(sz["nv"]))
     assert i0["iters"][0] == 0 and not x0.any()
+    # batched Jacobi CG on the (SPD) momentum matrix, 3 right-hand sides
+    x, info = g.krylovSolve("lhs", b, method=1, tol=1e-10)
+    for c in range(3):
+        xo, io = o.krylovSolve("lhs", b[:, c], method=1, tol=1e-10)
+        assert info["iters"][c] == io["iters"], (c, info, io)
+        assert relerr(x[:, c], xo) <= 1e-12, (c, relerr(x[:, c], xo))
     g.setOptions(spmv_layout=0)
 
 
@@ -183,6 +190,7 @@ STEP_CASES = {
     "kuhn4_inlet_first": dict(kind="kuhn", n=4, inlet_before_init=True, steps=3),
     "cyl_config1": dict(kind="cyl", steps=20, pressure_solver=1),
     "cyl_bicgstab": dict(kind="cyl", steps=5),
+    "kuhn5_momentum_cg": dict(kind="kuhn", n=5, momentum_solver=1, pressure_solver=1, steps=3),
 }
 
 
