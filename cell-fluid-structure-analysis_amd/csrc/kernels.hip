@@ -36,7 +36,7 @@ template <class T, int KO, int ND, class Op>
 __global__ __launch_bounds__(kBlock) void k_csr_stream(const int *__restrict__ rowptr,
                                                        const int *__restrict__ win, int nwin,
                                                        Op op, double *__restrict__ wpart) {
-  constexpr int B = kStreamB<KO>;
+  constexpr int B = kStreamB<T, KO>;
   constexpr int R = kBlock / KO;   // rows summed per pass
   constexpr int PER = B / kBlock;  // staged entries per thread
   constexpr int NDs = ND > 0 ? ND : 1;
@@ -248,8 +248,8 @@ static int launch_spmv(hipStream_t s, const DevCsr &A, const Op &op, double *par
     CFSA_LAUNCH_CHECK();
     return G;
   }
-  const int *win = KO == 3 ? A.win3 : A.win1;
-  const int nwin = KO == 3 ? A.nwin3 : A.nwin1;
+  const int *win = KO == 1 ? A.win1 : (sizeof(T) == 4 ? A.win3f : A.win3);
+  const int nwin = KO == 1 ? A.nwin1 : (sizeof(T) == 4 ? A.nwin3f : A.nwin3);
   if (!win || nwin < 1) throw std::logic_error("CSR pattern without stream windows");
   const int G = (nwin + kStreamGroup - 1) / kStreamGroup;
   if (ND > 0 && (!A.wpart || (int64_t)KO * ND * nwin > A.wpart_cap))
@@ -680,6 +680,52 @@ CFSA_INST_SPMV(float)
 // ===========================================================================
 int vec_grid(int n) { return std::max(1, std::min(div_up(n, kBlock), kVecGrid)); }
 
+// Krylov vector kernels run over the flat element index i of a [n][K] array
+// (one lane per component: consecutive lanes touch consecutive 4/8-byte words,
+// three times fewer cache lines per load than one lane per node).  The grid is
+// a multiple of K, so a lane always meets the same component c = i % K.
+static int vec_grid_k(int n, int K) {
+  int G = std::max(K, std::min(div_up((int64_t)n * K, kBlock), kVecGrid));
+  return G - G % K;
+}
+
+// Block reduction of per-lane pairs binned by component: lane t of block b holds
+// component (b * kBlock + t) % K.  32 lanes per (component, value) merge every
+// K-th lane in a fixed order, then a fixed shuffle tree; lane 0 writes
+// partials[(c * ND + d) * gridDim.x + b].
+template <int K, int ND>
+__device__ void block_partials_binned(DD (&v)[ND], double *__restrict__ partials) {
+  __shared__ double rh[ND][kBlock], rl[ND][kBlock];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    rh[d][tid] = v[d].hi;
+    rl[d][tid] = v[d].lo;
+  }
+  __syncthreads();
+  static_assert(K * ND * 32 <= kBlock, "lanes per value");
+  if (tid < K * ND * 32) {
+    const int q = tid / 32, l = tid % 32;
+    const int c = q / ND, d = q % ND;
+    const int c0 = (int)(((size_t)blockIdx.x * kBlock) % K);
+    const int t0 = (c - c0 + K) % K;
+    DD a;
+    dd_zero(a);
+    for (int t = t0 + K * l; t < kBlock; t += K * 32) dd_merge(a, rh[d][t], rl[d][t]);
+#pragma unroll
+    for (int st = 16; st > 0; st >>= 1) {
+      const double oh = __shfl_down(a.hi, st, 32);
+      const double ol = __shfl_down(a.lo, st, 32);
+      dd_merge(a, oh, ol);
+    }
+    if (l == 0) {
+      const size_t o = 2 * ((size_t)(c * ND + d) * gridDim.x + blockIdx.x);
+      partials[o] = a.hi;
+      partials[o + 1] = a.lo;
+    }
+  }
+}
+
 // ===========================================================================
 // BiCGSTAB (Eigen's right-preconditioned van der Vorst form)
 // ===========================================================================
@@ -692,27 +738,22 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(int n, const T *__restrict
                                                       double *__restrict__ partials) {
   // partials: [2c] = stopping norm of b (|b|^2, or |D^-1 b|^2 when sc != null),
   //           [2c+1] = |b|^2 = r0.r of the first iteration
-  DD acc[2 * K];
-#pragma unroll
-  for (int i = 0; i < 2 * K; ++i) dd_zero(acc[i]);
-  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
-       e += (size_t)gridDim.x * kBlock) {
-    const double de = sc ? (double)sc[e] : 1.0;
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      const size_t i = e * K + c;
-      const T bi = b[i];
-      x[i] = T(0);
-      r[i] = bi;
-      r0[i] = bi;
-      p[i] = T(0);
-      v[i] = T(0);
-      const double ds = sc ? de * (double)bi : (double)bi;
-      dd_fma(acc[2 * c], ds, ds);
-      dd_fma(acc[2 * c + 1], (double)bi, (double)bi);
-    }
+  DD acc[2];
+  dd_zero(acc[0]);
+  dd_zero(acc[1]);
+  const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+    const T bi = b[i];
+    x[i] = T(0);
+    r[i] = bi;
+    r0[i] = bi;
+    p[i] = T(0);
+    v[i] = T(0);
+    const double ds = sc ? (double)sc[i / K] * (double)bi : (double)bi;
+    dd_fma(acc[0], ds, ds);
+    dd_fma(acc[1], (double)bi, (double)bi);
   }
-  block_partials<2 * K>(acc, partials);
+  block_partials_binned<K, 2>(acc, partials);
 }
 
 template <int K>
@@ -844,26 +885,16 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v1(int n, const KrylovState *__
                                                     const T *__restrict__ invd,
                                                     T *__restrict__ y) {
   if (!st->any_active) return;
-  int act[K];
-  double beta[K], om[K];
-#pragma unroll
-  for (int c = 0; c < K; ++c) {
-    act[c] = st->active[c];
-    beta[c] = st->beta[c];
-    om[c] = st->omega[c];
-  }
-  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
-       e += (size_t)gridDim.x * kBlock) {
-    const double di = (double)invd[e];
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      if (!act[c]) continue;
-      const size_t i = e * K + c;
-      const double pi = (double)r[i] + beta[c] * ((double)p[i] - om[c] * (double)v[i]);
-      const T pt = (T)pi;
-      p[i] = pt;
-      y[i] = (T)(di * (double)pt);
-    }
+  const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const int c = (int)(i0 % K);
+  if (!st->active[c]) return;
+  const double beta = st->beta[c], om = st->omega[c];
+  const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = i0; i < N; i += stride) {
+    const double pi = (double)r[i] + beta * ((double)p[i] - om * (double)v[i]);
+    const T pt = (T)pi;
+    p[i] = pt;
+    y[i] = (T)((double)invd[i / K] * (double)pt);
   }
 }
 
@@ -874,24 +905,15 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v2(int n, const KrylovState *__
                                                     const T *__restrict__ invd,
                                                     T *__restrict__ sv, T *__restrict__ z) {
   if (!st->any_active) return;
-  int act[K];
-  double al[K];
-#pragma unroll
-  for (int c = 0; c < K; ++c) {
-    act[c] = st->active[c];
-    al[c] = st->alpha[c];
-  }
-  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
-       e += (size_t)gridDim.x * kBlock) {
-    const double di = (double)invd[e];
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      if (!act[c]) continue;
-      const size_t i = e * K + c;
-      const T si = (T)((double)r[i] - al[c] * (double)v[i]);
-      sv[i] = si;
-      z[i] = (T)(di * (double)si);
-    }
+  const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const int c = (int)(i0 % K);
+  if (!st->active[c]) return;
+  const double al = st->alpha[c];
+  const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = i0; i < N; i += stride) {
+    const T si = (T)((double)r[i] - al * (double)v[i]);
+    sv[i] = si;
+    z[i] = (T)((double)invd[i / K] * (double)si);
   }
 }
 
@@ -901,40 +923,30 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v3(
     const T *__restrict__ z, const T *__restrict__ sv, const T *__restrict__ t,
     const T *__restrict__ r0, T *__restrict__ x, T *__restrict__ r,
     const T *__restrict__ sc, double *__restrict__ partials) {
-  if (!st->any_active) return;
-  int act[K];
-  double al[K], om[K];
-  DD acc[2 * K];
-#pragma unroll
-  for (int c = 0; c < K; ++c) {
-    act[c] = st->active[c];
-    al[c] = st->alpha[c];
-    om[c] = st->omega[c];
-    dd_zero(acc[2 * c]);
-    dd_zero(acc[2 * c + 1]);
+  if (!st->any_active) return;  // grid-uniform: the partials are not read then
+  const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const int c = (int)(i0 % K);
+  const bool act = st->active[c] != 0;
+  const double al = st->alpha[c], om = st->omega[c];
+  DD acc[2];
+  dd_zero(acc[0]);
+  dd_zero(acc[1]);
+  const size_t N = act ? (size_t)n * K : 0, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = i0; i < N; i += stride) {
+    x[i] = (T)((double)x[i] + (al * (double)y[i] + om * (double)z[i]));
+    const T ri = (T)((double)sv[i] - om * (double)t[i]);
+    r[i] = ri;
+    const double rs = sc ? (double)sc[i / K] * (double)ri : (double)ri;
+    dd_fma(acc[0], rs, rs);
+    dd_fma(acc[1], (double)r0[i], (double)ri);
   }
-  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
-       e += (size_t)gridDim.x * kBlock) {
-    const double de = sc ? (double)sc[e] : 1.0;
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      if (!act[c]) continue;
-      const size_t i = e * K + c;
-      x[i] = (T)((double)x[i] + (al[c] * (double)y[i] + om[c] * (double)z[i]));
-      const T ri = (T)((double)sv[i] - om[c] * (double)t[i]);
-      r[i] = ri;
-      const double rs = sc ? de * (double)ri : (double)ri;
-      dd_fma(acc[2 * c], rs, rs);
-      dd_fma(acc[2 * c + 1], (double)r0[i], (double)ri);
-    }
-  }
-  block_partials<2 * K>(acc, partials);
+  block_partials_binned<K, 2>(acc, partials);
 }
 
 template <class T, int K>
 void bicg_init(hipStream_t s, int n, const T *b, T *x, T *r, T *r0, T *p, T *v,
                const T *stop_scale, double *partials, int *grid) {
-  const int G = vec_grid(n);
+  const int G = vec_grid_k(n, K);
   *grid = G;
   hipLaunchKernelGGL((k_bicg_init<T, K>), dim3(G), dim3(kBlock), 0, s, n, b, x, r, r0, p,
                      v, stop_scale, partials);
@@ -970,14 +982,14 @@ void bicg_s_restart(hipStream_t s, KrylovState *st, const double *partials, int 
 template <class T, int K>
 void bicg_v1(hipStream_t s, int n, const KrylovState *st, const T *r, T *p, const T *v,
              const T *invd, T *y) {
-  hipLaunchKernelGGL((k_bicg_v1<T, K>), dim3(vec_grid(n)), dim3(kBlock), 0, s, n, st, r, p,
+  hipLaunchKernelGGL((k_bicg_v1<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st, r, p,
                      v, invd, y);
   CFSA_LAUNCH_CHECK();
 }
 template <class T, int K>
 void bicg_v2(hipStream_t s, int n, const KrylovState *st, const T *r, const T *v,
              const T *invd, T *sv, T *z) {
-  hipLaunchKernelGGL((k_bicg_v2<T, K>), dim3(vec_grid(n)), dim3(kBlock), 0, s, n, st, r, v,
+  hipLaunchKernelGGL((k_bicg_v2<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st, r, v,
                      invd, sv, z);
   CFSA_LAUNCH_CHECK();
 }
@@ -986,7 +998,7 @@ void bicg_v3(hipStream_t s, int n, const KrylovState *st, const T *y, const T *z
              const T *sv, const T *t, const T *r0, T *x, T *r, const T *stop_scale,
              double *partials,
              int *grid) {
-  const int G = vec_grid(n);
+  const int G = vec_grid_k(n, K);
   *grid = G;
   hipLaunchKernelGGL((k_bicg_v3<T, K>), dim3(G), dim3(kBlock), 0, s, n, st, y, z, sv, t, r0,
                      x, r, stop_scale, partials);
@@ -1029,26 +1041,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int n, const T *__restrict__
                                                     T *__restrict__ p, int scaled,
                                                     double *__restrict__ partials) {
   // partials: [2c] = stopping norm of b, [2c+1] = b . M^-1 b
-  DD acc[2 * K];
-#pragma unroll
-  for (int i = 0; i < 2 * K; ++i) dd_zero(acc[i]);
-  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
-       e += (size_t)gridDim.x * kBlock) {
-    const double de = (double)invd[e];
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      const size_t i = e * K + c;
-      const T bi = b[i];
-      x[i] = T(0);
-      r[i] = bi;
-      const T pi = (T)(de * (double)bi);
-      p[i] = pi;
-      const double sn = scaled ? (double)pi : (double)bi;
-      dd_fma(acc[2 * c], sn, sn);
-      dd_fma(acc[2 * c + 1], (double)bi, (double)pi);
-    }
+  DD acc[2];
+  dd_zero(acc[0]);
+  dd_zero(acc[1]);
+  const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+    const T bi = b[i];
+    x[i] = T(0);
+    r[i] = bi;
+    const T pi = (T)((double)invd[i / K] * (double)bi);
+    p[i] = pi;
+    const double sn = scaled ? (double)pi : (double)bi;
+    dd_fma(acc[0], sn, sn);
+    dd_fma(acc[1], (double)bi, (double)pi);
   }
-  block_partials<2 * K>(acc, partials);
+  block_partials_binned<K, 2>(acc, partials);
 }
 
 template <int K>
@@ -1139,35 +1146,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__re
                                                   T *__restrict__ x, T *__restrict__ r,
                                                   T *__restrict__ z, int scaled,
                                                   double *__restrict__ partials) {
-  if (!st->any_active) return;
-  int act[K];
-  double al[K];
-  DD acc[2 * K];
-#pragma unroll
-  for (int c = 0; c < K; ++c) {
-    act[c] = st->active[c];
-    al[c] = st->alpha[c];
-    dd_zero(acc[2 * c]);
-    dd_zero(acc[2 * c + 1]);
+  if (!st->any_active) return;  // grid-uniform: the partials are not read then
+  const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const int c = (int)(i0 % K);
+  const bool act = st->active[c] != 0;
+  const double al = st->alpha[c];
+  DD acc[2];
+  dd_zero(acc[0]);
+  dd_zero(acc[1]);
+  const size_t N = act ? (size_t)n * K : 0, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = i0; i < N; i += stride) {
+    x[i] = (T)((double)x[i] + al * (double)p[i]);
+    const T ri = (T)((double)r[i] - al * (double)q[i]);
+    r[i] = ri;
+    const T zi = (T)((double)invd[i / K] * (double)ri);
+    z[i] = zi;
+    const double sn = scaled ? (double)zi : (double)ri;
+    dd_fma(acc[0], sn, sn);
+    dd_fma(acc[1], (double)ri, (double)zi);
   }
-  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
-       e += (size_t)gridDim.x * kBlock) {
-    const double de = (double)invd[e];
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      if (!act[c]) continue;
-      const size_t i = e * K + c;
-      x[i] = (T)((double)x[i] + al[c] * (double)p[i]);
-      const T ri = (T)((double)r[i] - al[c] * (double)q[i]);
-      r[i] = ri;
-      const T zi = (T)(de * (double)ri);
-      z[i] = zi;
-      const double sn = scaled ? (double)zi : (double)ri;
-      dd_fma(acc[2 * c], sn, sn);
-      dd_fma(acc[2 * c + 1], (double)ri, (double)zi);
-    }
-  }
-  block_partials<2 * K>(acc, partials);
+  block_partials_binned<K, 2>(acc, partials);
 }
 
 template <class T, int K>
@@ -1175,22 +1173,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_v2(int n, const KrylovState *__re
                                                   const T *__restrict__ z,
                                                   T *__restrict__ p) {
   if (!st->any_active) return;
-  int act[K];
-  double be[K];
-#pragma unroll
-  for (int c = 0; c < K; ++c) {
-    act[c] = st->active[c];
-    be[c] = st->beta[c];
-  }
-  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < (size_t)n;
-       e += (size_t)gridDim.x * kBlock) {
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      if (!act[c]) continue;
-      const size_t i = e * K + c;
-      p[i] = (T)((double)z[i] + be[c] * (double)p[i]);
-    }
-  }
+  const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const int c = (int)(i0 % K);
+  if (!st->active[c]) return;
+  const double be = st->beta[c];
+  const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = i0; i < N; i += stride) p[i] = (T)((double)z[i] + be * (double)p[i]);
 }
 
 // Ends the iteration of the right-hand sides that reached the limit in
@@ -1208,7 +1196,7 @@ __global__ void k_cg_s_end(KrylovState *st) {
 template <class T, int K>
 void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p, int scaled,
              double *partials, int *grid) {
-  const int G = vec_grid(n);
+  const int G = vec_grid_k(n, K);
   *grid = G;
   hipLaunchKernelGGL((k_cg_init<T, K>), dim3(G), dim3(kBlock), 0, s, n, b, invd, x, r, p,
                      scaled, partials);
@@ -1232,7 +1220,7 @@ void cg_s_beta(hipStream_t s, KrylovState *st, const double *partials, int G) {
 template <class T, int K>
 void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
            const T *invd, T *x, T *r, T *z, int scaled, double *partials, int *grid) {
-  const int G = vec_grid(n);
+  const int G = vec_grid_k(n, K);
   *grid = G;
   hipLaunchKernelGGL((k_cg_v1<T, K>), dim3(G), dim3(kBlock), 0, s, n, st, p, q, invd, x, r, z,
                      scaled, partials);
@@ -1240,7 +1228,7 @@ void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
 }
 template <class T, int K>
 void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *z, T *p) {
-  hipLaunchKernelGGL((k_cg_v2<T, K>), dim3(vec_grid(n)), dim3(kBlock), 0, s, n, st, z, p);
+  hipLaunchKernelGGL((k_cg_v2<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st, z, p);
   CFSA_LAUNCH_CHECK();
   hipLaunchKernelGGL((k_cg_s_end<K>), dim3(1), dim3(1), 0, s, const_cast<KrylovState *>(st));
   CFSA_LAUNCH_CHECK();

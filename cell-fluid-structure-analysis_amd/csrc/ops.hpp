@@ -19,18 +19,20 @@ struct DevCsr {
   const int *rlen = nullptr;
   const int *perm = nullptr;
   int nslices = 0;
-  // CSR-stream windows (row starts, nwin + 1) for 1 and 3 right-hand sides
-  const int *win1 = nullptr, *win3 = nullptr;
-  int nwin1 = 0, nwin3 = 0;
+  // CSR-stream windows (row starts, nwin + 1): 1 right-hand side, 3 (FP64),
+  // 3 (FP32)
+  const int *win1 = nullptr, *win3 = nullptr, *win3f = nullptr;
+  int nwin1 = 0, nwin3 = 0, nwin3f = 0;
   // scratch of the CSR-stream inner products: pairs per (value, window)
   double *wpart = nullptr;
   int64_t wpart_cap = 0;  // values x windows the pair scratch holds
 };
 
-// Nonzeros staged per CSR-stream window (one block): 3 components of FP64
-// products take 24 KiB of LDS, one component 16 KiB.
-template <int KO>
-constexpr int kStreamB = KO == 3 ? 1024 : 2048;
+// Nonzeros staged per CSR-stream window (one block), measured best at 96^3
+// with the fused inner products: 3 components 1024 (FP64, 24 KiB of LDS) or
+// 1536 (FP32, 18 KiB); one component 2048.
+template <class T, int KO>
+constexpr int kStreamB = KO == 3 ? (sizeof(T) == 4 ? 1536 : 1024) : 2048;
 
 // out[e] = src[e] >= 0 ? (T)in[src[e]] : 0 over the stored entries
 template <class T>

@@ -143,22 +143,30 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   A_ = DevCsr{nu, nnzA, d_Arp.p, d_Acol.p};
   P_ = DevCsr{nv, nnzP, d_Prp.p, d_Pcol.p};
   {
-    const std::vector<int32_t> a1 = stream_windows(PA, kStreamB<1>);
-    const std::vector<int32_t> a3 = stream_windows(PA, kStreamB<3>);
-    const std::vector<int32_t> p1 = stream_windows(PP, kStreamB<1>);
-    const std::vector<int32_t> p3 = stream_windows(PP, kStreamB<3>);
+    const std::vector<int32_t> a1 = stream_windows(PA, kStreamB<double, 1>);
+    const std::vector<int32_t> a3 = stream_windows(PA, kStreamB<double, 3>);
+    const std::vector<int32_t> a3f = stream_windows(PA, kStreamB<float, 3>);
+    const std::vector<int32_t> p1 = stream_windows(PP, kStreamB<double, 1>);
+    const std::vector<int32_t> p3 = stream_windows(PP, kStreamB<double, 3>);
+    const std::vector<int32_t> p3f = stream_windows(PP, kStreamB<float, 3>);
     d_Aw1.upload(a1.data(), a1.size());
     d_Aw3.upload(a3.data(), a3.size());
+    d_Aw3f.upload(a3f.data(), a3f.size());
     d_Pw1.upload(p1.data(), p1.size());
     d_Pw3.upload(p3.data(), p3.size());
+    d_Pw3f.upload(p3f.data(), p3f.size());
     A_.win1 = d_Aw1.p;
     A_.win3 = d_Aw3.p;
+    A_.win3f = d_Aw3f.p;
     A_.nwin1 = (int)a1.size() - 1;
     A_.nwin3 = (int)a3.size() - 1;
+    A_.nwin3f = (int)a3f.size() - 1;
     P_.win1 = d_Pw1.p;
     P_.win3 = d_Pw3.p;
+    P_.win3f = d_Pw3f.p;
     P_.nwin1 = (int)p1.size() - 1;
     P_.nwin3 = (int)p3.size() - 1;
+    P_.nwin3f = (int)p3f.size() - 1;
     const int64_t cap = std::max({(int64_t)2 * A_.nwin1, (int64_t)6 * A_.nwin3,
                                   (int64_t)2 * P_.nwin1, (int64_t)6 * P_.nwin3});
     d_wpart.alloc((size_t)(2 * cap));

@@ -202,7 +202,7 @@ class NSSolver {
   std::vector<int32_t> h_tets, h_ten;
   // patterns
   DBuf<int> d_Arp, d_Acol, d_Adiag, d_Prp, d_Pcol, d_Pdiag;
-  DBuf<int> d_Aw1, d_Aw3, d_Pw1, d_Pw3;  // CSR-stream windows
+  DBuf<int> d_Aw1, d_Aw3, d_Aw3f, d_Pw1, d_Pw3, d_Pw3f;  // CSR-stream windows
   DBuf<double> d_wpart;                  // CSR-stream inner-product scratch
   DevCsr A_{}, P_{};
   // sliced-ELL copies of the two Krylov operators' patterns (lhs, L_bc)
