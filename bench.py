@@ -61,7 +61,7 @@ def make_solver(P, mesh, prof, args, precision, momentum_solver=0):
     g.setOptions(momentum_tol=args.momentum_tol, pressure_tol=args.pressure_tol,
                  precision=precision, momentum_solver=momentum_solver, timing=2,
                  check_every=args.check_every, stop_rule=args.stop_rule,
-                 pressure_solver=args.pressure_solver)
+                 pressure_solver=args.pressure_solver, inner_tol=args.inner_tol)
     return g
 
 
@@ -143,6 +143,8 @@ def main():
     ap.add_argument("--momentum-tol", type=float, default=1e-10,
                     help="tight enough that two correct solves agree to 1e-7 in velocity")
     ap.add_argument("--check-every", type=int, default=8)
+    ap.add_argument("--inner-tol", type=float, default=1e-4,
+                    help="relative tolerance of each FP32 inner solve (mixed variants)")
     ap.add_argument("--stop-rule", type=int, default=1,
                     help="0: |r| <= tol |b| (reference; its 1e10 inlet-penalty rows dominate "
                          "|b|, so the interior is not solved to tol), 1: Jacobi-scaled")

@@ -589,11 +589,17 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double 
     hipLaunchKernelGGL(k_demote_masked, dim3(2048), dim3(kBlock), 0, s, n, K, d_r64.p,
                        d_mask.p, d_b32.p);
     CFSA_LAUNCH_CHECK();
+    // Inner tolerance: inner_tol, or looser when less than that is still needed
+    // (0.3 x the reduction the least-converged active rhs needs), at most 0.1.
+    double need = 1.0;
+    for (int c = 0; c < K; ++c)
+      if (!conv[c]) need = std::min(need, tol * std::sqrt(bb[c] / rr[c]));
+    const double tin = std::min(0.1, std::max(opt.inner_tol, 0.3 * need));
     SolveResult in;
     if (method == 1)
-      in = pcg<float, K>(A, val32, invd32, d_b32.p, d_x32.p, opt.inner_tol, maxit, w32);
+      in = pcg<float, K>(A, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
     else
-      in = bicgstab<float, K>(A, val32, invd32, d_b32.p, d_x32.p, opt.inner_tol, maxit, w32);
+      in = bicgstab<float, K>(A, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
     for (int c = 0; c < K; ++c) res.iters[c] += in.iters[c];
     hipLaunchKernelGGL(k_accumulate, dim3(2048), dim3(kBlock), 0, s, (int64_t)n * K, d_x32.p,
                        x);
