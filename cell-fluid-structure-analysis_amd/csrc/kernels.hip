@@ -1128,9 +1128,11 @@ __global__ void k_cg_s_beta(KrylovState *st, const double *partials, int G) {
     st->beta[c] = st->rho[c] / st->rho_old[c];
     st->iters[c] += 1;
     if (st->iters[c] >= st->maxit) {
-      // the direction update below still runs; the loop then ends
+      // Eigen still updates the direction before leaving the loop; the
+      // direction is not used afterwards, so the rhs stops here
       st->status[c] = KS_DONE;
       st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
+      st->active[c] = 0;
     }
   }
   int any = 0;
@@ -1181,18 +1183,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_v2(int n, const KrylovState *__re
   for (size_t i = i0; i < N; i += stride) p[i] = (T)((double)z[i] + be * (double)p[i]);
 }
 
-// Ends the iteration of the right-hand sides that reached the limit in
-// k_cg_s_beta.
-template <int K>
-__global__ void k_cg_s_end(KrylovState *st) {
-  int any = 0;
-  for (int c = 0; c < K; ++c) {
-    if (st->active[c] && st->status[c] == KS_DONE) st->active[c] = 0;
-    any |= st->active[c];
-  }
-  st->any_active = any;
-}
-
 template <class T, int K>
 void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p, int scaled,
              double *partials, int *grid) {
@@ -1229,8 +1219,6 @@ void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
 template <class T, int K>
 void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *z, T *p) {
   hipLaunchKernelGGL((k_cg_v2<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st, z, p);
-  CFSA_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_cg_s_end<K>), dim3(1), dim3(1), 0, s, const_cast<KrylovState *>(st));
   CFSA_LAUNCH_CHECK();
 }
 #define CFSA_INST_CG(T, K)                                                                \

@@ -60,6 +60,7 @@ NSSolver::NSSolver() {
 
 NSSolver::~NSSolver() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  clear_graphs();
   for (auto e : ev_pool_) (void)hipEventDestroy(e);
   if (h_state) (void)hipHostFree(h_state);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -69,6 +70,7 @@ NSSolver::~NSSolver() {
 // Setup: reference NavierStokesSolver::initialize (NavierStokes.cpp:10-68)
 // ---------------------------------------------------------------------------
 void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
+  clear_graphs();
   if (m.nv <= 0 || m.nt <= 0 || m.ne < 0 || !m.coords || !m.tets || !m.ten || !m.p1_bc ||
       (m.ne > 0 && !m.p2_bc))
     throw std::invalid_argument("initialize: incomplete mesh arrays");
@@ -430,6 +432,45 @@ void NSSolver::spmv_event_end() {
   ev_used_ += 2;
 }
 
+void NSSolver::clear_graphs() {
+  for (auto &g : graphs_)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  graphs_.clear();
+}
+
+// Runs one chunk of Krylov iterations: replays the graph captured for this key
+// (all pointers, sizes and grids the chunk's launches use), capturing it on
+// first use.  The chunk holds only kernel launches (no memset or copy nodes).
+template <class F>
+void NSSolver::run_chunk(const std::vector<uintptr_t> &key, F &&launch) {
+  if (!opt.graphs) {
+    launch();
+    return;
+  }
+  for (auto &g : graphs_)
+    if (g.key == key) {
+      CFSA_HIP(hipGraphLaunch(g.exec, stream_));
+      return;
+    }
+  if (graphs_.size() >= 16) clear_graphs();
+  hipGraph_t graph = nullptr;
+  CFSA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  try {
+    launch();
+  } catch (...) {
+    (void)hipStreamEndCapture(stream_, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  CFSA_HIP(hipStreamEndCapture(stream_, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  CFSA_HIP(e);
+  graphs_.push_back({key, exec});
+  CFSA_HIP(hipGraphLaunch(exec, stream_));
+}
+
 template <class T, int K>
 SolveResult NSSolver::bicgstab(const DevCsr &A, const T *val, const T *invd, const T *b,
                                T *x, double tol, int maxit, KrylovWork<T> &w) {
@@ -445,29 +486,45 @@ SolveResult NSSolver::bicgstab(const DevCsr &A, const T *val, const T *invd, con
   bicg_s_init<K>(s, st, d_part_a.p, Gv);
   const int chunk = std::max(1, opt.check_every);
   const long max_chunks = (2L * maxit + 16) / chunk + 16;
-  for (long ch = 0;; ++ch) {
-    if (ch > max_chunks) throw std::runtime_error("BiCGSTAB driver did not terminate");
+  // SpMV launches are timed (per-SpMV events) on the second iteration of each
+  // chunk only (the first follows the host's state read); chunks run as captured graphs unless they are timed
+  auto body = [&](bool timed) {
     for (int j = 0; j < chunk; ++j) {
+      const bool ev = timed && j == std::min(1, chunk - 1);
       bicg_s_begin<K>(s, st, d_part_a.p, Gv);
       bicg_v1<T, K>(s, n, st, w.r.p, w.p.p, w.v.p, invd, w.y.p);
-      if (time_spmv) spmv_event_begin();
+      if (ev) spmv_event_begin();
       if (K == 3)
         spmv3_dot<T>(s, A, val, w.y.p, w.v.p, w.r0.p, act, d_part_b.p, &Gb);
       else
         spmv1_dot<T>(s, A, val, w.y.p, w.v.p, w.r0.p, act, d_part_b.p, &Gb);
-      if (time_spmv) spmv_event_end();
+      if (ev) spmv_event_end();
       bicg_s_alpha<K>(s, st, d_part_b.p, Gb);
       bicg_v2<T, K>(s, n, st, w.r.p, w.v.p, invd, w.s.p, w.z.p);
-      if (time_spmv) spmv_event_begin();
+      if (ev) spmv_event_begin();
       if (K == 3)
         spmv3_dot2<T>(s, A, val, w.z.p, w.t.p, w.s.p, act, d_part_c.p, &Gc);
       else
         spmv1_dot2<T>(s, A, val, w.z.p, w.t.p, w.s.p, act, d_part_c.p, &Gc);
-      if (time_spmv) spmv_event_end();
+      if (ev) spmv_event_end();
       bicg_s_omega<K>(s, st, d_part_c.p, Gc);
       bicg_v3<T, K>(s, n, st, w.y.p, w.z.p, w.s.p, w.t.p, w.r0.p, x, w.r.p, sc, d_part_a.p,
                     &Gv);
     }
+  };
+  const std::vector<uintptr_t> key = {
+      1, sizeof(T), K, (uintptr_t)n, (uintptr_t)chunk, (uintptr_t)A.rowptr,
+      (uintptr_t)A.col, (uintptr_t)A.sptr, (uintptr_t)val, (uintptr_t)invd, (uintptr_t)x,
+      (uintptr_t)w.r.p, (uintptr_t)w.r0.p, (uintptr_t)w.p.p, (uintptr_t)w.v.p,
+      (uintptr_t)w.y.p, (uintptr_t)w.s.p, (uintptr_t)w.z.p, (uintptr_t)w.t.p, (uintptr_t)sc,
+      (uintptr_t)st, (uintptr_t)d_part_a.p, (uintptr_t)d_part_b.p, (uintptr_t)d_part_c.p,
+      (uintptr_t)Gv};
+  for (long ch = 0;; ++ch) {
+    if (ch > max_chunks) throw std::runtime_error("BiCGSTAB driver did not terminate");
+    if (time_spmv)
+      body(true);
+    else
+      run_chunk(key, [&] { body(false); });
     read_state();
     bool all_done = true;
     int rs[3] = {0, 0, 0};
@@ -510,25 +567,38 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
   cg_s_init<K>(s, st, d_part_a.p, Gv);
   const int chunk = std::max(1, opt.check_every);
   const long max_chunks = (long)maxit / chunk + 16;
-  for (long ch = 0;; ++ch) {
-    if (ch > max_chunks) throw std::runtime_error("PCG driver did not terminate");
-    read_state();
-    bool done = true;
-    for (int c = 0; c < K; ++c) done = done && h_state->status[c] == KS_DONE;
-    if (done) break;
+  auto body = [&](bool timed) {
     for (int j = 0; j < chunk; ++j) {
-      if (time_spmv) spmv_event_begin();
+      const bool ev = timed && j == std::min(1, chunk - 1);
+      if (ev) spmv_event_begin();
       if (K == 3)
         spmv3_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
       else
         spmv1_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
-      if (time_spmv) spmv_event_end();
+      if (ev) spmv_event_end();
       cg_s_alpha<K>(s, st, d_part_b.p, Gb);
       cg_v1<T, K>(s, n, st, w.p.p, w.v.p, invd, x, w.r.p, w.z.p, opt.stop_scaled, d_part_a.p,
                   &Gv);
       cg_s_beta<K>(s, st, d_part_a.p, Gv);
       cg_v2<T, K>(s, n, st, w.z.p, w.p.p);
     }
+  };
+  const std::vector<uintptr_t> key = {
+      2, sizeof(T), K, (uintptr_t)n, (uintptr_t)chunk, (uintptr_t)A.rowptr,
+      (uintptr_t)A.col, (uintptr_t)A.sptr, (uintptr_t)val, (uintptr_t)invd, (uintptr_t)x,
+      (uintptr_t)w.r.p, (uintptr_t)w.p.p, (uintptr_t)w.v.p, (uintptr_t)w.z.p,
+      (uintptr_t)st, (uintptr_t)d_part_a.p, (uintptr_t)d_part_b.p,
+      (uintptr_t)opt.stop_scaled, (uintptr_t)Gv};
+  for (long ch = 0;; ++ch) {
+    if (ch > max_chunks) throw std::runtime_error("PCG driver did not terminate");
+    read_state();
+    bool done = true;
+    for (int c = 0; c < K; ++c) done = done && h_state->status[c] == KS_DONE;
+    if (done) break;
+    if (time_spmv)
+      body(true);
+    else
+      run_chunk(key, [&] { body(false); });
   }
   SolveResult res;
   res.ok = true;

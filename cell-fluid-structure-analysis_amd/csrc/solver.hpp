@@ -72,6 +72,7 @@ struct SolverOptions {
   int stop_scaled = 0;         // 1: stop on |D^-1 r| <= tol |D^-1 b| (Jacobi-scaled)
   int assembly = 0;            // 0 row gather (reference summation order), 1 coloured
   int spmv_layout = 0;         // Krylov SpMV: 0 auto, 1 CSR-stream, 2 sliced-ELL
+  int graphs = 1;              // replay Krylov iteration chunks as captured HIP graphs
 };
 
 struct SolveResult {
@@ -235,6 +236,15 @@ class NSSolver {
   KrylovState *h_state = nullptr;
   std::vector<hipEvent_t> ev_pool_;
   int ev_used_ = 0;
+  // captured Krylov iteration chunks, keyed by every argument they bake in
+  struct ChunkGraph {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<ChunkGraph> graphs_;
+  void clear_graphs();
+  template <class F>
+  void run_chunk(const std::vector<uintptr_t> &key, F &&launch);
   // adjoint state
   DBuf<int> d_Ptslot;  // slot of (j, i) for each slot (i, j) of the P1 pattern
   DBuf<double> d_G3T, d_lu, d_lus, d_lb, d_lg, d_lgp, d_lpc, d_lrp, d_lp, d_lp_new, d_contrib,

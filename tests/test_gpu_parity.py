@@ -234,14 +234,21 @@ def test_inlet_profile_parity(P, O):
     np.testing.assert_array_equal(g.getVelocity()[lab == 1], prof[lab == 1])
 
 
-def test_repeat_runs_bitwise_identical(P, O):
+@pytest.mark.parametrize("msolver,precision", [(0, 0), (1, 2), (0, 1)])
+def test_repeat_runs_bitwise_identical(P, O, msolver, precision):
+    """Repeated runs, and graph replay against eager launches, give the same
+    bits (each step replays its captured chunks many times, with eager work
+    and host synchronisation in between)."""
     out = []
-    for _ in range(2):
-        g, o, pm, om = make_case(P, O, "kuhn", 4)
+    for graphs in (1, 1, 0):
+        g, o, pm, om = make_case(P, O, "kuhn", 4, momentum_solver=msolver)
+        g.setOptions(graphs=graphs, precision=precision)
         for _ in range(3):
             assert g.pisoStep()
-        out.append((g.getVelocity(), g.getPressure()))
-    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+        out.append((g.getVelocity(), g.getPressure(), g.stepStats()))
+    for u, p, st in out[1:]:
+        assert np.array_equal(out[0][0], u) and np.array_equal(out[0][1], p)
+        assert st["momentum_iters"] == out[0][2]["momentum_iters"]
 
 
 # ------------------------------------------------------------------ failures
