@@ -57,6 +57,7 @@ def make_solver(P, mesh, prof, args, precision, momentum_solver=0):
     g = P.NavierStokesSolver()
     g.initialize(mesh, args.rho, args.mu)
     g.setDt(args.dt)
+    g.setNumberOfCorrections(args.corrections)
     g.setInletProfile(prof)
     g.setOptions(momentum_tol=args.momentum_tol, pressure_tol=args.pressure_tol,
                  precision=precision, momentum_solver=momentum_solver, timing=2,
@@ -135,10 +136,15 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=96, help="cells per side of the unit cube")
+    ap.add_argument("--workload", choices=["kuhn", "cyl2"], default="kuhn",
+                    help="kuhn: unit-cube Kuhn channel (default); cyl2: the README cylinder "
+                         "(length 30, radius 2, 35 divisions, max element 0.03; parabolic "
+                         "inlet peak 0.5, dt 0.01, 2 correctors)")
     ap.add_argument("--rho", type=float, default=1060.0)
     ap.add_argument("--mu", type=float, default=0.004)
     ap.add_argument("--re", type=float, default=100.0)
-    ap.add_argument("--dt", type=float, default=1e-3)
+    ap.add_argument("--dt", type=float, default=None, help="1e-3 (kuhn) / 0.01 (cyl2)")
+    ap.add_argument("--corrections", type=int, default=None, help="1 (kuhn) / 2 (cyl2)")
     ap.add_argument("--pressure-tol", type=float, default=1e-8)
     ap.add_argument("--momentum-tol", type=float, default=1e-10,
                     help="tight enough that two correct solves agree to 1e-7 in velocity")
@@ -175,13 +181,31 @@ def main():
 
     t0 = time.perf_counter()
     n = args.n
-    mesh = P.Mesh.structuredRectangularPrism(1.0, 1.0, 1.0, n, n, n)
-    mesh.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
-    U = args.re * args.mu / (args.rho * 1.0)
-    X = np.vstack([mesh.vertices(), mesh.edgeNodes()])
-    prof = np.zeros((len(X), 3))
-    prof[:, 0] = U * 16.0 * X[:, 1] * (1 - X[:, 1]) * X[:, 2] * (1 - X[:, 2])
-    log(f"mesh {n}^3: {mesh.sizes()} in {time.perf_counter() - t0:.1f}s")
+    cyl = args.workload == "cyl2"
+    if args.dt is None:
+        args.dt = 0.01 if cyl else 1e-3
+    if args.corrections is None:
+        args.corrections = 2 if cyl else 1
+    if cyl:
+        R = 2.0
+        mesh = P.Mesh.cylinder(30.0, R, 35, 0.03)
+        mesh.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+        U = 0.5
+        X = np.vstack([mesh.vertices(), mesh.edgeNodes()])
+        prof = np.zeros((len(X), 3))
+        prof[:, 0] = U * np.clip(1.0 - (X[:, 1] ** 2 + X[:, 2] ** 2) / R ** 2, 0.0, None)
+        workload = "cylinder_L30_R2_div35_a0.03"
+        mesh_desc = "own cylinder mesher (TetGen is not available), length 30, radius 2"
+    else:
+        mesh = P.Mesh.structuredRectangularPrism(1.0, 1.0, 1.0, n, n, n)
+        mesh.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+        U = args.re * args.mu / (args.rho * 1.0)
+        X = np.vstack([mesh.vertices(), mesh.edgeNodes()])
+        prof = np.zeros((len(X), 3))
+        prof[:, 0] = U * 16.0 * X[:, 1] * (1 - X[:, 1]) * X[:, 2] * (1 - X[:, 2])
+        workload = f"kuhn{n}_channel"
+        mesh_desc = f"unit cube {n}^3 Kuhn 6-tet"
+    log(f"mesh {workload}: {mesh.sizes()} in {time.perf_counter() - t0:.1f}s")
 
     t0 = time.perf_counter()
     g64 = make_solver(P, mesh, prof, args, 0)
@@ -252,7 +276,7 @@ def main():
         roof = {"bound": "hbm", "kernel": "k_csr_stream<float,3> (momentum SpMV, 3 rhs, "
                 "fused inner products)", "bytes_per_launch": b32, "avg_launch_ms": ms}
         ach = b32 / ms / 1e6
-    traffic = profiled_traffic() if (primary != "fp64" and n == 96) else None
+    traffic = profiled_traffic() if (primary != "fp64" and not cyl and n == 96) else None
     roof.update({"achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                  "traffic_source": str(TRAFFIC_PROFILE.relative_to(REPO)) if traffic else None})
@@ -271,11 +295,11 @@ def main():
            "scaling": "weak", "vs_baseline": None,
            "dtype": "f64" if primary == "fp64" else "f32 inner / f64 refinement",
            "data": "synthetic",
-           "config": {"workload": f"kuhn{n}_channel", "mesh": f"unit cube {n}^3 Kuhn 6-tet",
+           "config": {"workload": workload, "mesh": mesh_desc, "corrections": args.corrections,
                       "tets": sz["nt"], "velocity_dofs": 3 * sz["nu"],
                       "pressure_dofs": sz["nv"], "nnz_momentum": sz["nnzA"],
                       "nnz_pressure": sz["nnzP"], "density": args.rho, "viscosity": args.mu,
-                      "inlet_peak": U, "reynolds": args.re, "dt": args.dt,
+                      "inlet_peak": U, "reynolds": None if cyl else args.re, "dt": args.dt,
                       "pressure_tol": args.pressure_tol, "momentum_tol": args.momentum_tol,
                       "stop_rule": "jacobi_scaled" if args.stop_rule else "reference",
                       "pressure_solver": "cg_jacobi" if args.pressure_solver else "bicgstab_jacobi",

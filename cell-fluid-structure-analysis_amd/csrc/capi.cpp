@@ -63,6 +63,7 @@ cfsa::SolverOptions to_opts(const cfsa_solver_options &o) {
   s.inner_tol = o.inner_tol;
   s.max_refine = o.max_refine;
   s.check_every = o.check_every;
+  s.check_every_1 = o.check_every_1;
   s.timing = o.timing;
   s.stop_scaled = o.stop_rule;
   s.assembly = o.assembly;
@@ -452,7 +453,7 @@ int cfsa_ns_set_time(cfsa_ns *h, double t) {
 int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt) {
   CHECK_PTR(h);
   CHECK_PTR(opt);
-  if (opt->max_iter < 0 || opt->check_every < 1 || opt->precision < 0 || opt->precision > 2 ||
+  if (opt->max_iter < 0 || opt->check_every < 1 || opt->check_every_1 < 1 || opt->precision < 0 || opt->precision > 2 ||
       opt->pressure_solver < 0 || opt->pressure_solver > 1 || !(opt->momentum_tol >= 0) ||
       !(opt->pressure_tol >= 0) || !(opt->inner_tol > 0) || opt->max_refine < 0 ||
       opt->stop_rule < 0 || opt->stop_rule > 1 || opt->assembly < 0 || opt->assembly > 1 ||
@@ -474,6 +475,7 @@ int cfsa_ns_get_options(const cfsa_ns *h, cfsa_solver_options *o) {
   o->inner_tol = s.inner_tol;
   o->max_refine = s.max_refine;
   o->check_every = s.check_every;
+  o->check_every_1 = s.check_every_1;
   o->timing = s.timing;
   o->stop_rule = s.stop_scaled;
   o->assembly = s.assembly;

@@ -484,7 +484,7 @@ SolveResult NSSolver::bicgstab(const DevCsr &A, const T *val, const T *invd, con
   const T *sc = opt.stop_scaled ? invd : nullptr;
   bicg_init<T, K>(s, n, b, x, w.r.p, w.r0.p, w.p.p, w.v.p, sc, d_part_a.p, &Gv);
   bicg_s_init<K>(s, st, d_part_a.p, Gv);
-  const int chunk = std::max(1, opt.check_every);
+  const int chunk = std::max(1, K == 1 ? opt.check_every_1 : opt.check_every);
   const long max_chunks = (2L * maxit + 16) / chunk + 16;
   // SpMV launches are timed (per-SpMV events) on the second iteration of each
   // chunk only (the first follows the host's state read); chunks run as captured graphs unless they are timed
@@ -565,7 +565,7 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
   int Gv = 0, Gb = 0;
   cg_init<T, K>(s, n, b, invd, x, w.r.p, w.p.p, opt.stop_scaled, d_part_a.p, &Gv);
   cg_s_init<K>(s, st, d_part_a.p, Gv);
-  const int chunk = std::max(1, opt.check_every);
+  const int chunk = std::max(1, K == 1 ? opt.check_every_1 : opt.check_every);
   const long max_chunks = (long)maxit / chunk + 16;
   auto body = [&](bool timed) {
     for (int j = 0; j < chunk; ++j) {

@@ -67,7 +67,8 @@ struct SolverOptions {
   int precision = 0;           // 0 FP64, 1 FP32 inner + FP64 refinement, 2 momentum only
   double inner_tol = 1e-4;     // relative tolerance of each FP32 inner solve
   int max_refine = 40;         // outer refinement sweeps (mixed)
-  int check_every = 8;         // Krylov iterations between host status reads
+  int check_every = 8;         // Krylov iterations between host status reads (3 rhs)
+  int check_every_1 = 32;      // the same for single-rhs solves (pressure: short iterations)
   int timing = 0;              // 1: per-phase events; 2: also per-SpMV events
   int stop_scaled = 0;         // 1: stop on |D^-1 r| <= tol |D^-1 b| (Jacobi-scaled)
   int assembly = 0;            // 0 row gather (reference summation order), 1 coloured

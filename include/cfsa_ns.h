@@ -70,6 +70,8 @@ typedef struct cfsa_solver_options {
                             1 CSR-stream, 2 sliced-ELL; every layout gives the same bits */
   int momentum_solver;   /* 0 BiCGSTAB (reference), 1 Jacobi PCG (the momentum matrix
                             M/dt + K + penalty diagonal is symmetric positive definite) */
+  int check_every_1;     /* the same for single right-hand-side (pressure) solves, 32: their
+                            iterations are short, so host reads cost relatively more */
   int graphs;            /* 1 (default): each chunk of check_every Krylov iterations is
                             captured once as a HIP graph and replayed; 0: launched eagerly.
                             Same results either way.  Chunks with per-SpMV timing
