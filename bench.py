@@ -53,7 +53,7 @@ def spmv_bytes(n, nnz, w, k=3):
     return (w + 4) * nnz + 4 * (n + 1) + 2 * k * w * n
 
 
-def make_solver(P, mesh, prof, args, precision, momentum_solver=0):
+def make_solver(P, mesh, prof, args, precision, momentum_solver=0, initial_guess=0):
     g = P.NavierStokesSolver()
     g.initialize(mesh, args.rho, args.mu)
     g.setDt(args.dt)
@@ -62,7 +62,8 @@ def make_solver(P, mesh, prof, args, precision, momentum_solver=0):
     g.setOptions(momentum_tol=args.momentum_tol, pressure_tol=args.pressure_tol,
                  precision=precision, momentum_solver=momentum_solver, timing=2,
                  check_every=args.check_every, stop_rule=args.stop_rule,
-                 pressure_solver=args.pressure_solver, inner_tol=args.inner_tol)
+                 pressure_solver=args.pressure_solver, inner_tol=args.inner_tol,
+                 initial_guess=initial_guess)
     return g
 
 
@@ -126,7 +127,10 @@ def _cpu_baseline(O, g, fp64_stats, sz, pressure_method):
 # mixed-precision variants: (name, precision, momentum solver)
 #   precision 1: FP32 inner solves for momentum and pressure, 2: momentum only
 #   momentum solver 0: BiCGSTAB (the reference's), 1: Jacobi CG (lhs is SPD)
-MIXED = [("mixed_bicgstab_all", 1, 0), ("mixed_bicgstab", 2, 0), ("mixed_cg", 2, 1)]
+MIXED = [("mixed_bicgstab_all", 1, 0, 0), ("mixed_bicgstab", 2, 0, 0), ("mixed_cg", 2, 1, 0),
+         ("mixed_cg_warm", 2, 1, 1)]
+# variants that keep the reference's zero initial guess compete for the headline
+HEADLINE = {"mixed_bicgstab_all", "mixed_bicgstab", "mixed_cg"}
 VEL_ERR_MAX = 1e-7
 
 
@@ -227,11 +231,11 @@ def main():
                     "spmv_avg_ms": spmv_avg(st64)}}
     primary, best = "fp64", None
     if not args.fp64_only:
-        for name, precision, msolver in MIXED:
+        for name, precision, msolver, guess in MIXED:
             if args.only and name != args.only:
                 continue
             t0 = time.perf_counter()
-            g = make_solver(P, mesh, prof, args, precision, msolver)
+            g = make_solver(P, mesh, prof, args, precision, msolver, guess)
             el, st = run(P, g, args.steps, args.warmup, torch)
             log(f"{name}: {args.steps} steps in {el:.3f}s; last {st[-1]}")
             u, p = g.getVelocity(), g.getPressure()
@@ -242,13 +246,14 @@ def main():
             res[name] = {"steps_per_s": args.steps / el, "ms_per_step": 1e3 * el / args.steps,
                          "precision": precision,
                          "momentum_solver": "cg_jacobi" if msolver else "bicgstab_jacobi",
+                         "initial_guess": "u^n" if guess else "zero",
                          "momentum_inner_iters": st[-1]["momentum_iters"],
                          "momentum_refinements": st[-1]["momentum_refinements"],
                          "pressure_iters": st[-1]["pressure_iters"],
                          "pressure_refinements": st[-1]["pressure_refinements"],
                          "spmv_avg_ms": spmv_avg(st),
                          "speedup_vs_fp64": el64 / el, "error_vs_fp64": err}
-            ok = err["velocity_rel_l2"] <= VEL_ERR_MAX
+            ok = err["velocity_rel_l2"] <= VEL_ERR_MAX and name in HEADLINE
             if ok and (best is None or el < best[1]):
                 best = (name, el)
             del g

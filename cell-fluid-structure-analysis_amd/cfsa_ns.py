@@ -39,7 +39,7 @@ class SolverOptions(C.Structure):
                 ("max_refine", C.c_int), ("check_every", C.c_int), ("timing", C.c_int),
                 ("stop_rule", C.c_int), ("assembly", C.c_int),
                 ("spmv_layout", C.c_int), ("momentum_solver", C.c_int),
-                ("check_every_1", C.c_int), ("graphs", C.c_int)]
+                ("check_every_1", C.c_int), ("graphs", C.c_int), ("initial_guess", C.c_int)]
 
 
 class StepStats(C.Structure):

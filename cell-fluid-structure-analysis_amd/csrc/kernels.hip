@@ -1146,8 +1146,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__re
                                                   const T *__restrict__ q,
                                                   const T *__restrict__ invd,
                                                   T *__restrict__ x, T *__restrict__ r,
-                                                  T *__restrict__ z, int scaled,
-                                                  double *__restrict__ partials) {
+                                                  int scaled, double *__restrict__ partials) {
   if (!st->any_active) return;  // grid-uniform: the partials are not read then
   const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
   const int c = (int)(i0 % K);
@@ -1161,8 +1160,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__re
     x[i] = (T)((double)x[i] + al * (double)p[i]);
     const T ri = (T)((double)r[i] - al * (double)q[i]);
     r[i] = ri;
-    const T zi = (T)((double)invd[i / K] * (double)ri);
-    z[i] = zi;
+    const T zi = (T)((double)invd[i / K] * (double)ri);  // z = M^-1 r, not stored
     const double sn = scaled ? (double)zi : (double)ri;
     dd_fma(acc[0], sn, sn);
     dd_fma(acc[1], (double)ri, (double)zi);
@@ -1170,9 +1168,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__re
   block_partials_binned<K, 2>(acc, partials);
 }
 
+// p = z + beta p with z = M^-1 r recomputed from r (same rounding as in k_cg_v1)
 template <class T, int K>
 __global__ __launch_bounds__(kBlock) void k_cg_v2(int n, const KrylovState *__restrict__ st,
-                                                  const T *__restrict__ z,
+                                                  const T *__restrict__ r,
+                                                  const T *__restrict__ invd,
                                                   T *__restrict__ p) {
   if (!st->any_active) return;
   const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
@@ -1180,7 +1180,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_v2(int n, const KrylovState *__re
   if (!st->active[c]) return;
   const double be = st->beta[c];
   const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
-  for (size_t i = i0; i < N; i += stride) p[i] = (T)((double)z[i] + be * (double)p[i]);
+  for (size_t i = i0; i < N; i += stride) {
+    const T zi = (T)((double)invd[i / K] * (double)r[i]);
+    p[i] = (T)((double)zi + be * (double)p[i]);
+  }
 }
 
 template <class T, int K>
@@ -1209,24 +1212,25 @@ void cg_s_beta(hipStream_t s, KrylovState *st, const double *partials, int G) {
 }
 template <class T, int K>
 void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
-           const T *invd, T *x, T *r, T *z, int scaled, double *partials, int *grid) {
+           const T *invd, T *x, T *r, int scaled, double *partials, int *grid) {
   const int G = vec_grid_k(n, K);
   *grid = G;
-  hipLaunchKernelGGL((k_cg_v1<T, K>), dim3(G), dim3(kBlock), 0, s, n, st, p, q, invd, x, r, z,
+  hipLaunchKernelGGL((k_cg_v1<T, K>), dim3(G), dim3(kBlock), 0, s, n, st, p, q, invd, x, r,
                      scaled, partials);
   CFSA_LAUNCH_CHECK();
 }
 template <class T, int K>
-void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *z, T *p) {
-  hipLaunchKernelGGL((k_cg_v2<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st, z, p);
+void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *r, const T *invd, T *p) {
+  hipLaunchKernelGGL((k_cg_v2<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st, r,
+                     invd, p);
   CFSA_LAUNCH_CHECK();
 }
 #define CFSA_INST_CG(T, K)                                                                \
   template void cg_init<T, K>(hipStream_t, int, const T *, const T *, T *, T *, T *, int,   \
                               double *, int *);                                           \
   template void cg_v1<T, K>(hipStream_t, int, const KrylovState *, const T *, const T *,  \
-                            const T *, T *, T *, T *, int, double *, int *);              \
-  template void cg_v2<T, K>(hipStream_t, int, const KrylovState *, const T *, T *);
+                            const T *, T *, T *, int, double *, int *);                   \
+  template void cg_v2<T, K>(hipStream_t, int, const KrylovState *, const T *, const T *, T *);
 CFSA_INST_CG(double, 1)
 CFSA_INST_CG(double, 3)
 CFSA_INST_CG(float, 1)

@@ -127,9 +127,9 @@ template <int K>
 void cg_s_beta(hipStream_t s, KrylovState *st, const double *partials, int G);
 template <class T, int K>
 void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
-           const T *invd, T *x, T *r, T *z, int scaled, double *partials, int *grid);
+           const T *invd, T *x, T *r, int scaled, double *partials, int *grid);
 template <class T, int K>
-void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *z, T *p);
+void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *r, const T *invd, T *p);
 
 // ---- FEM element / node kernels ----
 void assemble_colour(hipStream_t s, int ntc, const int *order, const int *tets,

@@ -577,16 +577,15 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
         spmv1_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
       if (ev) spmv_event_end();
       cg_s_alpha<K>(s, st, d_part_b.p, Gb);
-      cg_v1<T, K>(s, n, st, w.p.p, w.v.p, invd, x, w.r.p, w.z.p, opt.stop_scaled, d_part_a.p,
-                  &Gv);
+      cg_v1<T, K>(s, n, st, w.p.p, w.v.p, invd, x, w.r.p, opt.stop_scaled, d_part_a.p, &Gv);
       cg_s_beta<K>(s, st, d_part_a.p, Gv);
-      cg_v2<T, K>(s, n, st, w.z.p, w.p.p);
+      cg_v2<T, K>(s, n, st, w.r.p, invd, w.p.p);
     }
   };
   const std::vector<uintptr_t> key = {
       2, sizeof(T), K, (uintptr_t)n, (uintptr_t)chunk, (uintptr_t)A.rowptr,
       (uintptr_t)A.col, (uintptr_t)A.sptr, (uintptr_t)val, (uintptr_t)invd, (uintptr_t)x,
-      (uintptr_t)w.r.p, (uintptr_t)w.p.p, (uintptr_t)w.v.p, (uintptr_t)w.z.p,
+      (uintptr_t)w.r.p, (uintptr_t)w.p.p, (uintptr_t)w.v.p,
       (uintptr_t)st, (uintptr_t)d_part_a.p, (uintptr_t)d_part_b.p,
       (uintptr_t)opt.stop_scaled, (uintptr_t)Gv};
   for (long ch = 0;; ++ch) {
@@ -616,12 +615,12 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
 template <int K>
 SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double *invd64,
                              const float *val32, const float *invd32, const double *b,
-                             double *x, double tol, int maxit, int method) {
+                             double *x, double tol, int maxit, int method, bool keep_x) {
   hipStream_t s = stream_;
   const int n = A.n;
   KrylovWork<double> &w64 = (K == 3) ? w3_ : w1_;
   KrylovWork<float> &w32 = (K == 3) ? f3_ : f1_;
-  CFSA_HIP(hipMemsetAsync(x, 0, (size_t)n * K * sizeof(double), s));
+  if (!keep_x) CFSA_HIP(hipMemsetAsync(x, 0, (size_t)n * K * sizeof(double), s));
   int Gr = 0;
   set_mask(1, K == 3, K == 3);
   // the outer test uses the same stopping rule as the FP64 solvers
@@ -641,8 +640,19 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double 
     CFSA_HIP(hipStreamSynchronize(s));
     return std::vector<double>(out, out + 3);
   };
-  std::vector<double> bb = residual();  // x = 0: r = b
-  std::vector<double> rr = bb;
+  std::vector<double> bb, rr;
+  if (keep_x) {
+    // stopping norms of b itself, then the residual of the initial guess
+    double *xs = x;
+    x = d_xref.p;
+    CFSA_HIP(hipMemsetAsync(x, 0, (size_t)n * K * sizeof(double), s));
+    bb = residual();
+    x = xs;
+    rr = residual();
+  } else {
+    bb = residual();  // x = 0: r = b
+    rr = bb;
+  }
   SolveResult res;
   int conv[3] = {1, 1, 1};
   for (int k = 0;; ++k) {
@@ -686,9 +696,15 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double 
 
 // precision 1: FP32 inner solves for both systems; 2: for the momentum solve only
 SolveResult NSSolver::solve_momentum(const double *b, double *x) {
-  if (opt.precision == 1 || opt.precision == 2)
+  if (opt.precision == 1 || opt.precision == 2) {
+    // initial_guess 1: start the refinement from the current velocity u^n
+    const bool guess = opt.initial_guess == 1 && x != d_u.p;
+    if (guess)
+      CFSA_HIP(hipMemcpyAsync(x, d_u.p, (size_t)nu * 3 * sizeof(double),
+                              hipMemcpyDeviceToDevice, stream_));
     return refine<3>(momA(), momV(), d_invdA.p, d_lhs32.p, d_invdA32.p, b, x,
-                     opt.momentum_tol, opt.max_iter, opt.momentum_solver);
+                     opt.momentum_tol, opt.max_iter, opt.momentum_solver, guess);
+  }
   if (opt.momentum_solver == 1)
     return pcg<double, 3>(momA(), momV(), d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
                           w3_);

@@ -74,6 +74,7 @@ struct SolverOptions {
   int assembly = 0;            // 0 row gather (reference summation order), 1 coloured
   int spmv_layout = 0;         // Krylov SpMV: 0 auto, 1 CSR-stream, 2 sliced-ELL
   int graphs = 1;              // replay Krylov iteration chunks as captured HIP graphs
+  int initial_guess = 0;       // momentum solve from 0 (reference) or 1: from u^n (mixed)
 };
 
 struct SolveResult {
@@ -285,7 +286,7 @@ class NSSolver {
   template <int K>
   SolveResult refine(const DevCsr &A, const double *val64, const double *invd64,
                      const float *val32, const float *invd32, const double *b, double *x,
-                     double tol, int maxit, int method);
+                     double tol, int maxit, int method, bool keep_x = false);
   SolveResult solve_momentum(const double *b, double *x);
   SolveResult solve_pressure(const double *b, double *x);
 };

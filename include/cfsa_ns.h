@@ -76,6 +76,8 @@ typedef struct cfsa_solver_options {
                             captured once as a HIP graph and replayed; 0: launched eagerly.
                             Same results either way.  Chunks with per-SpMV timing
                             (timing = 2) always run eagerly. */
+  int initial_guess;     /* 0 (reference): the momentum solve starts from zero; 1: the
+                            mixed-precision momentum refinement starts from u^n */
 } cfsa_solver_options;
 
 typedef struct cfsa_step_stats {
