@@ -349,11 +349,9 @@ int cfsa_ns_build_sparse_matrices(cfsa_ns *h) {
     return fail(CFSA_ERR_NOT_INIT, "[NS] Error: mesh not initialized");
   }
   return guard([&]() -> int {
-    if (h->mesh) {
-      const cfsa::Mesh &m = h->mesh->m;
-      h->s.update_mesh(m.vertices.data(), m.p1_bc.data(), m.p2_bc.data());
-      h->synced = h->mesh->version;
-    }
+    // the reference re-reads the mesh here; geometry and labels are pushed
+    // when the mesh changed since the last push
+    sync_mesh(h);
     h->s.build_sparse_matrices();
     return CFSA_OK;
   });

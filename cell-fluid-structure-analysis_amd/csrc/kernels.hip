@@ -1348,6 +1348,108 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(
   }
 }
 
+// Slot map of the row gather (built once per pattern): for incidence entry k
+// (row r, tet t = inct[k]), loc[k] = local index a of r in t, amap[10 k + b] =
+// slot of (r, node b of t) in the P2 pattern and, for vertex rows (k <
+// incp[nv]), pmap[4 k + b] = slot of (r, vertex b) in the P1 pattern.
+__global__ __launch_bounds__(kBlock) void k_slot_map(
+    int nu, const int *__restrict__ incp, const int *__restrict__ inct,
+    const int *__restrict__ tets, const int *__restrict__ ten, int nv,
+    const int *__restrict__ Arp, const int *__restrict__ Acol, const int *__restrict__ Prp,
+    const int *__restrict__ Pcol, uint8_t *__restrict__ loc, int *__restrict__ amap,
+    int *__restrict__ pmap) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nu) return;
+  for (int k = incp[r]; k < incp[r + 1]; ++k) {
+    int node[10];
+    tet_nodes(inct[k], tets, ten, nv, node);
+    int a = 0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+      if (node[i] == r) a = i;
+    loc[k] = (uint8_t)a;
+    for (int b = 0; b < 10; ++b) amap[(size_t)k * 10 + b] = find_slot(Arp, Acol, r, node[b]);
+    if (r < nv)
+      for (int b = 0; b < 4; ++b) pmap[(size_t)k * 4 + b] = find_slot(Prp, Pcol, r, node[b]);
+  }
+}
+
+void build_slot_map(hipStream_t s, int nu, const int *incp, const int *inct, const int *tets,
+                    const int *ten, int nv, const DevCsr &A, const DevCsr &P, uint8_t *loc,
+                    int *amap, int *pmap) {
+  if (nu <= 0) return;
+  hipLaunchKernelGGL(k_slot_map, dim3(div_up(nu, kBlock)), dim3(kBlock), 0, s, nu, incp, inct,
+                     tets, ten, nv, A.rowptr, A.col, P.rowptr, P.col, loc, amap, pmap);
+  CFSA_LAUNCH_CHECK();
+}
+
+// Row gather with the slot map: the same contributions, in the same ascending
+// tet order per row, without searching the rows for the slots.
+__global__ __launch_bounds__(kBlock) void k_assemble_mapped(
+    int nu, int nv, const int *__restrict__ incp, const int *__restrict__ inct,
+    const uint8_t *__restrict__ loc, const int *__restrict__ amap,
+    const int *__restrict__ pmap, const double *__restrict__ vol,
+    const double *__restrict__ grad, double rho, double inv_mu, double *__restrict__ M,
+    double *__restrict__ K, double *__restrict__ L, double *__restrict__ G3) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nu) return;
+  for (int k = incp[r]; k < incp[r + 1]; ++k) {
+    const int t = inct[k];
+    const int a = loc[k];
+    double g[10][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g[i][c] = grad[(size_t)t * 12 + i * 3 + c];
+#pragma unroll
+    for (int e = 0; e < 6; ++e)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g[4 + e][c] = g[kEdgeV[e][0]][c] + g[kEdgeV[e][1]][c];
+    const double v = vol[t];
+    const double mass = rho * v;
+    const double muv = inv_mu * v;
+    int slot[10];
+#pragma unroll
+    for (int b = 0; b < 10; ++b) slot[b] = amap[(size_t)k * 10 + b];
+#pragma unroll
+    for (int b = 0; b < 10; ++b) {
+      double m;
+      if (a < 4 && b < 4)
+        m = (a == b) ? mass / 10.0 : mass / 20.0;
+      else if (a >= 4 && b >= 4)
+        m = (a == b) ? mass / 15.0 : mass / 60.0;
+      else
+        m = mass / 30.0;
+      // vertex-edge pairs use grad_edge . grad_vertex in the reference
+      const double kv = (a >= 4 && b < 4) || (a < 4 && b >= 4)
+                            ? muv * dot3(g[a >= 4 ? a : b], g[a >= 4 ? b : a])
+                            : muv * dot3(g[a], g[b]);
+      M[slot[b]] += m;
+      K[slot[b]] += kv;
+    }
+    if (r < nv) {
+      const double contrib = v / 4.0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int ps = pmap[(size_t)k * 4 + b];
+        L[ps] += v * dot3(g[a], g[b]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) G3[(size_t)ps * 3 + c] += contrib * g[b][c];
+      }
+    }
+  }
+}
+
+void assemble_mapped(hipStream_t s, int nu, int nv, const int *incp, const int *inct,
+                     const uint8_t *loc, const int *amap, const int *pmap, const double *vol,
+                     const double *grad, double rho, double inv_mu, double *M, double *K,
+                     double *L, double *G3) {
+  if (nu <= 0) return;
+  hipLaunchKernelGGL(k_assemble_mapped, dim3(div_up(nu, kBlock)), dim3(kBlock), 0, s, nu, nv,
+                     incp, inct, loc, amap, pmap, vol, grad, rho, inv_mu, M, K, L, G3);
+  CFSA_LAUNCH_CHECK();
+}
+
 void assemble_rows(hipStream_t s, int nu, const int *incp, const int *inct, const int *tets,
                    const int *ten, int nv, const double *vol, const double *grad, double rho,
                    double inv_mu, const DevCsr &A, double *M, double *K, const DevCsr &P,
@@ -1387,6 +1489,56 @@ void assemble_colour(hipStream_t s, int ntc, const int *order, const int *tets,
   hipLaunchKernelGGL(k_assemble, dim3(div_up((int64_t)ntc * 10, kBlock)), dim3(kBlock), 0,
                      s, ntc, order, tets, ten, nv, vol, grad, rho, inv_mu, A.rowptr, A.col,
                      M, K, P.rowptr, P.col, L, G3);
+  CFSA_LAUNCH_CHECK();
+}
+
+// Tet volumes and P1 gradients from the vertex coordinates, the same
+// expressions in the same order as the host mesh layer (Mesh::tetGeometry;
+// reference Mesh.cpp:455 and computeShapeFunctionGradients, :590-614), so a
+// moved mesh is re-measured on the GPU with identical bits.
+struct DV3 {
+  double x, y, z;
+};
+__device__ __forceinline__ DV3 dsub(DV3 a, DV3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ DV3 dcross(DV3 l, DV3 r) {
+  return {l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x};
+}
+__device__ __forceinline__ double ddot(DV3 a, DV3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ DV3 dscale(double s, DV3 a) { return {s * a.x, s * a.y, s * a.z}; }
+
+__global__ __launch_bounds__(kBlock) void k_tet_geometry(int nt, const double *__restrict__ xyz,
+                                                         const int *__restrict__ tets,
+                                                         double *__restrict__ vol,
+                                                         double *__restrict__ grad) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nt) return;
+  DV3 p[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int v = tets[(size_t)t * 4 + i];
+    p[i] = {xyz[(size_t)v * 3], xyz[(size_t)v * 3 + 1], xyz[(size_t)v * 3 + 2]};
+  }
+  vol[t] = fabs(ddot(dcross(dsub(p[1], p[0]), dsub(p[2], p[0])), dsub(p[3], p[0])) / 6.0);
+  const DV3 e1 = dsub(p[1], p[0]), e2 = dsub(p[2], p[0]), e3 = dsub(p[3], p[0]);
+  const double sixv = ddot(e1, dcross(e2, e3));
+  const double inv = 1.0 / sixv;
+  const DV3 g[4] = {dscale(inv, dcross(dsub(p[3], p[1]), dsub(p[2], p[1]))),
+                    dscale(inv, dcross(dsub(p[2], p[0]), dsub(p[3], p[0]))),
+                    dscale(inv, dcross(dsub(p[3], p[0]), dsub(p[1], p[0]))),
+                    dscale(inv, dcross(dsub(p[1], p[0]), dsub(p[2], p[0])))};
+  double *o = grad + (size_t)t * 12;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    o[i * 3] = g[i].x;
+    o[i * 3 + 1] = g[i].y;
+    o[i * 3 + 2] = g[i].z;
+  }
+}
+void tet_geometry(hipStream_t s, int nt, const double *xyz, const int *tets, double *vol,
+                  double *grad) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_tet_geometry, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, s, nt, xyz,
+                     tets, vol, grad);
   CFSA_LAUNCH_CHECK();
 }
 

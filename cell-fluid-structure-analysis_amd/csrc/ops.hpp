@@ -140,6 +140,15 @@ void assemble_rows(hipStream_t s, int nu, const int *incp, const int *inct, cons
                    const int *ten, int nv, const double *vol, const double *grad, double rho,
                    double inv_mu, const DevCsr &A, double *M, double *K, const DevCsr &P,
                    double *L, double *G3);
+void build_slot_map(hipStream_t s, int nu, const int *incp, const int *inct, const int *tets,
+                    const int *ten, int nv, const DevCsr &A, const DevCsr &P, uint8_t *loc,
+                    int *amap, int *pmap);
+void assemble_mapped(hipStream_t s, int nu, int nv, const int *incp, const int *inct,
+                     const uint8_t *loc, const int *amap, const int *pmap, const double *vol,
+                     const double *grad, double rho, double inv_mu, double *M, double *K,
+                     double *L, double *G3);
+void tet_geometry(hipStream_t s, int nt, const double *xyz, const int *tets, double *vol,
+                  double *grad);
 void lumped_mass_inverse(hipStream_t s, const DevCsr &A, const double *M, double *ilm);
 void build_lhs(hipStream_t s, int64_t nnz, const double *M, const double *K, double dt,
                double *lhs);

@@ -118,6 +118,9 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   d_grad.upload(grad.data(), grad.size());
   d_bc.upload(h_bc.data(), h_bc.size());
   d_order.upload(C.order.data(), C.order.size());
+  d_amap.release();  // the row-gather slot map belongs to the previous patterns
+  d_pmap.release();
+  d_loc.release();
   d_incp.upload(inc.ptr.data(), inc.ptr.size());
   d_inct.upload(inc.tet.data(), inc.tet.size());
   d_Arp.upload(PA.rowptr.data(), PA.rowptr.size());
@@ -252,11 +255,11 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
 void NSSolver::update_mesh(const double *coords, const uint8_t *p1_bc, const uint8_t *p2_bc) {
   if (!initialized) throw std::runtime_error("update_mesh before initialize");
   if (coords) {
+    // moved vertices: re-measure the tets on the GPU (identical arithmetic to
+    // the host mesh layer)
     h_coords.assign(coords, coords + (size_t)nv * 3);
-    std::vector<double> vol((size_t)nt), grad((size_t)nt * 12);
-    Mesh::tetGeometry(h_coords.data(), h_tets.data(), nt, vol.data(), grad.data());
-    d_vol.upload(vol.data(), vol.size());
-    d_grad.upload(grad.data(), grad.size());
+    d_xyz.upload(h_coords.data(), h_coords.size());
+    tet_geometry(stream_, nt, d_xyz.p, d_tets.p, d_vol.p, d_grad.p);
   }
   if (p1_bc) std::memcpy(h_bc.data(), p1_bc, (size_t)nv);
   if (p2_bc && ne) std::memcpy(h_bc.data() + nv, p2_bc, (size_t)ne);
@@ -274,8 +277,17 @@ void NSSolver::assemble() {
   d_G3.zero(stream_);
   const double inv_mu = 1.0 / viscosity;  // NavierStokes.cpp:153
   if (opt.assembly == 0) {
-    assemble_rows(stream_, nu, d_incp.p, d_inct.p, d_tets.p, d_ten.p, nv, d_vol.p, d_grad.p,
-                  density, inv_mu, A_, d_M.p, d_K.p, P_, d_L.p, d_G3.p);
+    if (d_amap.n == 0) {
+      // slot map of the row gather, built once (the patterns never change)
+      const size_t ninc = (size_t)nt * 10, nvinc = (size_t)nt * 4;
+      d_loc.alloc(ninc);
+      d_amap.alloc(ninc * 10);
+      d_pmap.alloc(nvinc * 4);
+      build_slot_map(stream_, nu, d_incp.p, d_inct.p, d_tets.p, d_ten.p, nv, A_, P_, d_loc.p,
+                     d_amap.p, d_pmap.p);
+    }
+    assemble_mapped(stream_, nu, nv, d_incp.p, d_inct.p, d_loc.p, d_amap.p, d_pmap.p, d_vol.p,
+                    d_grad.p, density, inv_mu, d_M.p, d_K.p, d_L.p, d_G3.p);
   } else {
     for (int c = 0; c < ncolours; ++c) {
       const int a = h_colour_offsets[(size_t)c], b = h_colour_offsets[(size_t)c + 1];

@@ -197,7 +197,9 @@ class NSSolver {
   hipStream_t stream_ = nullptr;
   // mesh on device
   DBuf<int> d_tets, d_ten, d_order, d_incp, d_inct;
-  DBuf<double> d_vol, d_grad;
+  DBuf<uint8_t> d_loc;       // row-gather slot map (assembly 0): local index per incidence
+  DBuf<int> d_amap, d_pmap;  // P2 slots [10 nt][10], P1 slots of vertex rows [4 nt][4]
+  DBuf<double> d_vol, d_grad, d_xyz;
   DBuf<uint8_t> d_bc;
   std::vector<int32_t> h_colour_offsets, h_order;
   std::vector<uint8_t> h_bc;
