@@ -117,6 +117,45 @@ __device__ void reduce_final(const double *__restrict__ partials, int G, double 
   __syncthreads();
 }
 
+// Every block of a grid reduces the same NV pair arrays of length G in the
+// same fixed order (lane-strided merge, wavefront shuffle tree, then the four
+// wavefronts in order); out[] is identical in every block and thread.
+template <int NV>
+__device__ void block_reduce_pairs(const double *__restrict__ partials, int G, double (&out)[NV]) {
+  constexpr int W = kBlock / 64;
+  __shared__ double wh[NV][W], wl[NV][W];
+  __shared__ double res[NV];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    DD a;
+    dd_zero(a);
+    for (int g = tid; g < G; g += kBlock) {
+      const size_t o = 2 * ((size_t)v * G + g);
+      dd_merge(a, partials[o], partials[o + 1]);
+    }
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) {
+      const double oh = __shfl_down(a.hi, st, 64);
+      const double ol = __shfl_down(a.lo, st, 64);
+      dd_merge(a, oh, ol);
+    }
+    if (lane == 0) {
+      wh[v][wv] = a.hi;
+      wl[v][wv] = a.lo;
+    }
+  }
+  __syncthreads();
+  if (tid < NV) {
+    DD a{wh[tid][0], wl[tid][0]};
+    for (int w = 1; w < W; ++w) dd_merge(a, wh[tid][w], wl[tid][w]);
+    res[tid] = a.hi + a.lo;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < NV; ++v) out[v] = res[v];
+}
+
 // out[0..NV) = reduced values of NV pair arrays (one block of kBlock threads).
 template <int NV>
 __global__ void k_reduce_pairs(const double *__restrict__ partials, int G, double *out) {

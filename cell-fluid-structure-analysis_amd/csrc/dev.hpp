@@ -55,6 +55,10 @@ struct KrylovState {
   double rho[3], rho_old[3], alpha[3], omega[3], beta[3];
   double r0_sq[3], rhs_sq[3], tol2[3], rr[3], err[3];
   double r0v[3], tt[3], ts[3];  // last reduced inner products (diagnostics)
+  // PCG: rho and iteration count in two slots by iteration parity, so that a
+  // kernel reads one slot while its block 0 writes the other
+  double rho_s[2][3];
+  int it_s[2][3];
   double tol;
   int iters[3], restarts[3], status[3], active[3];
   int maxit, nrhs, any_active, pad;

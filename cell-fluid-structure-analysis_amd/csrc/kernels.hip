@@ -1084,6 +1084,8 @@ __global__ void k_cg_s_init(KrylovState *st, const double *partials, int G) {
       continue;
     }
     st->rho[c] = tot[2 * c + 1];
+    st->rho_s[0][c] = tot[2 * c + 1];
+    st->it_s[0][c] = 0;
     st->status[c] = (st->maxit > 0) ? KS_RUNNING : KS_DONE;
     st->err[c] = 1.0;
     st->active[c] = st->maxit > 0;
@@ -1096,62 +1098,33 @@ __global__ void k_cg_s_init(KrylovState *st, const double *partials, int G) {
   st->any_active = any;
 }
 
-template <int K>
-__global__ void k_cg_s_alpha(KrylovState *st, const double *partials, int G) {
-  __shared__ double tot[K];
-  reduce_final<K>(partials, G, tot);
-  if (threadIdx.x != 0) return;
-  for (int c = 0; c < K; ++c) {
-    if (!st->active[c]) continue;
-    st->r0v[c] = tot[c];
-    st->alpha[c] = st->rho[c] / tot[c];
-  }
-}
-
-template <int K>
-__global__ void k_cg_s_beta(KrylovState *st, const double *partials, int G) {
-  __shared__ double tot[2 * K];
-  reduce_final<2 * K>(partials, G, tot);
-  if (threadIdx.x != 0) return;
-  for (int c = 0; c < K; ++c) {
-    if (!st->active[c]) continue;
-    const double rn2 = tot[2 * c];
-    st->rr[c] = rn2;
-    if (rn2 < st->tol2[c]) {
-      st->status[c] = KS_DONE;
-      st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
-      st->active[c] = 0;
-      continue;
-    }
-    st->rho_old[c] = st->rho[c];
-    st->rho[c] = tot[2 * c + 1];
-    st->beta[c] = st->rho[c] / st->rho_old[c];
-    st->iters[c] += 1;
-    if (st->iters[c] >= st->maxit) {
-      // Eigen still updates the direction before leaving the loop; the
-      // direction is not used afterwards, so the rhs stops here
-      st->status[c] = KS_DONE;
-      st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
-      st->active[c] = 0;
-    }
-  }
-  int any = 0;
-  for (int c = 0; c < K; ++c) any |= st->active[c];
-  st->any_active = any;
-}
-
+// One PCG iteration is SpMV (q = A p, p.q), then k_cg_xr, then k_cg_p.  Each
+// of the two vector kernels reduces the partial pairs it needs in every block
+// (same fixed order, same result), so no single-block scalar kernel sits
+// between them.  Iteration parity `par` selects the rho / count slot read; block
+// 0 writes the other slot, so no state field is both read and written within a
+// kernel.
 template <class T, int K>
-__global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__restrict__ st,
-                                                  const T *__restrict__ p,
+__global__ __launch_bounds__(kBlock) void k_cg_xr(int n, KrylovState *__restrict__ st,
+                                                  const double *__restrict__ pq_partials,
+                                                  int Gq, int par, const T *__restrict__ p,
                                                   const T *__restrict__ q,
                                                   const T *__restrict__ invd,
                                                   T *__restrict__ x, T *__restrict__ r,
                                                   int scaled, double *__restrict__ partials) {
   if (!st->any_active) return;  // grid-uniform: the partials are not read then
+  double pq[K];
+  block_reduce_pairs<K>(pq_partials, Gq, pq);
+  if (blockIdx.x == 0 && threadIdx.x < K && st->active[threadIdx.x]) {
+    const int c = threadIdx.x;
+    st->r0v[c] = pq[c];
+    st->alpha[c] = st->rho_s[par][c] / pq[c];
+  }
   const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
   const int c = (int)(i0 % K);
   const bool act = st->active[c] != 0;
-  const double al = st->alpha[c];
+  // alpha = rho / p.Ap (Eigen conjugate_gradient)
+  const double al = act ? st->rho_s[par][c] / pq[c] : 0.0;
   DD acc[2];
   dd_zero(acc[0]);
   dd_zero(acc[1]);
@@ -1168,17 +1141,77 @@ __global__ __launch_bounds__(kBlock) void k_cg_v1(int n, const KrylovState *__re
   block_partials_binned<K, 2>(acc, partials);
 }
 
-// p = z + beta p with z = M^-1 r recomputed from r (same rounding as in k_cg_v1)
+// Convergence test on |r|^2 (or |M^-1 r|^2), then beta = rho_new / rho and
+// p = z + beta p with z = M^-1 r recomputed from r (same rounding as in
+// k_cg_xr).  A rhs that converges or reaches maxit leaves the loop here.
 template <class T, int K>
-__global__ __launch_bounds__(kBlock) void k_cg_v2(int n, const KrylovState *__restrict__ st,
-                                                  const T *__restrict__ r,
-                                                  const T *__restrict__ invd,
-                                                  T *__restrict__ p) {
-  if (!st->any_active) return;
+__global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict__ st,
+                                                 const double *__restrict__ partials, int Gv,
+                                                 int par, const T *__restrict__ r,
+                                                 const T *__restrict__ invd,
+                                                 T *__restrict__ p) {
+  // Block 0 of this grid clears any_active when no rhs continues; a block
+  // that reads the cleared flag has nothing to do, but the flag is read once
+  // per block so that the block leaves (or stays) as a whole.
+  __shared__ int go;
+  if (threadIdx.x == 0) go = st->any_active;
+  __syncthreads();
+  if (!go) return;
+  double tot[2 * K];
+  block_reduce_pairs<2 * K>(partials, Gv, tot);
+  __shared__ double beta_s[K];
+  __shared__ int cont_s[K];
+  if (threadIdx.x == 0) {
+    const bool w = blockIdx.x == 0;  // the one block that writes the state
+    int any = 0;
+    for (int c = 0; c < K; ++c) {
+      int cont = 0;
+      double beta = 0.0;
+      if (st->active[c]) {
+        const double rn2 = tot[2 * c];
+        if (rn2 < st->tol2[c]) {
+          if (w) {
+            st->rr[c] = rn2;
+            st->status[c] = KS_DONE;
+            st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
+            st->active[c] = 0;
+          }
+        } else {
+          const double rz = tot[2 * c + 1];
+          const int it = st->it_s[par][c] + 1;
+          beta = rz / st->rho_s[par][c];
+          if (w) {
+            st->rr[c] = rn2;
+            st->rho_old[c] = st->rho_s[par][c];
+            st->rho[c] = rz;
+            st->beta[c] = beta;
+            st->rho_s[1 - par][c] = rz;
+            st->it_s[1 - par][c] = it;
+            st->iters[c] = it;
+          }
+          if (it >= st->maxit) {
+            // Eigen updates the direction before leaving; it is unused after
+            if (w) {
+              st->status[c] = KS_DONE;
+              st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
+              st->active[c] = 0;
+            }
+          } else {
+            cont = 1;
+            any = 1;
+          }
+        }
+      }
+      beta_s[c] = beta;
+      cont_s[c] = cont;
+    }
+    if (w) st->any_active = any;
+  }
+  __syncthreads();
   const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
   const int c = (int)(i0 % K);
-  if (!st->active[c]) return;
-  const double be = st->beta[c];
+  if (!cont_s[c]) return;
+  const double be = beta_s[c];
   const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
   for (size_t i = i0; i < N; i += stride) {
     const T zi = (T)((double)invd[i / K] * (double)r[i]);
@@ -1200,47 +1233,36 @@ void cg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G) {
   hipLaunchKernelGGL((k_cg_s_init<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G);
   CFSA_LAUNCH_CHECK();
 }
-template <int K>
-void cg_s_alpha(hipStream_t s, KrylovState *st, const double *partials, int G) {
-  hipLaunchKernelGGL((k_cg_s_alpha<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G);
-  CFSA_LAUNCH_CHECK();
-}
-template <int K>
-void cg_s_beta(hipStream_t s, KrylovState *st, const double *partials, int G) {
-  hipLaunchKernelGGL((k_cg_s_beta<K>), dim3(1), dim3(kBlock), 0, s, st, partials, G);
-  CFSA_LAUNCH_CHECK();
-}
 template <class T, int K>
-void cg_v1(hipStream_t s, int n, const KrylovState *st, const T *p, const T *q,
-           const T *invd, T *x, T *r, int scaled, double *partials, int *grid) {
+void cg_xr(hipStream_t s, int n, KrylovState *st, const double *pq_partials, int Gq, int par,
+           const T *p, const T *q, const T *invd, T *x, T *r, int scaled, double *partials,
+           int *grid) {
   const int G = vec_grid_k(n, K);
   *grid = G;
-  hipLaunchKernelGGL((k_cg_v1<T, K>), dim3(G), dim3(kBlock), 0, s, n, st, p, q, invd, x, r,
-                     scaled, partials);
+  hipLaunchKernelGGL((k_cg_xr<T, K>), dim3(G), dim3(kBlock), 0, s, n, st, pq_partials, Gq, par,
+                     p, q, invd, x, r, scaled, partials);
   CFSA_LAUNCH_CHECK();
 }
 template <class T, int K>
-void cg_v2(hipStream_t s, int n, const KrylovState *st, const T *r, const T *invd, T *p) {
-  hipLaunchKernelGGL((k_cg_v2<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st, r,
-                     invd, p);
+void cg_p(hipStream_t s, int n, KrylovState *st, const double *partials, int Gv, int par,
+          const T *r, const T *invd, T *p) {
+  hipLaunchKernelGGL((k_cg_p<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st, partials,
+                     Gv, par, r, invd, p);
   CFSA_LAUNCH_CHECK();
 }
-#define CFSA_INST_CG(T, K)                                                                \
-  template void cg_init<T, K>(hipStream_t, int, const T *, const T *, T *, T *, T *, int,   \
-                              double *, int *);                                           \
-  template void cg_v1<T, K>(hipStream_t, int, const KrylovState *, const T *, const T *,  \
-                            const T *, T *, T *, int, double *, int *);                   \
-  template void cg_v2<T, K>(hipStream_t, int, const KrylovState *, const T *, const T *, T *);
+#define CFSA_INST_CG(T, K)                                                                   \
+  template void cg_init<T, K>(hipStream_t, int, const T *, const T *, T *, T *, T *, int,      \
+                              double *, int *);                                              \
+  template void cg_xr<T, K>(hipStream_t, int, KrylovState *, const double *, int, int,       \
+                            const T *, const T *, const T *, T *, T *, int, double *, int *); \
+  template void cg_p<T, K>(hipStream_t, int, KrylovState *, const double *, int, int,        \
+                           const T *, const T *, T *);
 CFSA_INST_CG(double, 1)
 CFSA_INST_CG(double, 3)
 CFSA_INST_CG(float, 1)
 CFSA_INST_CG(float, 3)
-#define CFSA_INST_CG_S(K)                                                        \
-  template void cg_s_init<K>(hipStream_t, KrylovState *, const double *, int);   \
-  template void cg_s_alpha<K>(hipStream_t, KrylovState *, const double *, int);  \
-  template void cg_s_beta<K>(hipStream_t, KrylovState *, const double *, int);
-CFSA_INST_CG_S(1)
-CFSA_INST_CG_S(3)
+template void cg_s_init<1>(hipStream_t, KrylovState *, const double *, int);
+template void cg_s_init<3>(hipStream_t, KrylovState *, const double *, int);
 
 // ===========================================================================
 // Element assembly into the fixed patterns (graph-coloured, atomic-free)

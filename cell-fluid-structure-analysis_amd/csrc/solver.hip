@@ -577,7 +577,9 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
   int Gv = 0, Gb = 0;
   cg_init<T, K>(s, n, b, invd, x, w.r.p, w.p.p, opt.stop_scaled, d_part_a.p, &Gv);
   cg_s_init<K>(s, st, d_part_a.p, Gv);
-  const int chunk = std::max(1, K == 1 ? opt.check_every_1 : opt.check_every);
+  // even, so that every chunk starts on parity slot 0
+  int chunk = std::max(1, K == 1 ? opt.check_every_1 : opt.check_every);
+  chunk += chunk & 1;
   const long max_chunks = (long)maxit / chunk + 16;
   auto body = [&](bool timed) {
     for (int j = 0; j < chunk; ++j) {
@@ -588,10 +590,10 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
       else
         spmv1_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
       if (ev) spmv_event_end();
-      cg_s_alpha<K>(s, st, d_part_b.p, Gb);
-      cg_v1<T, K>(s, n, st, w.p.p, w.v.p, invd, x, w.r.p, opt.stop_scaled, d_part_a.p, &Gv);
-      cg_s_beta<K>(s, st, d_part_a.p, Gv);
-      cg_v2<T, K>(s, n, st, w.r.p, invd, w.p.p);
+      const int par = j & 1;
+      cg_xr<T, K>(s, n, st, d_part_b.p, Gb, par, w.p.p, w.v.p, invd, x, w.r.p, opt.stop_scaled,
+                  d_part_a.p, &Gv);
+      cg_p<T, K>(s, n, st, d_part_a.p, Gv, par, w.r.p, invd, w.p.p);
     }
   };
   const std::vector<uintptr_t> key = {
