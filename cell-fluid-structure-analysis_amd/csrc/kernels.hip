@@ -20,10 +20,12 @@ namespace cfsa {
 // CSR-stream core
 // ===========================================================================
 // Rows are cut into windows (host-built, see stream_windows) of at most
-// kStreamB<KO> nonzeros, one block per window.  The block first loads the
-// window's columns and values (clamped indices, so every load is issued
-// before any is used), then gathers x and stages the products in LDS, then
-// sums them per (row, component), kBlock / KO rows at a time.
+// kStreamB<KO> nonzeros and kStreamRows rows, one block per window.  The block
+// first loads the window's columns and values (clamped indices, so every load
+// is issued before any is used) and its rowptr slice, then gathers x and
+// stages the products in LDS, then sums them per (row, component), kBlock / KO
+// rows at a time.  Nothing is read from global memory between the barrier and
+// the sums except the epilogue operands of rows past the first pass.
 //
 // Inner products: each block reduces its rows' pairs and stores them in
 // wpart[v][window]; k_window_reduce then merges each group of kStreamGroup
@@ -39,10 +41,12 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(const int *__restrict__ r
   constexpr int B = kStreamB<T, KO>;
   constexpr int R = kBlock / KO;   // rows summed per pass
   constexpr int PER = B / kBlock;  // staged entries per thread
+  static_assert(PER * kBlock == B, "whole staging rows per thread");
   constexpr int NDs = ND > 0 ? ND : 1;
   constexpr int NV = KO * ND;
   // staged products; reused for the block's inner-product pairs at the end
   __shared__ T lds[B * KO];
+  __shared__ int rps[kBlock];  // rowptr slice of the window, relative to k0
   static_assert(sizeof(T) * B * KO >= sizeof(double) * 2 * NDs * kBlock, "LDS reuse");
   if (!op.any()) return;
   DD dots[NDs];
@@ -52,9 +56,10 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(const int *__restrict__ r
   const int myc = tid % KO;
   const bool act = op.active(myc) && tid < KO * R;
   const int w = blockIdx.x;
-  const int r0 = win[w], r1 = win[w + 1];
+  const int r0 = win[w], r1 = win[w + 1];  // r1 - r0 <= kStreamRows
   const int k0 = rowptr[r0];
   const int len = rowptr[r1] - k0;  // <= B by construction of the windows
+  const int rpv = rowptr[min(r0 + tid, r1)];
   const int row0 = r0 + tid / KO;
   typename Op::Pre pre0{};
   if (act && row0 < r1) pre0 = op.pre(row0, myc);
@@ -65,20 +70,21 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(const int *__restrict__ r
     T p[PER][KO];
 #pragma unroll
     for (int u = 0; u < PER; ++u) op.prod(f[u], p[u]);
+    // every slot i < B is written (PER * kBlock == B); slots past len hold
+    // copies of the last product and are never summed
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int i = tid + u * kBlock;
-      if (i < len) {
 #pragma unroll
-        for (int c = 0; c < KO; ++c) lds[i * KO + c] = p[u][c];
-      }
+      for (int c = 0; c < KO; ++c) lds[i * KO + c] = p[u][c];
     }
   }
+  rps[tid] = rpv - k0;
   __syncthreads();
   for (int row = row0; row < r1; row += R) {
     if (act) {
       const typename Op::Pre pr = row == row0 ? pre0 : op.pre(row, myc);
-      const int ra = rowptr[row] - k0, rb = rowptr[row + 1] - k0;
+      const int ra = rps[row - r0], rb = rps[row - r0 + 1];
       // sequential sum in ascending column order; LDS reads are batched
       T acc = T(0);
       int k = ra;

@@ -33,6 +33,9 @@ struct DevCsr {
 // 1536 (FP32, 18 KiB); one component 2048.
 template <class T, int KO>
 constexpr int kStreamB = KO == 3 ? (sizeof(T) == 4 ? 1536 : 1024) : 2048;
+// rows per CSR-stream window: the window's rowptr slice is staged in LDS, one
+// entry per thread of the block
+constexpr int kStreamRows = kBlock - 1;
 
 // out[e] = src[e] >= 0 ? (T)in[src[e]] : 0 over the stored entries
 template <class T>

@@ -31,14 +31,14 @@ void parallel_chunks(int64_t n, F &&f) {
 
 }  // namespace
 
-std::vector<int32_t> stream_windows(const CsrPattern &P, int max_nnz) {
+std::vector<int32_t> stream_windows(const CsrPattern &P, int max_nnz, int max_rows) {
   std::vector<int32_t> w{0};
   const int64_t n = P.n;
   int64_t r = 0;
   while (r < n) {
     const int32_t k0 = P.rowptr[(size_t)r];
     int64_t e = r;
-    while (e < n && P.rowptr[(size_t)e + 1] - k0 <= max_nnz) ++e;
+    while (e < n && e - r < max_rows && P.rowptr[(size_t)e + 1] - k0 <= max_nnz) ++e;
     if (e == r) throw std::runtime_error("CSR row longer than a stream window");
     w.push_back((int32_t)e);
     r = e;

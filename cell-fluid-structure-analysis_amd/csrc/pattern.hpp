@@ -71,7 +71,8 @@ struct SellPattern {
 SellPattern build_sell(const CsrPattern &P);
 
 // CSR-stream windows: consecutive rows packed greedily so that each window
-// holds at most max_nnz nonzeros; returns the nwin + 1 window start rows.
-std::vector<int32_t> stream_windows(const CsrPattern &P, int max_nnz);
+// holds at most max_nnz nonzeros and at most max_rows rows; returns the
+// nwin + 1 window start rows.
+std::vector<int32_t> stream_windows(const CsrPattern &P, int max_nnz, int max_rows);
 
 }  // namespace cfsa

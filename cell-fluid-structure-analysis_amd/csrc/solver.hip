@@ -148,12 +148,12 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   A_ = DevCsr{nu, nnzA, d_Arp.p, d_Acol.p};
   P_ = DevCsr{nv, nnzP, d_Prp.p, d_Pcol.p};
   {
-    const std::vector<int32_t> a1 = stream_windows(PA, kStreamB<double, 1>);
-    const std::vector<int32_t> a3 = stream_windows(PA, kStreamB<double, 3>);
-    const std::vector<int32_t> a3f = stream_windows(PA, kStreamB<float, 3>);
-    const std::vector<int32_t> p1 = stream_windows(PP, kStreamB<double, 1>);
-    const std::vector<int32_t> p3 = stream_windows(PP, kStreamB<double, 3>);
-    const std::vector<int32_t> p3f = stream_windows(PP, kStreamB<float, 3>);
+    const std::vector<int32_t> a1 = stream_windows(PA, kStreamB<double, 1>, kStreamRows);
+    const std::vector<int32_t> a3 = stream_windows(PA, kStreamB<double, 3>, kStreamRows);
+    const std::vector<int32_t> a3f = stream_windows(PA, kStreamB<float, 3>, kStreamRows);
+    const std::vector<int32_t> p1 = stream_windows(PP, kStreamB<double, 1>, kStreamRows);
+    const std::vector<int32_t> p3 = stream_windows(PP, kStreamB<double, 3>, kStreamRows);
+    const std::vector<int32_t> p3f = stream_windows(PP, kStreamB<float, 3>, kStreamRows);
     d_Aw1.upload(a1.data(), a1.size());
     d_Aw3.upload(a3.data(), a3.size());
     d_Aw3f.upload(a3f.data(), a3f.size());
