@@ -1147,15 +1147,69 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(int n, KrylovState *__restrict
   block_partials_binned<K, 2>(acc, partials);
 }
 
-// Convergence test on |r|^2 (or |M^-1 r|^2), then beta = rho_new / rho and
-// p = z + beta p with z = M^-1 r recomputed from r (same rounding as in
-// k_cg_xr).  A rhs that converges or reaches maxit leaves the loop here.
+// Convergence test of one PCG iteration for each rhs, on |r|^2 (or
+// |M^-1 r|^2) = tot[2c], rho_new = tot[2c+1]: a rhs that converges or reaches
+// maxit leaves the loop, the others get beta = rho_new / rho.  The state is
+// written only when `w`; cont[c] and beta[c] return the decision.
+template <int K>
+__device__ void cg_decide(KrylovState *__restrict__ st, const double *tot, int par, bool w,
+                          double *beta_out, int *cont_out) {
+  int any = 0;
+  for (int c = 0; c < K; ++c) {
+    int cont = 0;
+    double beta = 0.0;
+    if (st->active[c]) {
+      const double rn2 = tot[2 * c];
+      if (rn2 < st->tol2[c]) {
+        if (w) {
+          st->rr[c] = rn2;
+          st->status[c] = KS_DONE;
+          st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
+          st->active[c] = 0;
+        }
+      } else {
+        const double rz = tot[2 * c + 1];
+        const int it = st->it_s[par][c] + 1;
+        beta = rz / st->rho_s[par][c];
+        if (w) {
+          st->rr[c] = rn2;
+          st->rho_old[c] = st->rho_s[par][c];
+          st->rho[c] = rz;
+          st->beta[c] = beta;
+          st->rho_s[1 - par][c] = rz;
+          st->it_s[1 - par][c] = it;
+          st->iters[c] = it;
+        }
+        if (it >= st->maxit) {
+          // Eigen updates the direction before leaving; it is unused after
+          if (w) {
+            st->status[c] = KS_DONE;
+            st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
+            st->active[c] = 0;
+          }
+        } else {
+          cont = 1;
+          any = 1;
+        }
+      }
+    }
+    beta_out[c] = beta;
+    cont_out[c] = cont;
+  }
+  if (w) st->any_active = any;
+}
+
+// Every block reduces k_cg_xr's pairs and decides (block 0 writes the state),
+// then p = z + beta p with z = M^-1 r recomputed from r (same rounding as in
+// k_cg_xr) for the rhs that continue.
 template <class T, int K>
 __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict__ st,
                                                  const double *__restrict__ partials, int Gv,
                                                  int par, const T *__restrict__ r,
                                                  const T *__restrict__ invd,
                                                  T *__restrict__ p) {
+  __shared__ double beta_s[K];
+  __shared__ int cont_s[K];
   // Block 0 of this grid clears any_active when no rhs continues; a block
   // that reads the cleared flag has nothing to do, but the flag is read once
   // per block so that the block leaves (or stays) as a whole.
@@ -1165,54 +1219,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict_
   if (!go) return;
   double tot[2 * K];
   block_reduce_pairs<2 * K>(partials, Gv, tot);
-  __shared__ double beta_s[K];
-  __shared__ int cont_s[K];
-  if (threadIdx.x == 0) {
-    const bool w = blockIdx.x == 0;  // the one block that writes the state
-    int any = 0;
-    for (int c = 0; c < K; ++c) {
-      int cont = 0;
-      double beta = 0.0;
-      if (st->active[c]) {
-        const double rn2 = tot[2 * c];
-        if (rn2 < st->tol2[c]) {
-          if (w) {
-            st->rr[c] = rn2;
-            st->status[c] = KS_DONE;
-            st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
-            st->active[c] = 0;
-          }
-        } else {
-          const double rz = tot[2 * c + 1];
-          const int it = st->it_s[par][c] + 1;
-          beta = rz / st->rho_s[par][c];
-          if (w) {
-            st->rr[c] = rn2;
-            st->rho_old[c] = st->rho_s[par][c];
-            st->rho[c] = rz;
-            st->beta[c] = beta;
-            st->rho_s[1 - par][c] = rz;
-            st->it_s[1 - par][c] = it;
-            st->iters[c] = it;
-          }
-          if (it >= st->maxit) {
-            // Eigen updates the direction before leaving; it is unused after
-            if (w) {
-              st->status[c] = KS_DONE;
-              st->err[c] = sqrt(rn2 / st->rhs_sq[c]);
-              st->active[c] = 0;
-            }
-          } else {
-            cont = 1;
-            any = 1;
-          }
-        }
-      }
-      beta_s[c] = beta;
-      cont_s[c] = cont;
-    }
-    if (w) st->any_active = any;
-  }
+  if (threadIdx.x == 0) cg_decide<K>(st, tot, par, blockIdx.x == 0, beta_s, cont_s);
   __syncthreads();
   const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
   const int c = (int)(i0 % K);
