@@ -134,16 +134,128 @@ HEADLINE = {"mixed_bicgstab_all", "mixed_bicgstab", "mixed_cg"}
 VEL_ERR_MAX = 1e-7
 
 
+def cylinder_mesh(P):
+    """The README cylinder (length 30, radius 2, 35 divisions, max element 0.03)
+    from the library's own mesher; x is the flow axis."""
+    mesh = P.Mesh.cylinder(30.0, 2.0, 35, 0.03)
+    mesh.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+    return mesh
+
+
+def fourier_inlet_basis(X, R):
+    """u_x basis on the inlet disc: (1 - r^2/R^2) times 1, cos t, sin t, cos 2t."""
+    y, z = X[:, 1], X[:, 2]
+    base = np.clip(1.0 - (y * y + z * z) / R ** 2, 0.0, None)
+    th = np.arctan2(z, y)
+    B = np.zeros((4, len(X), 3))
+    for k, f in enumerate([np.ones_like(th), np.cos(th), np.sin(th), np.cos(2 * th)]):
+        B[k, :, 0] = base * f
+    return B
+
+
+def adjoint_workload(P, args, torch):
+    """Gradient of J = 1/2 sum_Outlet |u|^2 after `steps` PISO steps from rest on the
+    cylinder, w.r.t. 4 inlet Fourier coefficients c ~ 0.5 + N(0, 0.05)
+    (numpy default_rng(0)) and the viscosity, by the library's hand-written
+    adjoint (per-step checkpoints, u* and p' recomputed); every component is
+    checked against a central difference with h = 1e-6 of the GPU forward."""
+    R = 2.0
+    t0 = time.perf_counter()
+    mesh = cylinder_mesh(P)
+    X = np.vstack([mesh.vertices(), mesh.edgeNodes()])
+    B = fourier_inlet_basis(X, R)
+    c = 0.5 + np.random.default_rng(0).normal(0.0, 0.05, 4)
+    dt = 0.01 if args.dt is None else args.dt
+    ncorr = 2 if args.corrections is None else args.corrections
+    tol = args.adjoint_tol
+    g = P.NavierStokesSolver()
+    g.initialize(mesh, args.rho, args.mu)
+    g.setDt(dt)
+    g.setNumberOfCorrections(ncorr)
+    g.setOptions(momentum_tol=tol, pressure_tol=tol, stop_rule=1, max_iter=20000,
+                 pressure_solver=args.pressure_solver)
+    sz = g.sizes()
+    log(f"adjoint workload: mesh {mesh.sizes()}, solver ready in {time.perf_counter() - t0:.1f}s")
+    lab = np.r_[mesh.p1BC(), mesh.p2BC()]
+    outlet = lab == 2
+
+    def J_of(coef):
+        u = g.adjointForward(args.steps, np.tensordot(coef, B, axes=1))
+        return 0.5 * float((u[outlet] ** 2).sum())
+
+    g.adjointGradient(args.steps, B, c)  # warmup
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        J, dc, dmu = g.adjointGradient(args.steps, B, c)
+    torch.cuda.synchronize()
+    t_grad = (time.perf_counter() - t0) / reps
+    log(f"adjoint gradient {1e3 * t_grad:.1f} ms")
+    t0 = time.perf_counter()
+    J_fwd = J_of(c)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    h = 1e-6
+    fd = []
+    for k in range(4):
+        cp, cm = c.copy(), c.copy()
+        cp[k] += h
+        cm[k] -= h
+        fd.append((J_of(cp) - J_of(cm)) / (2 * h))
+        log(f"fd c{k}: {fd[-1]:.9e} vs adjoint {dc[k]:.9e}")
+    g.setViscosity(args.mu + h)
+    Jp = J_of(c)
+    g.setViscosity(args.mu - h)
+    Jm = J_of(c)
+    g.setViscosity(args.mu)
+    fdmu = (Jp - Jm) / (2 * h)
+    # five-point stencil with H = 10 h: truncation O(H^4) (about 4e-11 relative
+    # here) and 10x less solver noise than the h = 1e-6 quotient, whose signal
+    # J(mu + h) - J(mu - h) is only ~1e-7 of J
+    H = 10 * h
+    Jv = {}
+    for m in (-2, -1, 1, 2):
+        g.setViscosity(args.mu + m * H)
+        Jv[m] = J_of(c)
+    g.setViscosity(args.mu)
+    fdmu5 = (Jv[-2] - 8 * Jv[-1] + 8 * Jv[1] - Jv[2]) / (12 * H)
+    log(f"fd mu: central h {fdmu:.9e}, five-point 10h {fdmu5:.9e}, adjoint {dmu:.9e}")
+    scale = float(np.abs(dc).max())
+    err_c = [abs(f - d) / scale for f, d in zip(fd, dc)]
+    err_mu = abs(fdmu - dmu) / abs(dmu)
+    log(f"J {J:.12e} (forward only {J_fwd:.12e}); dJ/dc {dc}; fd {fd}; dJ/dmu {dmu:.6e} fd {fdmu:.6e}")
+    out = {"metric": "adjoint gradient of the outlet kinetic energy (4 inlet coefficients + "
+                     "viscosity), checked against central differences",
+           "value": 1.0 / t_grad, "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": 1, "ms_per_gradient": 1e3 * t_grad, "ms_forward_only": 1e3 * t_fwd,
+           "higher_is_better": True, "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "adjoint_cylinder_L30_R2_div35_a0.03",
+                      "tets": sz["nt"], "velocity_dofs": 3 * sz["nu"], "pressure_dofs": sz["nv"],
+                      "piso_steps": args.steps, "corrections": ncorr, "dt": dt,
+                      "density": args.rho, "viscosity": args.mu, "coefficients": c.tolist(),
+                      "solver_tol": tol, "stop_rule": "jacobi_scaled", "fd_step": h},
+           "J": J, "J_forward_only": J_fwd, "dJ_dcoef": dc.tolist(), "dJ_dviscosity": dmu,
+           "fd_dJ_dcoef": fd, "fd_dJ_dviscosity": fdmu, "fd5_dJ_dviscosity_10h": fdmu5,
+           "fd5_error_viscosity_rel": abs(fdmu5 - dmu) / abs(dmu),
+           "fd_error_coef_normwise": err_c, "fd_error_viscosity_rel": err_mu,
+           "fd_bound": 1e-5, "fd_ok": bool(max(err_c) <= 1e-5 and err_mu <= 1e-5)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None, help="20 (kuhn) / 50 (cyl2) / 10 (adjoint)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=96, help="cells per side of the unit cube")
-    ap.add_argument("--workload", choices=["kuhn", "cyl2"], default="kuhn",
+    ap.add_argument("--workload", choices=["kuhn", "cyl2", "adjoint"], default="kuhn",
                     help="kuhn: unit-cube Kuhn channel (default); cyl2: the README cylinder "
                          "(length 30, radius 2, 35 divisions, max element 0.03; parabolic "
-                         "inlet peak 0.5, dt 0.01, 2 correctors)")
+                         "inlet peak 0.5, dt 0.01, 2 correctors, 50 steps); adjoint: "
+                         "gradient of the outlet kinetic energy after 10 PISO steps on the "
+                         "cylinder w.r.t. 4 inlet Fourier coefficients and the viscosity, "
+                         "checked against central differences")
     ap.add_argument("--rho", type=float, default=1060.0)
     ap.add_argument("--mu", type=float, default=0.004)
     ap.add_argument("--re", type=float, default=100.0)
@@ -159,6 +271,8 @@ def main():
                     help="0: |r| <= tol |b| (reference; its 1e10 inlet-penalty rows dominate "
                          "|b|, so the interior is not solved to tol), 1: Jacobi-scaled")
     ap.add_argument("--pressure-solver", type=int, default=1, help="0 BiCGSTAB, 1 Jacobi CG")
+    ap.add_argument("--adjoint-tol", type=float, default=1e-14,
+                    help="Krylov tolerance (Jacobi-scaled rule) of the adjoint workload")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--only", default=None, help="run only this mixed variant (profiling)")
@@ -183,6 +297,12 @@ def main():
                           "copy_probe_bytes_written": dst.numel() * 4, "launches": 5}))
         return
 
+    if args.steps is None:
+        args.steps = {"kuhn": 20, "cyl2": 50, "adjoint": 10}[args.workload]
+    if args.workload == "adjoint":
+        adjoint_workload(P, args, torch)
+        return
+
     t0 = time.perf_counter()
     n = args.n
     cyl = args.workload == "cyl2"
@@ -192,8 +312,7 @@ def main():
         args.corrections = 2 if cyl else 1
     if cyl:
         R = 2.0
-        mesh = P.Mesh.cylinder(30.0, R, 35, 0.03)
-        mesh.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+        mesh = cylinder_mesh(P)
         U = 0.5
         X = np.vstack([mesh.vertices(), mesh.edgeNodes()])
         prof = np.zeros((len(X), 3))
