@@ -63,7 +63,8 @@ def make_solver(P, mesh, prof, args, precision, momentum_solver=0, initial_guess
                  precision=precision, momentum_solver=momentum_solver, timing=2,
                  check_every=args.check_every, stop_rule=args.stop_rule,
                  pressure_solver=args.pressure_solver, inner_tol=args.inner_tol,
-                 initial_guess=initial_guess)
+                 initial_guess=initial_guess,
+                 reuse_pressure_correction=0 if args.resolve_correctors else 1)
     return g
 
 
@@ -273,6 +274,10 @@ def main():
     ap.add_argument("--pressure-solver", type=int, default=1, help="0 BiCGSTAB, 1 Jacobi CG")
     ap.add_argument("--adjoint-tol", type=float, default=1e-14,
                     help="Krylov tolerance (Jacobi-scaled rule) of the adjoint workload")
+    ap.add_argument("--resolve-correctors", action="store_true",
+                    help="re-solve the pressure system in every PISO corrector as the reference "
+                         "does, instead of reusing the first corrector's p' (same bits; "
+                         "matters only with 2 or more correctors)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--only", default=None, help="run only this mixed variant (profiling)")
@@ -435,6 +440,9 @@ def main():
                       "stop_rule": "jacobi_scaled" if args.stop_rule else "reference",
                       "pressure_solver": "cg_jacobi" if args.pressure_solver else "bicgstab_jacobi",
                       "fp64_momentum_solver": "bicgstab_jacobi", "headline": primary,
+                      "later_correctors": ("re-solve" if args.resolve_correctors else
+                                           "reuse the first corrector's p' (identical inputs, "
+                                           "same bits)") if args.corrections > 1 else None,
                       "velocity_error_bound": VEL_ERR_MAX, "parallelism": "single GPU"},
            "dofs_per_s": value * (3 * sz["nu"] + sz["nv"]),
            "speedup_vs_fp64": res["fp64"]["ms_per_step"] / res[primary]["ms_per_step"],

@@ -39,7 +39,8 @@ class SolverOptions(C.Structure):
                 ("max_refine", C.c_int), ("check_every", C.c_int), ("timing", C.c_int),
                 ("stop_rule", C.c_int), ("assembly", C.c_int),
                 ("spmv_layout", C.c_int), ("momentum_solver", C.c_int),
-                ("check_every_1", C.c_int), ("graphs", C.c_int), ("initial_guess", C.c_int)]
+                ("check_every_1", C.c_int), ("graphs", C.c_int), ("initial_guess", C.c_int),
+                ("reuse_pressure_correction", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -49,7 +50,8 @@ class StepStats(C.Structure):
                 ("pressure_refinements", C.c_int * 8),
                 ("ms_conv", C.c_double), ("ms_rhs", C.c_double), ("ms_momentum", C.c_double),
                 ("ms_pressure", C.c_double), ("ms_correct", C.c_double),
-                ("ms_total", C.c_double), ("spmv_launches", C.c_int), ("spmv_ms", C.c_double)]
+                ("ms_total", C.c_double), ("spmv_launches", C.c_int), ("spmv_ms", C.c_double),
+                ("pressure_solves", C.c_int)]
 
     def as_dict(self) -> dict:
         n = self.n_pressure
@@ -61,7 +63,7 @@ class StepStats(C.Structure):
                 "pressure_refinements": list(self.pressure_refinements)[:n],
                 "ms_momentum": self.ms_momentum, "ms_pressure": self.ms_pressure,
                 "ms_total": self.ms_total, "spmv_launches": self.spmv_launches,
-                "spmv_ms": self.spmv_ms}
+                "spmv_ms": self.spmv_ms, "pressure_solves": self.pressure_solves}
 
 
 class SolveResult(C.Structure):

@@ -71,6 +71,7 @@ cfsa::SolverOptions to_opts(const cfsa_solver_options &o) {
   s.momentum_solver = o.momentum_solver;
   s.graphs = o.graphs;
   s.initial_guess = o.initial_guess;
+  s.reuse_correction = o.reuse_pressure_correction;
   return s;
 }
 
@@ -458,7 +459,8 @@ int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt) {
       opt->stop_rule < 0 || opt->stop_rule > 1 || opt->assembly < 0 || opt->assembly > 1 ||
       opt->spmv_layout < 0 || opt->spmv_layout > 2 || opt->momentum_solver < 0 ||
       opt->momentum_solver > 1 || opt->graphs < 0 || opt->graphs > 1 ||
-      opt->initial_guess < 0 || opt->initial_guess > 1)
+      opt->initial_guess < 0 || opt->initial_guess > 1 || opt->reuse_pressure_correction < 0 ||
+      opt->reuse_pressure_correction > 1)
     return fail(CFSA_ERR_INVALID, "invalid solver options");
   h->s.opt = to_opts(*opt);
   return CFSA_OK;
@@ -483,6 +485,7 @@ int cfsa_ns_get_options(const cfsa_ns *h, cfsa_solver_options *o) {
   o->momentum_solver = s.momentum_solver;
   o->graphs = s.graphs;
   o->initial_guess = s.initial_guess;
+  o->reuse_pressure_correction = s.reuse_correction;
   return CFSA_OK;
 }
 

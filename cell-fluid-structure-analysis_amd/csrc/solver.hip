@@ -771,6 +771,7 @@ int NSSolver::pressure_poisson() {
   div_gather_prhs(stream_, nv, d_incp.p, d_inct.p, d_wT.p, d_bc.p, mode, reference_node, dt,
                   outlet_pressure, kPressurePenalty, d_div.p, d_rhsp.p);
   SolveResult r = solve_pressure(d_rhsp.p, d_pc.p);
+  last.pressure_solves++;
   const int k = std::min(last.n_pressure, 7);
   last.pressure_iters[k] = r.iters[0];
   last.pressure_err[k] = r.err[0];
@@ -786,6 +787,23 @@ int NSSolver::pressure_poisson() {
                   outlet_pressure);
   pressure_normalize(stream_, nv, d_p.p, reference_node, d_scalar.p);
   return ST_OK;
+}
+
+// A corrector after the first: solvePressurePoisson (NavierStokes.cpp:586-668)
+// would rebuild the same right-hand side from the unchanged u* and solve the
+// same L_bc system from zero, so p' (d_pc) is already its result; only the
+// pressure update, outlet reset and normalization repeat.  correctVelocity
+// (:702-771) would recompute u from the same u* and p', so u is already its
+// result too.  The statistics repeat the first solve's.
+void NSSolver::repeat_pressure_correction() {
+  const int k = std::min(last.n_pressure, 7);
+  last.pressure_iters[k] = last.pressure_iters[0];
+  last.pressure_err[k] = last.pressure_err[0];
+  last.pressure_refinements[k] = last.pressure_refinements[0];
+  last.n_pressure++;
+  pressure_update(stream_, nv, d_p.p, d_pc.p, relax_p, d_bc.p, outlet_type == 0,
+                  outlet_pressure);
+  pressure_normalize(stream_, nv, d_p.p, reference_node, d_scalar.p);
 }
 
 void NSSolver::correct_velocity() {
@@ -831,6 +849,10 @@ int NSSolver::piso_step() {
   if (timing) CFSA_HIP(hipEventRecord(ev[1], stream_));
   if (st == ST_OK) {
     for (int k = 0; k < n_corrections && st == ST_OK; ++k) {
+      if (k > 0 && opt.reuse_correction) {
+        repeat_pressure_correction();
+        continue;
+      }
       st = pressure_poisson();
       if (st == ST_OK) correct_velocity();
     }

@@ -75,6 +75,10 @@ struct SolverOptions {
   int spmv_layout = 0;         // Krylov SpMV: 0 auto, 1 CSR-stream, 2 sliced-ELL
   int graphs = 1;              // replay Krylov iteration chunks as captured HIP graphs
   int initial_guess = 0;       // momentum solve from 0 (reference) or 1: from u^n (mixed)
+  // Correctors after the first re-solve the first one's system (u*, L_bc and the
+  // right-hand side do not change between correctors), so 1 reuses its p'
+  // instead: the same bits, one pressure solve per step.  0 re-solves.
+  int reuse_correction = 1;
 };
 
 struct SolveResult {
@@ -97,6 +101,7 @@ struct StepStats {
   double ms_conv, ms_rhs, ms_momentum, ms_pressure, ms_correct, ms_total;
   int spmv_launches;
   double spmv_ms;  // summed event time of momentum-matrix SpMV launches
+  int pressure_solves;  // pressure solves run (correctors that reused p' excluded)
 };
 
 struct MeshInput {
@@ -269,6 +274,7 @@ class NSSolver {
   void prepare_operators();
   void upload_inlet();
   int momentum_predictor();
+  void repeat_pressure_correction();
   int pressure_poisson();
   void correct_velocity();
   int finish_step();

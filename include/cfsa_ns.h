@@ -78,6 +78,12 @@ typedef struct cfsa_solver_options {
                             (timing = 2) always run eagerly. */
   int initial_guess;     /* 0 (reference): the momentum solve starts from zero; 1: the
                             mixed-precision momentum refinement starts from u^n */
+  int reuse_pressure_correction; /* 1 (default): with several PISO correctors, correctors
+                            after the first reuse the first one's pressure correction p'.
+                            The reference re-solves (NavierStokes.cpp:392-405), but every
+                            input of that solve (u*, L_bc, right-hand side) is unchanged
+                            between correctors, so the result is the same bits; p still
+                            accumulates relax_p p' per corrector.  0: re-solve. */
 } cfsa_solver_options;
 
 typedef struct cfsa_step_stats {
@@ -91,6 +97,7 @@ typedef struct cfsa_step_stats {
   double ms_conv, ms_rhs, ms_momentum, ms_pressure, ms_correct, ms_total;
   int spmv_launches;
   double spmv_ms;
+  int pressure_solves;  /* pressure solves run this step */
 } cfsa_step_stats;
 
 typedef struct cfsa_solve_result {

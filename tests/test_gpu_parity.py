@@ -212,6 +212,33 @@ def test_piso_step_parity(P, O, name):
     assert abs(g.getTime() - steps * spec.get("dt", 0.01)) < 1e-12
 
 
+@pytest.mark.parametrize("ncorr", [2, 3])
+def test_corrector_reuse_bit_identical(P, O, ncorr):
+    """With several correctors, correctors after the first reuse the first one's
+    pressure correction (reuse_pressure_correction = 1, the default): every input
+    of the reference's re-solve is unchanged, so re-solving (option 0) and the
+    oracle, which re-solves, give the same bits and iteration counts, with one
+    pressure solve per step instead of ncorr."""
+    out = []
+    for reuse in (1, 0):
+        g, o, pm, om = make_case(P, O, "box", 3, ncorr=ncorr, relax=(0.5, 0.7),
+                                 pressure_solver=1, p_out=1.5)
+        g.setOptions(reuse_pressure_correction=reuse)
+        for s in range(3):
+            assert g.pisoStepStatus() == 0, P.last_error()
+            st = g.stepStats()
+            assert st["pressure_solves"] == (1 if reuse else ncorr)
+            assert len(st["pressure_iters"]) == ncorr
+            assert o.pisoStep() == 0
+            assert st["pressure_iters"] == o.stats()["pressure_iters"][:ncorr], (s, st)
+        out.append((g.getVelocity(), g.getPressure()))
+        assert relerr(out[-1][0], o.getVelocity()) <= 1e-8
+        assert relerr(out[-1][1], o.getPressure()) <= 1e-8
+        with pytest.raises(Exception):
+            g.setOptions(reuse_pressure_correction=2)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_simple_step_parity(P, O):
     g, o, pm, om = make_case(P, O, "kuhn", 4, p_out=2.0)
     for _ in range(3):
