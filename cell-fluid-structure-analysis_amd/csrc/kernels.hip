@@ -1226,7 +1226,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict_
   if (!cont_s[c]) return;
   const double be = beta_s[c];
   const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
-  for (size_t i = i0; i < N; i += stride) {
+  // four values per pass, loads first (stride is a multiple of K, so all four
+  // belong to component c)
+  size_t i = i0;
+  for (; i + 3 * stride < N; i += 4 * stride) {
+    T rv[4], pv[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t j = i + u * stride;
+      rv[u] = r[j];
+      pv[u] = p[j];
+      dv[u] = invd[j / K];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const T zi = (T)((double)dv[u] * (double)rv[u]);
+      p[i + u * stride] = (T)((double)zi + be * (double)pv[u]);
+    }
+  }
+  for (; i < N; i += stride) {
     const T zi = (T)((double)invd[i / K] * (double)r[i]);
     p[i] = (T)((double)zi + be * (double)p[i]);
   }
