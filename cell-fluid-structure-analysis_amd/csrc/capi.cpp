@@ -456,7 +456,7 @@ int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt) {
   if (opt->max_iter < 0 || opt->check_every < 1 || opt->check_every_1 < 1 || opt->precision < 0 || opt->precision > 2 ||
       opt->pressure_solver < 0 || opt->pressure_solver > 1 || !(opt->momentum_tol >= 0) ||
       !(opt->pressure_tol >= 0) || !(opt->inner_tol > 0) || opt->max_refine < 0 ||
-      opt->stop_rule < 0 || opt->stop_rule > 1 || opt->assembly < 0 || opt->assembly > 1 ||
+      opt->stop_rule < 0 || opt->stop_rule > 1 || opt->assembly < 0 || opt->assembly > 2 ||
       opt->spmv_layout < 0 || opt->spmv_layout > 2 || opt->momentum_solver < 0 ||
       opt->momentum_solver > 1 || opt->graphs < 0 || opt->graphs > 1 ||
       opt->initial_guess < 0 || opt->initial_guess > 1 || opt->reuse_pressure_correction < 0 ||

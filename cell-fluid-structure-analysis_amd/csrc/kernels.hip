@@ -1493,6 +1493,194 @@ __global__ __launch_bounds__(kBlock) void k_assemble_mapped(
   }
 }
 
+// Row gather with six matrix rows per wavefront (a group of 10 lanes per row,
+// lane b adding entry b of each incident tet's 10 x 10 block; the 10 slots of
+// one tet are distinct), 24 rows per block.  Each row's accumulators live in
+// LDS; a group visits its row's tets in ascending order (one tet ahead loaded),
+// then writes the row once and its lane 0 forms the lumped-mass inverse in CSR
+// order (k_lumped).  Same arithmetic and order as k_assemble_mapped, so the
+// same bits.  Dynamic LDS: 24 x (2 maxA + 4 maxP) doubles.
+constexpr int kAsmGroup = 10;
+constexpr int kAsmGroupsPerWave = 6;
+constexpr int kAsmRows = (kBlock / 64) * kAsmGroupsPerWave;
+
+// Vertex gradient q of a tet (selects over the four, no indexed registers).
+__device__ __forceinline__ void asm_vert(const double (&g4)[4][3], int q, double (&o)[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) o[c] = g4[0][c];
+#pragma unroll
+  for (int i = 1; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = q == i ? g4[i][c] : o[c];
+}
+// "Gradient" of local node q: the P1 gradient for a vertex, g[v0] + g[v1] for
+// the edge (v0, v1) (NavierStokes.cpp:184-194), edges in kEdgeV order.
+__device__ __forceinline__ void asm_node(const double (&g4)[4][3], int q, double (&o)[3]) {
+  if (q < 4) {
+    asm_vert(g4, q, o);
+  } else {
+    const int e = q - 4;  // 01 02 03 12 13 23
+    const int v0 = e < 3 ? 0 : (e < 5 ? 1 : 2);
+    const int v1 = e < 3 ? e + 1 : (e < 5 ? e - 1 : 3);
+    double x0[3], x1[3];
+    asm_vert(g4, v0, x0);
+    asm_vert(g4, v1, x1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = x0[c] + x1[c];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_assemble_waves(
+    int nu, int nv, int maxA, int maxP, const int *__restrict__ incp,
+    const int *__restrict__ inct, const uint8_t *__restrict__ loc,
+    const int *__restrict__ amap, const int *__restrict__ pmap,
+    const int *__restrict__ rowptrA, const int *__restrict__ rowptrP,
+    const double *__restrict__ vol, const double *__restrict__ grad, double rho, double inv_mu,
+    double *__restrict__ M, double *__restrict__ K, double *__restrict__ L,
+    double *__restrict__ G3, double *__restrict__ ilm) {
+  extern __shared__ double asm_lds[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int grp = lane / kAsmGroup, b = lane % kAsmGroup;  // grp 6: idle lanes 60..63
+  const int base = grp * kAsmGroup;                        // first lane of the group
+  const int r = (blockIdx.x * (kBlock / 64) + wv) * kAsmGroupsPerWave + grp;
+  const bool live = grp < kAsmGroupsPerWave && r < nu;
+  const int slot_grp = live ? wv * kAsmGroupsPerWave + grp : 0;
+  double *accM = asm_lds + (size_t)slot_grp * (2 * maxA + 4 * maxP);
+  double *accK = accM + maxA;
+  double *accL = accK + maxA;
+  double *accG = accL + maxP;
+  int a0 = 0, la = 0, p0 = 0, lp = 0, kb = 0, ke = 0;
+  const bool prow = live && r < nv;
+  if (live) {
+    a0 = rowptrA[r];
+    la = rowptrA[r + 1] - a0;
+    kb = incp[r];
+    ke = incp[r + 1];
+  }
+  if (prow) {
+    p0 = rowptrP[r];
+    lp = rowptrP[r + 1] - p0;
+  }
+  for (int i = b; i < la; i += kAsmGroup) {
+    accM[i] = 0.0;
+    accK[i] = 0.0;
+  }
+  for (int i = b; i < lp; i += kAsmGroup) {
+    accL[i] = 0.0;
+    accG[3 * i] = accG[3 * i + 1] = accG[3 * i + 2] = 0.0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int cnt = ke - kb;  // incident tets of this lane's row (0 when idle)
+  double gn[4][3] = {}, vn = 0.0;
+  int sln = 0, psn = 0;
+  int tl = 0, al = 0;
+  for (int j = 0; __ballot(j < cnt) != 0; ++j) {
+    // the group's lane q holds tet j0 + q of its row (j0 = j rounded down to 10)
+    if (j % kAsmGroup == 0) {
+      const int kk = kb + min(j + b, max(cnt - 1, 0));
+      tl = live && cnt > 0 ? inct[kk] : 0;
+      al = live && cnt > 0 ? loc[kk] : 0;
+      if (j == 0) {
+        const int t = __shfl(tl, base, 64);
+        if (cnt > 0) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gn[i][c] = grad[(size_t)t * 12 + i * 3 + c];
+          vn = vol[t];
+          sln = amap[(size_t)kb * 10 + b] - a0;
+          if (prow && b < 4) psn = pmap[(size_t)kb * 4 + b] - p0;
+        }
+      }
+    }
+    const bool act = j < cnt;
+    double g4[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g4[i][c] = gn[i][c];
+    const double v = vn;
+    const int sl = sln, ps = psn;
+    const int a = __shfl(al, base + j % kAsmGroup, 64);
+    // one tet ahead: its id is in the current group of 10, or the next one
+    const int jn = j + 1;
+    int tn;
+    if (jn % kAsmGroup == 0) {
+      const int kk = kb + min(jn, max(cnt - 1, 0));
+      tn = live && jn < cnt ? inct[kk] : 0;
+    } else {
+      tn = __shfl(tl, base + jn % kAsmGroup, 64);
+    }
+    if (jn < cnt) {
+      const size_t kn = (size_t)kb + jn;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gn[i][c] = grad[(size_t)tn * 12 + i * 3 + c];
+      vn = vol[tn];
+      sln = amap[kn * 10 + b] - a0;
+      if (prow && b < 4) psn = pmap[kn * 4 + b] - p0;
+    }
+    if (act) {
+      double ga[3], gb[3];
+      asm_node(g4, a, ga);
+      asm_node(g4, b, gb);
+      const double mass = rho * v;
+      const double muv = inv_mu * v;
+      double m;
+      if (a < 4 && b < 4)
+        m = (a == b) ? mass / 10.0 : mass / 20.0;
+      else if (a >= 4 && b >= 4)
+        m = (a == b) ? mass / 15.0 : mass / 60.0;
+      else
+        m = mass / 30.0;
+      // vertex-edge pairs use grad_edge . grad_vertex in the reference
+      const double kv = (a < 4 && b >= 4) ? muv * dot3(gb, ga) : muv * dot3(ga, gb);
+      accM[sl] += m;
+      accK[sl] += kv;
+      if (prow && b < 4) {
+        accL[ps] += v * dot3(ga, gb);
+        const double contrib = v / 4.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) accG[3 * ps + c] += contrib * gb[c];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int i = b; i < la; i += kAsmGroup) {
+    M[(size_t)a0 + i] = accM[i];
+    K[(size_t)a0 + i] = accK[i];
+  }
+  for (int i = b; i < lp; i += kAsmGroup) {
+    L[(size_t)p0 + i] = accL[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) G3[((size_t)p0 + i) * 3 + c] = accG[3 * i + c];
+  }
+  if (live && b == 0) {
+    // inv_lumped_mass_ = 1 / (M * ones), NavierStokes.cpp:290-306
+    double sum = 0.0;
+    for (int i = 0; i < la; ++i) sum += accM[i] * 1.0;
+    ilm[r] = (fabs(sum) > 1e-14) ? 1.0 / sum : 1.0;
+  }
+}
+
+// Launches k_assemble_waves; false (nothing launched) when the rows'
+// accumulators do not fit in LDS, for the caller to fall back.
+bool assemble_waves(hipStream_t s, int nu, int nv, int maxA, int maxP, const int *incp,
+                    const int *inct, const uint8_t *loc, const int *amap, const int *pmap,
+                    const DevCsr &A, const DevCsr &P, const double *vol, const double *grad,
+                    double rho, double inv_mu, double *M, double *K, double *L, double *G3,
+                    double *ilm) {
+  const size_t lds = (size_t)kAsmRows * (2 * maxA + 4 * maxP) * sizeof(double);
+  if (lds > 64 * 1024) return false;
+  if (nu <= 0) return true;
+  hipLaunchKernelGGL(k_assemble_waves, dim3(div_up(nu, kAsmRows)), dim3(kBlock), lds, s, nu, nv,
+                     maxA, maxP, incp, inct, loc, amap, pmap, A.rowptr, P.rowptr, vol, grad,
+                     rho, inv_mu, M, K, L, G3, ilm);
+  CFSA_LAUNCH_CHECK();
+  return true;
+}
+
 void assemble_mapped(hipStream_t s, int nu, int nv, const int *incp, const int *inct,
                      const uint8_t *loc, const int *amap, const int *pmap, const double *vol,
                      const double *grad, double rho, double inv_mu, double *M, double *K,

@@ -143,6 +143,14 @@ void assemble_rows(hipStream_t s, int nu, const int *incp, const int *inct, cons
                    const int *ten, int nv, const double *vol, const double *grad, double rho,
                    double inv_mu, const DevCsr &A, double *M, double *K, const DevCsr &P,
                    double *L, double *G3);
+// row gather through the slot map, one wavefront per row with LDS
+// accumulators; also writes ilm.  Returns false when the longest row does not
+// fit (maxA / maxP: longest P2 / P1 row).
+bool assemble_waves(hipStream_t s, int nu, int nv, int maxA, int maxP, const int *incp,
+                    const int *inct, const uint8_t *loc, const int *amap, const int *pmap,
+                    const DevCsr &A, const DevCsr &P, const double *vol, const double *grad,
+                    double rho, double inv_mu, double *M, double *K, double *L, double *G3,
+                    double *ilm);
 void build_slot_map(hipStream_t s, int nu, const int *incp, const int *inct, const int *tets,
                     const int *ten, int nv, const DevCsr &A, const DevCsr &P, uint8_t *loc,
                     int *amap, int *pmap);

@@ -145,6 +145,11 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   }
   nnzA = PA.nnz();
   nnzP = PP.nnz();
+  maxrowA_ = maxrowP_ = 0;
+  for (int64_t i = 0; i < nu; ++i)
+    maxrowA_ = std::max(maxrowA_, PA.rowptr[(size_t)i + 1] - PA.rowptr[(size_t)i]);
+  for (int64_t i = 0; i < nv; ++i)
+    maxrowP_ = std::max(maxrowP_, PP.rowptr[(size_t)i + 1] - PP.rowptr[(size_t)i]);
   A_ = DevCsr{nu, nnzA, d_Arp.p, d_Acol.p};
   P_ = DevCsr{nv, nnzP, d_Prp.p, d_Pcol.p};
   {
@@ -271,21 +276,32 @@ void NSSolver::update_mesh(const double *coords, const uint8_t *p1_bc, const uin
 // Coloured, atomic-free assembly of M, K (P2 clique pattern) and L, Gx/y/z
 // (P1 pattern); reference build order G, M, L, K (NavierStokes.hpp:97-102).
 void NSSolver::assemble() {
+  const double inv_mu = 1.0 / viscosity;  // NavierStokes.cpp:153
+  if (opt.assembly != 1 && d_amap.n == 0) {
+    // slot map of the row gather, built once (the patterns never change)
+    const size_t ninc = (size_t)nt * 10, nvinc = (size_t)nt * 4;
+    d_loc.alloc(ninc);
+    d_amap.alloc(ninc * 10);
+    d_pmap.alloc(nvinc * 4);
+    build_slot_map(stream_, nu, d_incp.p, d_inct.p, d_tets.p, d_ten.p, nv, A_, P_, d_loc.p,
+                   d_amap.p, d_pmap.p);
+  }
+  // assembly 0: six rows per wavefront with LDS accumulators, unless the longest
+  // row does not fit; 2 (and that fallback): one thread per row
+  if (opt.assembly == 0 &&
+      assemble_waves(stream_, nu, nv, maxrowA_, maxrowP_, d_incp.p, d_inct.p, d_loc.p, d_amap.p,
+                     d_pmap.p, A_, P_, d_vol.p, d_grad.p, density, inv_mu, d_M.p, d_K.p, d_L.p,
+                     d_G3.p, d_ilm.p)) {
+    // every slot and every ilm entry written
+    lhs_dirty = lbc_dirty = true;
+    g3t_dirty = true;
+    return;
+  }
   d_M.zero(stream_);
   d_K.zero(stream_);
   d_L.zero(stream_);
   d_G3.zero(stream_);
-  const double inv_mu = 1.0 / viscosity;  // NavierStokes.cpp:153
-  if (opt.assembly == 0) {
-    if (d_amap.n == 0) {
-      // slot map of the row gather, built once (the patterns never change)
-      const size_t ninc = (size_t)nt * 10, nvinc = (size_t)nt * 4;
-      d_loc.alloc(ninc);
-      d_amap.alloc(ninc * 10);
-      d_pmap.alloc(nvinc * 4);
-      build_slot_map(stream_, nu, d_incp.p, d_inct.p, d_tets.p, d_ten.p, nv, A_, P_, d_loc.p,
-                     d_amap.p, d_pmap.p);
-    }
+  if (opt.assembly != 1) {
     assemble_mapped(stream_, nu, nv, d_incp.p, d_inct.p, d_loc.p, d_amap.p, d_pmap.p, d_vol.p,
                     d_grad.p, density, inv_mu, d_M.p, d_K.p, d_L.p, d_G3.p);
   } else {

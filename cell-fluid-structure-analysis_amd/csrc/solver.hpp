@@ -71,7 +71,8 @@ struct SolverOptions {
   int check_every_1 = 32;      // the same for single-rhs solves (pressure: short iterations)
   int timing = 0;              // 1: per-phase events; 2: also per-SpMV events
   int stop_scaled = 0;         // 1: stop on |D^-1 r| <= tol |D^-1 b| (Jacobi-scaled)
-  int assembly = 0;            // 0 row gather (reference summation order), 1 coloured
+  int assembly = 0;            // 0 row gather (reference summation order), 1 coloured,
+                               // 2 row gather with one thread per row (same bits as 0)
   int spmv_layout = 0;         // Krylov SpMV: 0 auto, 1 CSR-stream, 2 sliced-ELL
   int graphs = 1;              // replay Krylov iteration chunks as captured HIP graphs
   int initial_guess = 0;       // momentum solve from 0 (reference) or 1: from u^n (mixed)
@@ -204,6 +205,7 @@ class NSSolver {
   DBuf<int> d_tets, d_ten, d_order, d_incp, d_inct;
   DBuf<uint8_t> d_loc;       // row-gather slot map (assembly 0): local index per incidence
   DBuf<int> d_amap, d_pmap;  // P2 slots [10 nt][10], P1 slots of vertex rows [4 nt][4]
+  int maxrowA_ = 0, maxrowP_ = 0;  // longest P2 / P1 pattern row
   DBuf<double> d_vol, d_grad, d_xyz;
   DBuf<uint8_t> d_bc;
   std::vector<int32_t> h_colour_offsets, h_order;

@@ -64,8 +64,11 @@ typedef struct cfsa_solver_options {
                             D the Jacobi diagonal (for adjoint / finite-difference runs,
                             where the 1e10 penalty rows dominate |b|) */
   int assembly;          /* 0 row gather in ascending tet order (sums every entry in the
-                            reference's order: bit-exact matrices), 1 graph-coloured
-                            element scatter (same values up to rounding order) */
+                            reference's order: bit-exact matrices), six rows per
+                            wavefront; 1 graph-coloured element scatter (same values up
+                            to rounding order); 2 the row gather with one thread per row
+                            (same bits as 0; 0 falls back to it for rows too long for
+                            LDS) */
   int spmv_layout;       /* Krylov SpMV: 0 auto (momentum CSR-stream, pressure sliced-ELL),
                             1 CSR-stream, 2 sliced-ELL; every layout gives the same bits */
   int momentum_solver;   /* 0 BiCGSTAB (reference), 1 Jacobi PCG (the momentum matrix

@@ -119,6 +119,30 @@ def test_coloured_assembly_close(P, O, case, request):
 
 
 @pytest.mark.parametrize("case", ["kuhn4", "cyl"])
+def test_row_gather_forms_bit_identical(P, O, case, request):
+    """The default row gather (six rows per wavefront, LDS accumulators) and its
+    fallback (one thread per row, assembly = 2) write the same bits, also for
+    the lumped-mass inverse, and match the oracle."""
+    g, o, pm, om = request.getfixturevalue(case)
+    names = ["M", "K", "L", "Gx", "Gy", "Gz"]
+    try:
+        g.setOptions(assembly=2)
+        g.build_sparse_matrices()
+        thread = {n: g.getMatrix(n)[2] for n in names}
+        ilm2 = g.getInvLumpedMass()
+    finally:
+        g.setOptions(assembly=0)
+        g.build_sparse_matrices()
+    for n in names:
+        assert np.array_equal(thread[n], g.getMatrix(n)[2]), n
+        assert np.array_equal(thread[n], o.getMatrix(n)[2]), n
+    assert np.array_equal(ilm2, g.getInvLumpedMass())
+    assert np.array_equal(ilm2, o.getInvLumpedMass())
+    with pytest.raises(Exception):
+        g.setOptions(assembly=3)
+
+
+@pytest.mark.parametrize("case", ["kuhn4", "cyl"])
 @pytest.mark.parametrize("layout", [1, 2])
 def test_spmv_bit_exact(P, O, case, layout, request):
     """Both Krylov SpMV layouts (CSR-stream, sliced-ELL) and the CSR-stream
