@@ -236,8 +236,8 @@ def test_piso_step_parity(P, O, name):
     assert abs(g.getTime() - steps * spec.get("dt", 0.01)) < 1e-12
 
 
-@pytest.mark.parametrize("ncorr", [2, 3])
-def test_corrector_reuse_bit_identical(P, O, ncorr):
+@pytest.mark.parametrize("ncorr,outlet", [(2, 0), (3, 0), (2, 1)])
+def test_corrector_reuse_bit_identical(P, O, ncorr, outlet):
     """With several correctors, correctors after the first reuse the first one's
     pressure correction (reuse_pressure_correction = 1, the default): every input
     of the reference's re-solve is unchanged, so re-solving (option 0) and the
@@ -246,7 +246,7 @@ def test_corrector_reuse_bit_identical(P, O, ncorr):
     out = []
     for reuse in (1, 0):
         g, o, pm, om = make_case(P, O, "box", 3, ncorr=ncorr, relax=(0.5, 0.7),
-                                 pressure_solver=1, p_out=1.5)
+                                 pressure_solver=1, p_out=1.5, outlet=outlet)
         g.setOptions(reuse_pressure_correction=reuse)
         for s in range(3):
             assert g.pisoStepStatus() == 0, P.last_error()
@@ -260,6 +260,21 @@ def test_corrector_reuse_bit_identical(P, O, ncorr):
         assert relerr(out[-1][1], o.getPressure()) <= 1e-8
         with pytest.raises(Exception):
             g.setOptions(reuse_pressure_correction=2)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_corrector_reuse_mixed_pressure(P, O):
+    """Reuse with FP32 inner pressure solves (precision 1): the same bits as
+    re-solving, one pressure solve per step."""
+    out = []
+    for reuse in (1, 0):
+        g, o, pm, om = make_case(P, O, "kuhn", 4, ncorr=2, pressure_solver=1)
+        g.setOptions(precision=1, reuse_pressure_correction=reuse, stop_rule=1,
+                     momentum_tol=1e-10, pressure_tol=1e-10)
+        for _ in range(3):
+            assert g.pisoStepStatus() == 0, P.last_error()
+            assert g.stepStats()["pressure_solves"] == (1 if reuse else 2)
+        out.append((g.getVelocity(), g.getPressure()))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
