@@ -113,6 +113,8 @@ typedef struct cfsa_solve_result {
 } cfsa_solve_result;
 
 const char *cfsa_last_error(void);
+/* version of this C ABI (no counterpart in the reference: lets a binding check
+ * it was built against the same struct layouts) */
 int cfsa_abi_version(void);
 
 /* ------------------------------------------------------------------------
@@ -148,8 +150,10 @@ int cfsa_mesh_get_edge_nodes(const cfsa_mesh *m, double *out);         /* ne x 3
 int cfsa_mesh_get_edge_endpoints(const cfsa_mesh *m, int32_t *out);    /* ne x 2 */
 int cfsa_mesh_get_p1_bc(const cfsa_mesh *m, uint8_t *out);             /* nv */
 int cfsa_mesh_get_p2_bc(const cfsa_mesh *m, uint8_t *out);             /* ne */
-/* faces in sorted-triple order: verts nf x 3; tets nf x 4 (tet_a, tet_b,
- * face_id_a, face_id_b); geom nf x 7 (normal, area, center); any may be NULL */
+/* Mesh::getFaces (include/Mesh/Mesh.hpp:123, Face :23-33; built by
+ * Mesh::buildConnectivity, src/Mesh/Mesh.cpp:494-588): faces in sorted-triple
+ * order: verts nf x 3; tets nf x 4 (tet_a, tet_b, face_id_a, face_id_b); geom
+ * nf x 7 (normal, area, center); any may be NULL */
 int cfsa_mesh_get_faces(const cfsa_mesh *m, int32_t *verts, int32_t *tets, double *geom);
 int cfsa_mesh_get_tet_faces(const cfsa_mesh *m, int32_t *faceids, int32_t *neighbors);
 int cfsa_mesh_get_min_edge_length(const cfsa_mesh *m, double *out);
