@@ -59,6 +59,8 @@ struct KrylovState {
   // kernel reads one slot while its block 0 writes the other
   double rho_s[2][3];
   int it_s[2][3];
+  // PCG: the active flags as k_cg_xr saw them (what k_cg_p updates x for)
+  int act_s[2][3];
   double tol;
   int iters[3], restarts[3], status[3], active[3];
   int maxit, nrhs, any_active, pad;

@@ -1108,16 +1108,19 @@ __global__ void k_cg_s_init(KrylovState *st, const double *partials, int G) {
 // of the two vector kernels reduces the partial pairs it needs in every block
 // (same fixed order, same result), so no single-block scalar kernel sits
 // between them.  Iteration parity `par` selects the rho / count slot read; block
-// 0 writes the other slot, so no state field is both read and written within a
-// kernel.
+// 0 writes the other slot.  The one state field read and written within a
+// kernel is `active` in k_cg_p: block 0 clears it only for a rhs that stops, so
+// a block that reads the cleared flag decides the same; the x update there
+// goes by the snapshot act_s[par] that k_cg_xr took.
 template <class T, int K>
 __global__ __launch_bounds__(kBlock) void k_cg_xr(int n, KrylovState *__restrict__ st,
                                                   const double *__restrict__ pq_partials,
-                                                  int Gq, int par, const T *__restrict__ p,
-                                                  const T *__restrict__ q,
+                                                  int Gq, int par, const T *__restrict__ q,
                                                   const T *__restrict__ invd,
-                                                  T *__restrict__ x, T *__restrict__ r,
-                                                  int scaled, double *__restrict__ partials) {
+                                                  T *__restrict__ r, int scaled,
+                                                  double *__restrict__ partials) {
+  // the snapshot k_cg_p updates x by, written even when nothing is active
+  if (blockIdx.x == 0 && threadIdx.x < K) st->act_s[par][threadIdx.x] = st->active[threadIdx.x];
   if (!st->any_active) return;  // grid-uniform: the partials are not read then
   double pq[K];
   block_reduce_pairs<K>(pq_partials, Gq, pq);
@@ -1129,14 +1132,35 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(int n, KrylovState *__restrict
   const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
   const int c = (int)(i0 % K);
   const bool act = st->active[c] != 0;
-  // alpha = rho / p.Ap (Eigen conjugate_gradient)
+  // alpha = rho / p.Ap (Eigen conjugate_gradient); x += alpha p follows in k_cg_p
   const double al = act ? st->rho_s[par][c] / pq[c] : 0.0;
   DD acc[2];
   dd_zero(acc[0]);
   dd_zero(acc[1]);
   const size_t N = act ? (size_t)n * K : 0, stride = (size_t)gridDim.x * kBlock;
-  for (size_t i = i0; i < N; i += stride) {
-    x[i] = (T)((double)x[i] + al * (double)p[i]);
+  // four values per pass, loads first; the pairs accumulate in the same order
+  // as one value per pass
+  size_t i = i0;
+  for (; i + 3 * stride < N; i += 4 * stride) {
+    T rv[4], qv[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t j = i + u * stride;
+      rv[u] = r[j];
+      qv[u] = q[j];
+      dv[u] = invd[j / K];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const T ri = (T)((double)rv[u] - al * (double)qv[u]);
+      r[i + u * stride] = ri;
+      const T zi = (T)((double)dv[u] * (double)ri);  // z = M^-1 r, not stored
+      const double sn = scaled ? (double)zi : (double)ri;
+      dd_fma(acc[0], sn, sn);
+      dd_fma(acc[1], (double)ri, (double)zi);
+    }
+  }
+  for (; i < N; i += stride) {
     const T ri = (T)((double)r[i] - al * (double)q[i]);
     r[i] = ri;
     const T zi = (T)((double)invd[i / K] * (double)ri);  // z = M^-1 r, not stored
@@ -1200,53 +1224,73 @@ __device__ void cg_decide(KrylovState *__restrict__ st, const double *tot, int p
 }
 
 // Every block reduces k_cg_xr's pairs and decides (block 0 writes the state),
-// then p = z + beta p with z = M^-1 r recomputed from r (same rounding as in
-// k_cg_xr) for the rhs that continue.
+// then, per value: x += alpha p for the rhs k_cg_xr updated (Eigen's update of
+// x, moved here because this kernel reads p anyway), and p = z + beta p with
+// z = M^-1 r recomputed from r (same rounding as in k_cg_xr) for the rhs that
+// continue.
 template <class T, int K>
 __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict__ st,
                                                  const double *__restrict__ partials, int Gv,
                                                  int par, const T *__restrict__ r,
                                                  const T *__restrict__ invd,
-                                                 T *__restrict__ p) {
-  __shared__ double beta_s[K];
-  __shared__ int cont_s[K];
-  // Block 0 of this grid clears any_active when no rhs continues; a block
-  // that reads the cleared flag has nothing to do, but the flag is read once
-  // per block so that the block leaves (or stays) as a whole.
-  __shared__ int go;
-  if (threadIdx.x == 0) go = st->any_active;
-  __syncthreads();
+                                                 T *__restrict__ p, T *__restrict__ x) {
+  __shared__ double beta_s[K], alpha_s[K];
+  __shared__ int cont_s[K], xact_s[K];
+  // act_s[par] is stable during this kernel (block 0 rewrites only `active`),
+  // so every block takes the same early exit
+  int go = 0;
+#pragma unroll
+  for (int c = 0; c < K; ++c) go |= st->act_s[par][c];
   if (!go) return;
   double tot[2 * K];
   block_reduce_pairs<2 * K>(partials, Gv, tot);
-  if (threadIdx.x == 0) cg_decide<K>(st, tot, par, blockIdx.x == 0, beta_s, cont_s);
+  if (threadIdx.x == 0) {
+    cg_decide<K>(st, tot, par, blockIdx.x == 0, beta_s, cont_s);
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      xact_s[c] = st->act_s[par][c];
+      alpha_s[c] = xact_s[c] ? st->alpha[c] : 0.0;
+    }
+  }
   __syncthreads();
   const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
   const int c = (int)(i0 % K);
-  if (!cont_s[c]) return;
-  const double be = beta_s[c];
+  if (!xact_s[c]) return;
+  const double be = beta_s[c], al = alpha_s[c];
+  const bool cont = cont_s[c] != 0;
   const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
   // four values per pass, loads first (stride is a multiple of K, so all four
   // belong to component c)
   size_t i = i0;
   for (; i + 3 * stride < N; i += 4 * stride) {
-    T rv[4], pv[4], dv[4];
+    T rv[4], pv[4], dv[4], xv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const size_t j = i + u * stride;
-      rv[u] = r[j];
       pv[u] = p[j];
-      dv[u] = invd[j / K];
+      xv[u] = x[j];
+      if (cont) {
+        rv[u] = r[j];
+        dv[u] = invd[j / K];
+      }
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const T zi = (T)((double)dv[u] * (double)rv[u]);
-      p[i + u * stride] = (T)((double)zi + be * (double)pv[u]);
+      const size_t j = i + u * stride;
+      x[j] = (T)((double)xv[u] + al * (double)pv[u]);
+      if (cont) {
+        const T zi = (T)((double)dv[u] * (double)rv[u]);
+        p[j] = (T)((double)zi + be * (double)pv[u]);
+      }
     }
   }
   for (; i < N; i += stride) {
-    const T zi = (T)((double)invd[i / K] * (double)r[i]);
-    p[i] = (T)((double)zi + be * (double)p[i]);
+    const T pi = p[i];
+    x[i] = (T)((double)x[i] + al * (double)pi);
+    if (cont) {
+      const T zi = (T)((double)invd[i / K] * (double)r[i]);
+      p[i] = (T)((double)zi + be * (double)pi);
+    }
   }
 }
 
@@ -1266,28 +1310,27 @@ void cg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G) {
 }
 template <class T, int K>
 void cg_xr(hipStream_t s, int n, KrylovState *st, const double *pq_partials, int Gq, int par,
-           const T *p, const T *q, const T *invd, T *x, T *r, int scaled, double *partials,
-           int *grid) {
+           const T *q, const T *invd, T *r, int scaled, double *partials, int *grid) {
   const int G = vec_grid_k(n, K);
   *grid = G;
   hipLaunchKernelGGL((k_cg_xr<T, K>), dim3(G), dim3(kBlock), 0, s, n, st, pq_partials, Gq, par,
-                     p, q, invd, x, r, scaled, partials);
+                     q, invd, r, scaled, partials);
   CFSA_LAUNCH_CHECK();
 }
 template <class T, int K>
 void cg_p(hipStream_t s, int n, KrylovState *st, const double *partials, int Gv, int par,
-          const T *r, const T *invd, T *p) {
+          const T *r, const T *invd, T *p, T *x) {
   hipLaunchKernelGGL((k_cg_p<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st, partials,
-                     Gv, par, r, invd, p);
+                     Gv, par, r, invd, p, x);
   CFSA_LAUNCH_CHECK();
 }
 #define CFSA_INST_CG(T, K)                                                                   \
   template void cg_init<T, K>(hipStream_t, int, const T *, const T *, T *, T *, T *, int,      \
                               double *, int *);                                              \
   template void cg_xr<T, K>(hipStream_t, int, KrylovState *, const double *, int, int,       \
-                            const T *, const T *, const T *, T *, T *, int, double *, int *); \
+                            const T *, const T *, T *, int, double *, int *);                \
   template void cg_p<T, K>(hipStream_t, int, KrylovState *, const double *, int, int,        \
-                           const T *, const T *, T *);
+                           const T *, const T *, T *, T *);
 CFSA_INST_CG(double, 1)
 CFSA_INST_CG(double, 3)
 CFSA_INST_CG(float, 1)

@@ -124,15 +124,14 @@ void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p, 
              double *partials, int *grid);
 template <int K>
 void cg_s_init(hipStream_t s, KrylovState *st, const double *partials, int G);
-// PCG iteration after the SpMV: x, r and the stopping / rho products
-// (reducing the p.Ap pairs itself), then the convergence test and p update
+// PCG iteration after the SpMV: r and the stopping / rho products (reducing
+// the p.Ap pairs itself), then the convergence test, x and p updates
 template <class T, int K>
 void cg_xr(hipStream_t s, int n, KrylovState *st, const double *pq_partials, int Gq, int par,
-           const T *p, const T *q, const T *invd, T *x, T *r, int scaled, double *partials,
-           int *grid);
+           const T *q, const T *invd, T *r, int scaled, double *partials, int *grid);
 template <class T, int K>
 void cg_p(hipStream_t s, int n, KrylovState *st, const double *partials, int Gv, int par,
-          const T *r, const T *invd, T *p);
+          const T *r, const T *invd, T *p, T *x);
 
 // ---- FEM element / node kernels ----
 void assemble_colour(hipStream_t s, int ntc, const int *order, const int *tets,

@@ -607,9 +607,9 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
         spmv1_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
       if (ev) spmv_event_end();
       const int par = j & 1;
-      cg_xr<T, K>(s, n, st, d_part_b.p, Gb, par, w.p.p, w.v.p, invd, x, w.r.p, opt.stop_scaled,
-                  d_part_a.p, &Gv);
-      cg_p<T, K>(s, n, st, d_part_a.p, Gv, par, w.r.p, invd, w.p.p);
+      cg_xr<T, K>(s, n, st, d_part_b.p, Gb, par, w.v.p, invd, w.r.p, opt.stop_scaled, d_part_a.p,
+                  &Gv);
+      cg_p<T, K>(s, n, st, d_part_a.p, Gv, par, w.r.p, invd, w.p.p, x);
     }
   };
   const std::vector<uintptr_t> key = {
