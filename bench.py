@@ -99,7 +99,7 @@ def cpu_baseline(O, g, fp64_stats, sz, pressure_method):
 def _cpu_baseline(O, g, fp64_stats, sz, pressure_method):
     rp, col, val = g.getMatrix("lhs")
     rng = np.random.default_rng(0)
-    k_mom = 15
+    k_mom = 30
     b = rng.uniform(-1, 1, sz["nu"])
     t0 = time.perf_counter()
     O.csr_krylov(rp, col, val, b, method=0, tol=0.0, maxit=k_mom)
@@ -107,7 +107,7 @@ def _cpu_baseline(O, g, fp64_stats, sz, pressure_method):
     del rp, col, val
     rp, col, val = g.getMatrix("Lbc")
     bp = rng.uniform(-1, 1, sz["nv"])
-    k_p = 200
+    k_p = 400
     t0 = time.perf_counter()
     O.csr_krylov(rp, col, val, bp, method=pressure_method, tol=0.0, maxit=k_p)
     t_p = (time.perf_counter() - t0) / k_p
