@@ -526,9 +526,7 @@ int cfsa_ns_get_velocity_at_node(cfsa_ns *h, int64_t node, double out[3]) {
   if (int e = check_init(h)) return e;
   if (node < 0 || node >= h->s.nu) return fail(CFSA_ERR_INVALID, "node out of range");
   return guard([&]() -> int {
-    std::vector<double> u((size_t)h->s.nu * 3);
-    h->s.get_velocity(u.data());
-    for (int c = 0; c < 3; ++c) out[c] = u[(size_t)node * 3 + c];
+    h->s.get_velocity_at(node, out);
     return CFSA_OK;
   });
 }
@@ -538,9 +536,7 @@ int cfsa_ns_get_pressure_at_node(cfsa_ns *h, int64_t node, double *out) {
   if (int e = check_init(h)) return e;
   if (node < 0 || node >= h->s.nv) return fail(CFSA_ERR_INVALID, "node out of range");
   return guard([&]() -> int {
-    std::vector<double> p((size_t)h->s.nv);
-    h->s.get_pressure(p.data());
-    *out = p[(size_t)node];
+    h->s.get_pressure_at(node, out);
     return CFSA_OK;
   });
 }

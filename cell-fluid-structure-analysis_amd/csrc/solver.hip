@@ -927,6 +927,14 @@ void NSSolver::get_velocity(double *out) {
   CFSA_HIP(hipStreamSynchronize(stream_));
   d_u.download(out, (size_t)nu * 3);
 }
+void NSSolver::get_velocity_at(int64_t node, double out[3]) {
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  CFSA_HIP(hipMemcpy(out, d_u.p + (size_t)node * 3, 3 * sizeof(double), hipMemcpyDeviceToHost));
+}
+void NSSolver::get_pressure_at(int64_t node, double *out) {
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  CFSA_HIP(hipMemcpy(out, d_p.p + (size_t)node, sizeof(double), hipMemcpyDeviceToHost));
+}
 void NSSolver::set_velocity(const double *in) {
   CFSA_HIP(hipStreamSynchronize(stream_));
   CFSA_HIP(hipMemcpy(d_u.p, in, (size_t)nu * 3 * sizeof(double), hipMemcpyHostToDevice));

@@ -165,6 +165,8 @@ class NSSolver {
 
   // host <-> device field access
   void get_velocity(double *out);
+  void get_velocity_at(int64_t node, double out[3]);  // one node's 3 values only
+  void get_pressure_at(int64_t node, double *out);
   void set_velocity(const double *in);
   void get_velocity_star(double *out);
   void get_pressure(double *out);

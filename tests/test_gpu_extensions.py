@@ -128,3 +128,26 @@ def test_simulation_driver(P, tmp_path):
     nv = m.nVertices()
     u = np.array([[float(v) for v in r.split()] for r in txt[j + 1:j + 1 + nv]])
     np.testing.assert_allclose(u, g.getVelocity()[:nv], rtol=1e-5, atol=1e-10)
+
+
+def test_node_accessors(P):
+    """getVelocityAtNode / getPressureAtNode (NavierStokes.hpp:152-159) return the
+    node's entries of the full fields and reject nodes out of range."""
+    m = P.Mesh.structuredRectangularPrism(1.0, 1.0, 1.0, 3, 3, 3)
+    m.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+    g = P.NavierStokesSolver()
+    g.initialize(m, 1060.0, 0.004)
+    g.setMeanInletVelocity((0.5, 0.0, 0.0))
+    g.setDt(0.01)
+    for _ in range(2):
+        assert g.pisoStep(), P.last_error()
+    u, p = g.getVelocity(), g.getPressure()
+    sz = g.sizes()
+    for i in (0, 1, sz["nv"] - 1, sz["nv"], sz["nu"] - 1):
+        assert np.array_equal(g.getVelocityAtNode(i), u[i])
+    for i in (0, 5, sz["nv"] - 1):
+        assert g.getPressureAtNode(i) == p[i]
+    with pytest.raises(Exception):
+        g.getVelocityAtNode(sz["nu"])
+    with pytest.raises(Exception):
+        g.getPressureAtNode(sz["nv"])
