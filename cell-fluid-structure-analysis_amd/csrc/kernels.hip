@@ -1294,6 +1294,40 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict_
   }
 }
 
+// r = r0 = b, the residual of x = 0 (b - A 0 = b exactly), and the pairs of
+// its stopping norm |r|^2 (or |sc r|^2 with a per-row scale sc), binned by
+// component: what the residual SpMV returns for x = 0, without the product.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_residual_x0(int n, const double *__restrict__ b,
+                                                        double *__restrict__ r,
+                                                        double *__restrict__ r0,
+                                                        const double *__restrict__ sc,
+                                                        double *__restrict__ partials) {
+  DD acc[1];
+  dd_zero(acc[0]);
+  const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+    const double bi = b[i];
+    r[i] = bi;
+    r0[i] = bi;
+    const double rs = sc ? sc[i / K] * bi : bi;
+    dd_fma(acc[0], rs, rs);
+  }
+  block_partials_binned<K, 1>(acc, partials);
+}
+template <int K>
+void residual_x0(hipStream_t s, int n, const double *b, double *r, double *r0, const double *sc,
+                 double *partials, int *grid) {
+  const int G = vec_grid_k(n, K);
+  *grid = G;
+  hipLaunchKernelGGL((k_residual_x0<K>), dim3(G), dim3(kBlock), 0, s, n, b, r, r0, sc, partials);
+  CFSA_LAUNCH_CHECK();
+}
+template void residual_x0<1>(hipStream_t, int, const double *, double *, double *,
+                             const double *, double *, int *);
+template void residual_x0<3>(hipStream_t, int, const double *, double *, double *,
+                             const double *, double *, int *);
+
 template <class T, int K>
 void cg_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *p, int scaled,
              double *partials, int *grid) {

@@ -92,6 +92,11 @@ void spmv_grad3_acc(hipStream_t s, const DevCsr &P, const double *g, const doubl
 void momentum_rhs_adjoint(hipStream_t s, const DevCsr &A, const double *M, const double *lb,
                           const uint8_t *bc, double dt, double pen, double *lu, double *lg);
 
+// r = r0 = b (residual of x = 0) and the pairs of its stopping norm [K][G]
+template <int K>
+void residual_x0(hipStream_t s, int n, const double *b, double *r, double *r0, const double *sc,
+                 double *partials, int *grid);
+
 // ---- Krylov vector / scalar kernels ----
 int vec_grid(int n);
 template <class T, int K>

@@ -670,19 +670,22 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double 
     CFSA_HIP(hipStreamSynchronize(s));
     return std::vector<double>(out, out + 3);
   };
-  std::vector<double> bb, rr;
-  if (keep_x) {
-    // stopping norms of b itself, then the residual of the initial guess
-    double *xs = x;
-    x = d_xref.p;
-    CFSA_HIP(hipMemsetAsync(x, 0, (size_t)n * K * sizeof(double), s));
-    bb = residual();
-    x = xs;
-    rr = residual();
-  } else {
-    bb = residual();  // x = 0: r = b
+  // stopping norms of b itself: the residual of x = 0, r = b, without the SpMV
+  auto residual_zero = [&]() {
+    residual_x0<K>(s, n, b, d_r64.p, w64.r0.p, sc, d_part_r.p, &Gr);
+    hipLaunchKernelGGL((k_reduce_pairs<K>), dim3(1), dim3(kBlock), 0, s, d_part_r.p, Gr,
+                       d_scalar.p);
+    CFSA_LAUNCH_CHECK();
+    double out[3] = {0, 0, 0};
+    CFSA_HIP(hipMemcpyAsync(out, d_scalar.p, K * sizeof(double), hipMemcpyDeviceToHost, s));
+    CFSA_HIP(hipStreamSynchronize(s));
+    return std::vector<double>(out, out + 3);
+  };
+  std::vector<double> bb = residual_zero(), rr;
+  if (keep_x)
+    rr = residual();  // the residual of the initial guess
+  else
     rr = bb;
-  }
   SolveResult res;
   int conv[3] = {1, 1, 1};
   for (int k = 0;; ++k) {
