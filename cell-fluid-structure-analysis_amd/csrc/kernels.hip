@@ -1104,6 +1104,26 @@ __global__ void k_cg_s_init(KrylovState *st, const double *partials, int G) {
   st->any_active = any;
 }
 
+// Accesses to 3-rhs vectors that a kernel touches once and the next SpMV does
+// not gather (q in k_cg_xr, x in k_cg_p; tens of MB each): nontemporal, so they
+// do not displace the search direction p that the SpMV gathers next.  Measured
+// at 96^3: SpMV 407 -> 401 us, k_cg_p 83 -> 82 us.  Single-rhs vectors stay
+// cache resident and keep ordinary accesses.
+template <int K, class T>
+__device__ __forceinline__ T load_once(const T *p) {
+  if constexpr (K == 3)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+template <int K, class T>
+__device__ __forceinline__ void store_once(T v, T *p) {
+  if constexpr (K == 3)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 // One PCG iteration is SpMV (q = A p, p.q), then k_cg_xr, then k_cg_p.  Each
 // of the two vector kernels reduces the partial pairs it needs in every block
 // (same fixed order, same result), so no single-block scalar kernel sits
@@ -1147,7 +1167,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(int n, KrylovState *__restrict
     for (int u = 0; u < 4; ++u) {
       const size_t j = i + u * stride;
       rv[u] = r[j];
-      qv[u] = q[j];
+      qv[u] = load_once<K>(&q[j]);
       dv[u] = invd[j / K];
     }
 #pragma unroll
@@ -1161,7 +1181,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(int n, KrylovState *__restrict
     }
   }
   for (; i < N; i += stride) {
-    const T ri = (T)((double)r[i] - al * (double)q[i]);
+    const T ri = (T)((double)r[i] - al * (double)load_once<K>(&q[i]));
     r[i] = ri;
     const T zi = (T)((double)invd[i / K] * (double)ri);  // z = M^-1 r, not stored
     const double sn = scaled ? (double)zi : (double)ri;
@@ -1268,7 +1288,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict_
     for (int u = 0; u < 4; ++u) {
       const size_t j = i + u * stride;
       pv[u] = p[j];
-      xv[u] = x[j];
+      xv[u] = load_once<K>(&x[j]);
       if (cont) {
         rv[u] = r[j];
         dv[u] = invd[j / K];
@@ -1277,7 +1297,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict_
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const size_t j = i + u * stride;
-      x[j] = (T)((double)xv[u] + al * (double)pv[u]);
+      store_once<K>((T)((double)xv[u] + al * (double)pv[u]), &x[j]);
       if (cont) {
         const T zi = (T)((double)dv[u] * (double)rv[u]);
         p[j] = (T)((double)zi + be * (double)pv[u]);
@@ -1286,7 +1306,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict_
   }
   for (; i < N; i += stride) {
     const T pi = p[i];
-    x[i] = (T)((double)x[i] + al * (double)pi);
+    store_once<K>((T)((double)load_once<K>(&x[i]) + al * (double)pi), &x[i]);
     if (cont) {
       const T zi = (T)((double)invd[i / K] * (double)r[i]);
       p[i] = (T)((double)zi + be * (double)pi);
