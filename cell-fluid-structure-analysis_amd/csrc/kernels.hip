@@ -884,6 +884,28 @@ __global__ void k_bicg_s_restart(KrylovState *st, const double *partials, int G,
   }
 }
 
+// Accesses to 3-rhs Krylov vectors that a kernel touches once and the next
+// SpMV does not gather (tens of MB each): nontemporal, so they do not displace
+// the vector the SpMV gathers next from the caches.  Measured at 96^3: PCG
+// (q in k_cg_xr, x in k_cg_p) SpMV 407 -> 401 us, k_cg_p 83 -> 82 us; BiCGSTAB
+// FP64 (p, v in k_bicg_v1; x, y, z, s, t in k_bicg_v3) v1 174 -> 167 us, v3
+// 262 -> 245 us.  Single-rhs vectors stay cache resident and keep ordinary
+// accesses.
+template <int K, class T>
+__device__ __forceinline__ T load_once(const T *p) {
+  if constexpr (K == 3)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+template <int K, class T>
+__device__ __forceinline__ void store_once(T v, T *p) {
+  if constexpr (K == 3)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 template <class T, int K>
 __global__ __launch_bounds__(kBlock) void k_bicg_v1(int n, const KrylovState *__restrict__ st,
                                                     const T *__restrict__ r, T *__restrict__ p,
@@ -897,9 +919,10 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v1(int n, const KrylovState *__
   const double beta = st->beta[c], om = st->omega[c];
   const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
   for (size_t i = i0; i < N; i += stride) {
-    const double pi = (double)r[i] + beta * ((double)p[i] - om * (double)v[i]);
+    const double pi =
+        (double)r[i] + beta * ((double)load_once<K>(&p[i]) - om * (double)load_once<K>(&v[i]));
     const T pt = (T)pi;
-    p[i] = pt;
+    store_once<K>(pt, &p[i]);
     y[i] = (T)((double)invd[i / K] * (double)pt);
   }
 }
@@ -939,8 +962,10 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v3(
   dd_zero(acc[1]);
   const size_t N = act ? (size_t)n * K : 0, stride = (size_t)gridDim.x * kBlock;
   for (size_t i = i0; i < N; i += stride) {
-    x[i] = (T)((double)x[i] + (al * (double)y[i] + om * (double)z[i]));
-    const T ri = (T)((double)sv[i] - om * (double)t[i]);
+    const double xi = (double)load_once<K>(&x[i]);
+    store_once<K>(
+        (T)(xi + (al * (double)load_once<K>(&y[i]) + om * (double)load_once<K>(&z[i]))), &x[i]);
+    const T ri = (T)((double)load_once<K>(&sv[i]) - om * (double)load_once<K>(&t[i]));
     r[i] = ri;
     const double rs = sc ? (double)sc[i / K] * (double)ri : (double)ri;
     dd_fma(acc[0], rs, rs);
@@ -1102,26 +1127,6 @@ __global__ void k_cg_s_init(KrylovState *st, const double *partials, int G) {
     st->active[c] = 0;
   }
   st->any_active = any;
-}
-
-// Accesses to 3-rhs vectors that a kernel touches once and the next SpMV does
-// not gather (q in k_cg_xr, x in k_cg_p; tens of MB each): nontemporal, so they
-// do not displace the search direction p that the SpMV gathers next.  Measured
-// at 96^3: SpMV 407 -> 401 us, k_cg_p 83 -> 82 us.  Single-rhs vectors stay
-// cache resident and keep ordinary accesses.
-template <int K, class T>
-__device__ __forceinline__ T load_once(const T *p) {
-  if constexpr (K == 3)
-    return __builtin_nontemporal_load(p);
-  else
-    return *p;
-}
-template <int K, class T>
-__device__ __forceinline__ void store_once(T v, T *p) {
-  if constexpr (K == 3)
-    __builtin_nontemporal_store(v, p);
-  else
-    *p = v;
 }
 
 // One PCG iteration is SpMV (q = A p, p.q), then k_cg_xr, then k_cg_p.  Each
