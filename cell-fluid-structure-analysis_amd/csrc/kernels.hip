@@ -888,9 +888,9 @@ __global__ void k_bicg_s_restart(KrylovState *st, const double *partials, int G,
 // SpMV does not gather (tens of MB each): nontemporal, so they do not displace
 // the vector the SpMV gathers next from the caches.  Measured at 96^3: PCG
 // (q in k_cg_xr, x in k_cg_p) SpMV 407 -> 401 us, k_cg_p 83 -> 82 us; BiCGSTAB
-// FP64 (p, v in k_bicg_v1; x, y, z, s, t in k_bicg_v3) v1 174 -> 167 us, v3
-// 262 -> 245 us.  Single-rhs vectors stay cache resident and keep ordinary
-// accesses.
+// FP64 (p, v in k_bicg_v1; r, v in k_bicg_v2; x, y, z, s, t, r in k_bicg_v3)
+// v1 174 -> 153 us, v2 133 -> 126 us, v3 262 -> 247 us.  Single-rhs vectors
+// stay cache resident and keep ordinary accesses.
 template <int K, class T>
 __device__ __forceinline__ T load_once(const T *p) {
   if constexpr (K == 3)
@@ -940,7 +940,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v2(int n, const KrylovState *__
   const double al = st->alpha[c];
   const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
   for (size_t i = i0; i < N; i += stride) {
-    const T si = (T)((double)r[i] - al * (double)v[i]);
+    const T si = (T)((double)load_once<K>(&r[i]) - al * (double)load_once<K>(&v[i]));
     sv[i] = si;
     z[i] = (T)((double)invd[i / K] * (double)si);
   }
@@ -966,7 +966,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v3(
     store_once<K>(
         (T)(xi + (al * (double)load_once<K>(&y[i]) + om * (double)load_once<K>(&z[i]))), &x[i]);
     const T ri = (T)((double)load_once<K>(&sv[i]) - om * (double)load_once<K>(&t[i]));
-    r[i] = ri;
+    store_once<K>(ri, &r[i]);
     const double rs = sc ? (double)sc[i / K] * (double)ri : (double)ri;
     dd_fma(acc[0], rs, rs);
     dd_fma(acc[1], (double)r0[i], (double)ri);
