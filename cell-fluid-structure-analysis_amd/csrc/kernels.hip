@@ -33,6 +33,15 @@ namespace cfsa {
 // last-block-done merge inside this kernel would need a device-scope release
 // per block, which writes back the XCD's whole L2 and costs far more.)
 constexpr int kStreamGroup = 256;
+// Window order: blocks are dispatched round-robin over the 8 XCDs (block b on
+// XCD b % 8).  Block b takes window ((q / C) * 8 + b % 8) * C + q % C with
+// q = b / 8 and C = kStreamChunk: every XCD runs chunks of C consecutive
+// windows, whose x gathers overlap in its L2, and the 8 XCDs work on
+// neighbouring chunks.  Measured against window = block at 96^3: FP32 momentum
+// SpMV 396 -> 390 us, FP64 residual SpMV 793 -> 777 us; cylinder 70 -> 68 us.
+// The grid is rounded up to whole rounds of 8 C blocks; blocks past the last
+// window exit at once.
+constexpr int kStreamChunk = 16;
 
 template <class T, int KO, int ND, class Op>
 __global__ __launch_bounds__(kBlock) void k_csr_stream(const int *__restrict__ rowptr,
@@ -55,7 +64,9 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(const int *__restrict__ r
   const int tid = threadIdx.x;
   const int myc = tid % KO;
   const bool act = op.active(myc) && tid < KO * R;
-  const int w = blockIdx.x;
+  const int q = (int)(blockIdx.x >> 3);
+  const int w = ((q / kStreamChunk) * 8 + (int)(blockIdx.x & 7)) * kStreamChunk + q % kStreamChunk;
+  if (w >= nwin) return;
   const int r0 = win[w], r1 = win[w + 1];  // r1 - r0 <= kStreamRows
   const int k0 = rowptr[r0];
   const int len = rowptr[r1] - k0;  // <= B by construction of the windows
@@ -262,7 +273,9 @@ static int launch_spmv(hipStream_t s, const DevCsr &A, const Op &op, double *par
     throw std::logic_error("CSR-stream inner products need window scratch");
   if (G > kMaxPartials)
     throw std::runtime_error("CSR-stream: matrix too large for the partial buffers");
-  hipLaunchKernelGGL((k_csr_stream<T, KO, ND, Op>), dim3(nwin), dim3(kBlock), 0, s, A.rowptr,
+  constexpr int round = 8 * kStreamChunk;
+  const int grid = (nwin + round - 1) / round * round;
+  hipLaunchKernelGGL((k_csr_stream<T, KO, ND, Op>), dim3(grid), dim3(kBlock), 0, s, A.rowptr,
                      win, nwin, op, A.wpart);
   CFSA_LAUNCH_CHECK();
   if constexpr (ND > 0) {
