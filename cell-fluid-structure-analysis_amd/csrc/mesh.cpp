@@ -4,6 +4,8 @@
 // and squaredNorm: left-to-right sum of three products).
 #include "mesh.hpp"
 
+#include "parallel.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -352,27 +354,32 @@ void Mesh::computeGeometry() {
   const int64_t nt = nTets();
   volumes.resize((size_t)nt);
   centroids.resize((size_t)nt * 3);
-  for (int64_t t = 0; t < nt; ++t) {
-    const int32_t *v = &tets[(size_t)t * 4];
-    V3 a = load3(&vertices[(size_t)v[0] * 3]), b = load3(&vertices[(size_t)v[1] * 3]),
-       c = load3(&vertices[(size_t)v[2] * 3]), d = load3(&vertices[(size_t)v[3] * 3]);
-    centroids[(size_t)t * 3 + 0] = (((a.x + b.x) + c.x) + d.x) / 4.0;
-    centroids[(size_t)t * 3 + 1] = (((a.y + b.y) + c.y) + d.y) / 4.0;
-    centroids[(size_t)t * 3 + 2] = (((a.z + b.z) + c.z) + d.z) / 4.0;
-    volumes[(size_t)t] = std::fabs(dot(cross(sub(b, a), sub(c, a)), sub(d, a)) / 6.0);
-  }
-  for (Face &f : faces) {
-    V3 a = load3(&vertices[(size_t)f.vertids[0] * 3]),
-       b = load3(&vertices[(size_t)f.vertids[1] * 3]),
-       c = load3(&vertices[(size_t)f.vertids[2] * 3]);
-    V3 ab = sub(b, a), ac = sub(c, a);
-    V3 n = normalized(cross(ac, ab));
-    f.normal = {n.x, n.y, n.z};
-    V3 cr = cross(ab, ac);
-    f.area = 0.5 * std::sqrt(dot(cr, cr));
-    f.center = {((a.x + b.x) + c.x) / 3.0, ((a.y + b.y) + c.y) / 3.0,
-                ((a.z + b.z) + c.z) / 3.0};
-  }
+  parallel_chunks(nt, [&](int, int64_t t0, int64_t t1) {
+    for (int64_t t = t0; t < t1; ++t) {
+      const int32_t *v = &tets[(size_t)t * 4];
+      V3 a = load3(&vertices[(size_t)v[0] * 3]), b = load3(&vertices[(size_t)v[1] * 3]),
+         c = load3(&vertices[(size_t)v[2] * 3]), d = load3(&vertices[(size_t)v[3] * 3]);
+      centroids[(size_t)t * 3 + 0] = (((a.x + b.x) + c.x) + d.x) / 4.0;
+      centroids[(size_t)t * 3 + 1] = (((a.y + b.y) + c.y) + d.y) / 4.0;
+      centroids[(size_t)t * 3 + 2] = (((a.z + b.z) + c.z) + d.z) / 4.0;
+      volumes[(size_t)t] = std::fabs(dot(cross(sub(b, a), sub(c, a)), sub(d, a)) / 6.0);
+    }
+  });
+  parallel_chunks((int64_t)faces.size(), [&](int, int64_t f0, int64_t f1) {
+    for (int64_t k = f0; k < f1; ++k) {
+      Face &f = faces[(size_t)k];
+      V3 a = load3(&vertices[(size_t)f.vertids[0] * 3]),
+         b = load3(&vertices[(size_t)f.vertids[1] * 3]),
+         c = load3(&vertices[(size_t)f.vertids[2] * 3]);
+      V3 ab = sub(b, a), ac = sub(c, a);
+      V3 n = normalized(cross(ac, ab));
+      f.normal = {n.x, n.y, n.z};
+      V3 cr = cross(ab, ac);
+      f.area = 0.5 * std::sqrt(dot(cr, cr));
+      f.center = {((a.x + b.x) + c.x) / 3.0, ((a.y + b.y) + c.y) / 3.0,
+                  ((a.z + b.z) + c.z) / 3.0};
+    }
+  });
   computeShapeFunctionGradients();
 }
 
@@ -412,26 +419,28 @@ void Mesh::computeShapeFunctionGradients() {
 
 void Mesh::tetGeometry(const double *coords, const int32_t *tets, int64_t nt,
                        double *vol, double *grad) {
-  for (int64_t t = 0; t < nt; ++t) {
-    const int32_t *v = &tets[(size_t)t * 4];
-    V3 p0 = load3(&coords[(size_t)v[0] * 3]), p1 = load3(&coords[(size_t)v[1] * 3]),
-       p2 = load3(&coords[(size_t)v[2] * 3]), p3 = load3(&coords[(size_t)v[3] * 3]);
-    // volume: reference Mesh.cpp:455
-    vol[t] = std::fabs(dot(cross(sub(p1, p0), sub(p2, p0)), sub(p3, p0)) / 6.0);
-    V3 e1 = sub(p1, p0), e2 = sub(p2, p0), e3 = sub(p3, p0);
-    const double sixv = dot(e1, cross(e2, e3));
-    const double inv = 1.0 / sixv;
-    V3 g[4] = {scale(inv, cross(sub(p3, p1), sub(p2, p1))),
-               scale(inv, cross(sub(p2, p0), sub(p3, p0))),
-               scale(inv, cross(sub(p3, p0), sub(p1, p0))),
-               scale(inv, cross(sub(p1, p0), sub(p2, p0)))};
-    double *o = &grad[(size_t)t * 12];
-    for (int i = 0; i < 4; ++i) {
-      o[i * 3 + 0] = g[i].x;
-      o[i * 3 + 1] = g[i].y;
-      o[i * 3 + 2] = g[i].z;
+  parallel_chunks(nt, [&](int, int64_t t0, int64_t t1) {
+    for (int64_t t = t0; t < t1; ++t) {
+      const int32_t *v = &tets[(size_t)t * 4];
+      V3 p0 = load3(&coords[(size_t)v[0] * 3]), p1 = load3(&coords[(size_t)v[1] * 3]),
+         p2 = load3(&coords[(size_t)v[2] * 3]), p3 = load3(&coords[(size_t)v[3] * 3]);
+      // volume: reference Mesh.cpp:455
+      vol[t] = std::fabs(dot(cross(sub(p1, p0), sub(p2, p0)), sub(p3, p0)) / 6.0);
+      V3 e1 = sub(p1, p0), e2 = sub(p2, p0), e3 = sub(p3, p0);
+      const double sixv = dot(e1, cross(e2, e3));
+      const double inv = 1.0 / sixv;
+      V3 g[4] = {scale(inv, cross(sub(p3, p1), sub(p2, p1))),
+                 scale(inv, cross(sub(p2, p0), sub(p3, p0))),
+                 scale(inv, cross(sub(p3, p0), sub(p1, p0))),
+                 scale(inv, cross(sub(p1, p0), sub(p2, p0)))};
+      double *o = &grad[(size_t)t * 12];
+      for (int i = 0; i < 4; ++i) {
+        o[i * 3 + 0] = g[i].x;
+        o[i * 3 + 1] = g[i].y;
+        o[i * 3 + 2] = g[i].z;
+      }
     }
-  }
+  });
 }
 
 // reference: Mesh::buildP2EdgeNodes (Mesh.cpp:641-677).  Edge nodes are
@@ -490,16 +499,22 @@ void Mesh::buildP2EdgeNodes() {
 
 // reference: Mesh::computeMinEdgeLength (Mesh.cpp:419-446)
 void Mesh::computeMinEdgeLength() {
-  double mn = std::numeric_limits<double>::max();
   const int64_t nt = nTets();
-  for (int64_t t = 0; t < nt; ++t) {
-    const int32_t *v = &tets[(size_t)t * 4];
-    for (int i = 0; i < 4; ++i)
-      for (int j = i + 1; j < 4; ++j) {
-        V3 d = sub(load3(&vertices[(size_t)v[i] * 3]), load3(&vertices[(size_t)v[j] * 3]));
-        mn = std::min(mn, dot(d, d));
-      }
-  }
+  // per-thread minima; the minimum does not depend on the split
+  std::vector<double> part((size_t)host_threads(), std::numeric_limits<double>::max());
+  parallel_chunks(nt, [&](int th, int64_t t0, int64_t t1) {
+    double m = std::numeric_limits<double>::max();
+    for (int64_t t = t0; t < t1; ++t) {
+      const int32_t *v = &tets[(size_t)t * 4];
+      for (int i = 0; i < 4; ++i)
+        for (int j = i + 1; j < 4; ++j) {
+          V3 d = sub(load3(&vertices[(size_t)v[i] * 3]), load3(&vertices[(size_t)v[j] * 3]));
+          m = std::min(m, dot(d, d));
+        }
+    }
+    part[(size_t)th] = m;
+  });
+  const double mn = *std::min_element(part.begin(), part.end());
   min_edge_length = (mn == std::numeric_limits<double>::max()) ? 0.0 : std::sqrt(mn);
 }
 

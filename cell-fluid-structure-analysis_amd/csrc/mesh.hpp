@@ -58,11 +58,10 @@ class Mesh {
                                       double percent_coverage, Mesh &mesh);
   // reference: Mesh::setP2BoundariesFromP1Boundaries (Mesh.hpp:198-215)
   void setP2BoundariesFromP1Boundaries();
-  // reference: Mesh::updateGeometry (Mesh.hpp:98-101)
-  void updateGeometry() {
-    computeGeometry();
-    computeShapeFunctionGradients();
-  }
+  // reference: Mesh::updateGeometry (Mesh.hpp:98-101), which calls
+  // computeShapeFunctionGradients once more after computeGeometry (whose last
+  // step it already is); the repeat recomputes the same values and is dropped.
+  void updateGeometry() { computeGeometry(); }
 
   int64_t nVertices() const { return (int64_t)vertices.size() / 3; }
   int64_t nTets() const { return (int64_t)tets.size() / 4; }

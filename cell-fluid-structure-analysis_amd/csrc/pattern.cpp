@@ -3,33 +3,10 @@
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
-#include <thread>
+
+#include "parallel.hpp"
 
 namespace cfsa {
-namespace {
-
-int n_threads() {
-  unsigned h = std::thread::hardware_concurrency();
-  return (int)std::max(1u, std::min(h, 16u));
-}
-
-template <class F>
-void parallel_chunks(int64_t n, F &&f) {
-  const int nt = n_threads();
-  if (nt == 1 || n < 4096) {
-    f(0, (int64_t)0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  const int64_t chunk = (n + nt - 1) / nt;
-  for (int t = 0; t < nt; ++t) {
-    const int64_t a = std::min(n, t * chunk), b = std::min(n, a + chunk);
-    th.emplace_back([&f, t, a, b] { f(t, a, b); });
-  }
-  for (auto &x : th) x.join();
-}
-
-}  // namespace
 
 std::vector<int32_t> stream_windows(const CsrPattern &P, int max_nnz, int max_rows) {
   std::vector<int32_t> w{0};
@@ -123,7 +100,7 @@ CsrPattern build_pattern(const std::vector<int32_t> &elem_nodes, int npt,
                          const int32_t *inc_ptr, const int32_t *inc_tet, int64_t n) {
   CsrPattern P;
   P.n = n;
-  const int T = n_threads();
+  const int T = host_threads();
   std::vector<std::vector<int32_t>> cols((size_t)T);
   std::vector<std::vector<int32_t>> cnt((size_t)T);
   std::vector<int64_t> row0((size_t)T, 0), row1((size_t)T, 0);
