@@ -299,24 +299,33 @@ void Mesh::buildConnectivity() {
     int32_t tf;  // tet * 4 + local face
   };
   std::vector<Rec> recs((size_t)nt * 4);
-  for (int64_t t = 0; t < nt; ++t) {
-    const int32_t *v = &tets[(size_t)t * 4];
-    for (int f = 0; f < 4; ++f) {
-      int32_t a = v[kLocalFace[f][0]], b = v[kLocalFace[f][1]],
-              c = v[kLocalFace[f][2]];
-      if (a > b) std::swap(a, b);
-      if (b > c) std::swap(b, c);
-      if (a > b) std::swap(a, b);
-      recs[(size_t)t * 4 + f] = {{a, b, c}, (int32_t)(t * 4 + f)};
+  parallel_chunks(nt, [&](int, int64_t t0, int64_t t1) {
+    for (int64_t t = t0; t < t1; ++t) {
+      const int32_t *v = &tets[(size_t)t * 4];
+      for (int f = 0; f < 4; ++f) {
+        int32_t a = v[kLocalFace[f][0]], b = v[kLocalFace[f][1]],
+                c = v[kLocalFace[f][2]];
+        if (a > b) std::swap(a, b);
+        if (b > c) std::swap(b, c);
+        if (a > b) std::swap(a, b);
+        recs[(size_t)t * 4 + f] = {{a, b, c}, (int32_t)(t * 4 + f)};
+      }
     }
-  }
-  std::sort(recs.begin(), recs.end(), [](const Rec &x, const Rec &y) {
+  });
+  parallel_sort(recs, [](const Rec &x, const Rec &y) {
     if (x.k[0] != y.k[0]) return x.k[0] < y.k[0];
     if (x.k[1] != y.k[1]) return x.k[1] < y.k[1];
     if (x.k[2] != y.k[2]) return x.k[2] < y.k[2];
     return x.tf < y.tf;
   });
   faces.clear();
+  {
+    size_t nf = recs.empty() ? 0 : 1;
+    for (size_t q = 1; q < recs.size(); ++q)
+      nf += recs[q].k[0] != recs[q - 1].k[0] || recs[q].k[1] != recs[q - 1].k[1] ||
+            recs[q].k[2] != recs[q - 1].k[2];
+    faces.reserve(nf);
+  }
   tet_faceids.assign((size_t)nt * 4, -1);
   tet_neighbors.assign((size_t)nt * 4, -1);
   size_t i = 0;
@@ -454,15 +463,17 @@ void Mesh::buildP2EdgeNodes() {
     int64_t pos;  // tet * 6 + local edge
   };
   std::vector<Rec> recs((size_t)nt * 6);
-  for (int64_t t = 0; t < nt; ++t) {
-    const int32_t *v = &tets[(size_t)t * 4];
-    for (int e = 0; e < 6; ++e) {
-      int32_t a = v[kLocalEdge[e][0]], b = v[kLocalEdge[e][1]];
-      if (a > b) std::swap(a, b);
-      recs[(size_t)t * 6 + e] = {a, b, t * 6 + e};
+  parallel_chunks(nt, [&](int, int64_t t0, int64_t t1) {
+    for (int64_t t = t0; t < t1; ++t) {
+      const int32_t *v = &tets[(size_t)t * 4];
+      for (int e = 0; e < 6; ++e) {
+        int32_t a = v[kLocalEdge[e][0]], b = v[kLocalEdge[e][1]];
+        if (a > b) std::swap(a, b);
+        recs[(size_t)t * 6 + e] = {a, b, t * 6 + e};
+      }
     }
-  }
-  std::sort(recs.begin(), recs.end(), [](const Rec &x, const Rec &y) {
+  });
+  parallel_sort(recs, [](const Rec &x, const Rec &y) {
     if (x.lo != y.lo) return x.lo < y.lo;
     if (x.hi != y.hi) return x.hi < y.hi;
     return x.pos < y.pos;
@@ -477,23 +488,25 @@ void Mesh::buildP2EdgeNodes() {
   }
   std::vector<size_t> order(first.size());
   std::iota(order.begin(), order.end(), 0);
-  std::sort(order.begin(), order.end(),
-            [&](size_t a, size_t b) { return first[a].first < first[b].first; });
+  parallel_sort(order, [&](size_t a, size_t b) { return first[a].first < first[b].first; });
   const size_t ne = first.size();
   edge_nodes.resize(ne * 3);
   edge_endpoints.resize(ne * 2);
   tet_edge_nodes.assign((size_t)nt * 6, -1);
-  for (size_t id = 0; id < ne; ++id) {
-    const size_t start = first[order[id]].second;
-    const int32_t lo = recs[start].lo, hi = recs[start].hi;
-    edge_endpoints[id * 2 + 0] = lo;
-    edge_endpoints[id * 2 + 1] = hi;
-    for (int c = 0; c < 3; ++c)
-      edge_nodes[id * 3 + c] =
-          0.5 * (vertices[(size_t)lo * 3 + c] + vertices[(size_t)hi * 3 + c]);
-    for (size_t q = start; q < recs.size() && recs[q].lo == lo && recs[q].hi == hi; ++q)
-      tet_edge_nodes[(size_t)recs[q].pos] = (int32_t)id;
-  }
+  // every edge writes its own entries (and the tet slots of its own records)
+  parallel_chunks((int64_t)ne, [&](int, int64_t i0, int64_t i1) {
+    for (size_t id = (size_t)i0; id < (size_t)i1; ++id) {
+      const size_t start = first[order[id]].second;
+      const int32_t lo = recs[start].lo, hi = recs[start].hi;
+      edge_endpoints[id * 2 + 0] = lo;
+      edge_endpoints[id * 2 + 1] = hi;
+      for (int c = 0; c < 3; ++c)
+        edge_nodes[id * 3 + c] =
+            0.5 * (vertices[(size_t)lo * 3 + c] + vertices[(size_t)hi * 3 + c]);
+      for (size_t q = start; q < recs.size() && recs[q].lo == lo && recs[q].hi == hi; ++q)
+        tet_edge_nodes[(size_t)recs[q].pos] = (int32_t)id;
+    }
+  });
   p2_bc.assign(ne, BC_UNDEFINED);
 }
 

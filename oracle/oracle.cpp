@@ -1160,6 +1160,16 @@ int orc_mesh_from_arrays(int64_t nv, const double *coords, int64_t nt, const int
   });
 }
 
+// Mesh::updateGeometry after the vertices move (Mesh.hpp:98-101): element and
+// face geometry from the stored (possibly flipped) face vertex order.
+int orc_mesh_update_vertices(orc_mesh *m, const double *coords) {
+  for (size_t i = 0; i < m->vertices.size(); ++i)
+    m->vertices[i] = V3{coords[i * 3], coords[i * 3 + 1], coords[i * 3 + 2]};
+  m->computeGeometry();
+  m->computeShapeFunctionGradients();
+  return 0;
+}
+
 int orc_mesh_setup_bc(orc_mesh *m, const double dir[3], double coverage) {
   m->setupBoundaryConditions(dir, coverage);
   return 0;

@@ -36,6 +36,7 @@ int orc_mesh_structured_prism(double L, double W, double H, int nx, int ny, int 
                               orc_mesh **out);
 int orc_mesh_from_arrays(int64_t nv, const double *coords, int64_t nt, const int32_t *tets,
                          orc_mesh **out);
+int orc_mesh_update_vertices(orc_mesh *m, const double *coords);
 int orc_mesh_setup_bc(orc_mesh *m, const double dir[3], double coverage);
 int orc_mesh_set_p1_bc(orc_mesh *m, const uint8_t *bc);
 void orc_mesh_sizes(const orc_mesh *m, int64_t out[4]); /* nv nt ne nfaces */

@@ -36,6 +36,7 @@ _sig("orc_mesh_structured_prism", C.c_int, C.c_double, C.c_double, C.c_double, C
      C.c_int, C.POINTER(_p))
 _sig("orc_mesh_from_arrays", C.c_int, C.c_int64, _f64p, C.c_int64, _i32p, C.POINTER(_p))
 _sig("orc_mesh_setup_bc", C.c_int, _p, _f64p, C.c_double)
+_sig("orc_mesh_update_vertices", C.c_int, _p, _f64p)
 _sig("orc_mesh_set_p1_bc", C.c_int, _p, _u8p)
 _sig("orc_mesh_sizes", None, _p, C.POINTER(C.c_int64))
 for _n, _t in [("vertices", _f64p), ("tets", _i32p), ("volumes", _f64p), ("gradients", _f64p),
@@ -160,6 +161,11 @@ class OracleMesh:
 
     def setupBoundaryConditions(self, direction=(1.0, 0.0, 0.0), coverage=1.0):
         _lib.orc_mesh_setup_bc(self._h, np.asarray(direction, np.float64), float(coverage))
+
+    def updateVertices(self, coords):
+        coords = np.ascontiguousarray(coords, np.float64)
+        assert coords.shape == (self.sizes()[0], 3)
+        _lib.orc_mesh_update_vertices(self._h, coords)
 
     def setFluidVertexBCs(self, bc):
         _lib.orc_mesh_set_p1_bc(self._h, np.ascontiguousarray(bc, np.uint8))
