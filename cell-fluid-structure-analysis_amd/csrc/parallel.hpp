@@ -1,11 +1,13 @@
 // Host-side data parallelism for the set-up and geometry loops: a range split
 // into one contiguous chunk per thread (at most 16 threads).  Every loop that
 // uses it writes disjoint outputs per index, so results do not depend on the
-// thread count.
+// thread count.  An exception thrown on a worker thread (std::bad_alloc, say)
+// is rethrown on the calling thread after all workers have joined.
 #pragma once
 
 #include <algorithm>
 #include <cstdint>
+#include <exception>
 #include <thread>
 #include <vector>
 
@@ -16,6 +18,26 @@ inline int host_threads() {
   return (int)std::max(1u, std::min(h, 16u));
 }
 
+// Runs the tasks task(0) .. task(count - 1) on one thread each and rethrows
+// the first exception any of them raised.
+template <class F>
+void run_threads(int count, F &&task) {
+  std::vector<std::exception_ptr> err((size_t)count);
+  std::vector<std::thread> th;
+  th.reserve((size_t)count);
+  for (int t = 0; t < count; ++t)
+    th.emplace_back([&task, &err, t] {
+      try {
+        task(t);
+      } catch (...) {
+        err[(size_t)t] = std::current_exception();
+      }
+    });
+  for (auto &x : th) x.join();
+  for (auto &e : err)
+    if (e) std::rethrow_exception(e);
+}
+
 // f(thread, begin, end) over [0, n); runs inline below min_parallel items.
 template <class F>
 void parallel_chunks(int64_t n, F &&f, int64_t min_parallel = 4096) {
@@ -24,13 +46,11 @@ void parallel_chunks(int64_t n, F &&f, int64_t min_parallel = 4096) {
     f(0, (int64_t)0, n);
     return;
   }
-  std::vector<std::thread> th;
   const int64_t chunk = (n + nt - 1) / nt;
-  for (int t = 0; t < nt; ++t) {
+  run_threads(nt, [&](int t) {
     const int64_t a = std::min(n, t * chunk), b = std::min(n, a + chunk);
-    th.emplace_back([&f, t, a, b] { f(t, a, b); });
-  }
-  for (auto &x : th) x.join();
+    f(t, a, b);
+  });
 }
 
 // Sorts v by comp on up to host_threads() threads: contiguous chunks sorted in
@@ -46,27 +66,20 @@ void parallel_sort(std::vector<T> &v, C comp) {
   }
   std::vector<int64_t> b((size_t)nt + 1);
   for (int t = 0; t <= nt; ++t) b[(size_t)t] = n * t / nt;
-  {
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-      th.emplace_back([&v, &b, &comp, t] {
-        std::sort(v.begin() + b[(size_t)t], v.begin() + b[(size_t)t + 1], comp);
-      });
-    for (auto &x : th) x.join();
-  }
+  run_threads(nt, [&](int t) {
+    std::sort(v.begin() + b[(size_t)t], v.begin() + b[(size_t)t + 1], comp);
+  });
   std::vector<T> tmp(v.size());
   std::vector<T> *src = &v, *dst = &tmp;
   for (int width = 1; width < nt; width *= 2) {
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; t += 2 * width) {
+    const int pairs = (nt + 2 * width - 1) / (2 * width);
+    run_threads(pairs, [&, width](int q) {
+      const int t = q * 2 * width;
       const int64_t lo = b[(size_t)t], mid = b[(size_t)std::min(t + width, nt)],
                     hi = b[(size_t)std::min(t + 2 * width, nt)];
-      th.emplace_back([src, dst, lo, mid, hi, &comp] {
-        std::merge(src->begin() + lo, src->begin() + mid, src->begin() + mid,
-                   src->begin() + hi, dst->begin() + lo, comp);
-      });
-    }
-    for (auto &x : th) x.join();
+      std::merge(src->begin() + lo, src->begin() + mid, src->begin() + mid, src->begin() + hi,
+                 dst->begin() + lo, comp);
+    });
     std::swap(src, dst);
   }
   if (src != &v) v.swap(tmp);
