@@ -185,16 +185,13 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
       X->wpart_cap = cap;
     }
   }
-  {
-    SellPattern SA = build_sell(PA);
-    d_Asptr.upload(SA.sptr.data(), SA.sptr.size());
-    d_Arlen.upload(SA.rlen.data(), SA.rlen.size());
-    d_Aperm.upload(SA.perm.data(), SA.perm.size());
-    d_Ascol.upload(SA.col.data(), SA.col.size());
-    d_Asrc.upload(SA.src.data(), SA.src.size());
-    storedA = SA.stored();
-    As_ = DevCsr{nu, nnzA, nullptr, d_Ascol.p, d_Asptr.p, d_Arlen.p, d_Aperm.p, SA.nslices};
-  }
+  // the momentum operator's sliced-ELL copy (spmv_layout 2) is built when
+  // first used (prepare_operators)
+  for (DBuf<int> *b : {&d_Asptr, &d_Arlen, &d_Aperm, &d_Ascol, &d_Asrc}) b->release();
+  d_lhsS.release();
+  storedA = 0;
+  sellA_built_ = false;
+  sellA_ = -1;
   {
     SellPattern SP = build_sell(PP);
     d_Psptr.upload(SP.sptr.data(), SP.sptr.size());
@@ -322,6 +319,27 @@ void NSSolver::build_sparse_matrices() {
   CFSA_HIP(hipStreamSynchronize(stream_));
 }
 
+// length-sorted sliced-ELL copy of the momentum pattern (spmv_layout 2), built
+// from the device pattern when first used
+void NSSolver::ensure_sell_A() {
+  if (sellA_built_) return;
+  CsrPattern PA;
+  PA.n = nu;
+  PA.rowptr.resize((size_t)nu + 1);
+  PA.col.resize((size_t)nnzA);
+  d_Arp.download(PA.rowptr.data(), PA.rowptr.size());
+  d_Acol.download(PA.col.data(), PA.col.size());
+  SellPattern SA = build_sell(PA);
+  d_Asptr.upload(SA.sptr.data(), SA.sptr.size());
+  d_Arlen.upload(SA.rlen.data(), SA.rlen.size());
+  d_Aperm.upload(SA.perm.data(), SA.perm.size());
+  d_Ascol.upload(SA.col.data(), SA.col.size());
+  d_Asrc.upload(SA.src.data(), SA.src.size());
+  storedA = SA.stored();
+  As_ = DevCsr{nu, nnzA, nullptr, d_Ascol.p, d_Asptr.p, d_Arlen.p, d_Aperm.p, SA.nslices};
+  sellA_built_ = true;
+}
+
 void NSSolver::ensure_mixed_buffers() {
   const size_t na = (size_t)std::max<int64_t>(storedA, nnzA);
   const size_t np = (size_t)std::max<int64_t>(storedP, nnzP);
@@ -340,12 +358,13 @@ void NSSolver::ensure_mixed_buffers() {
 // lhs = M/dt + K with 1e10 on Inlet/Wall diagonals (NavierStokes.cpp:460-495);
 // L_bc = L with 1e12 on Outlet (or reference-node) diagonals (:603-626).
 void NSSolver::prepare_operators() {
-  if (opt.precision != 0) ensure_mixed_buffers();
   // Krylov SpMV layout: the momentum operator streams best as CSR-stream (its
   // gathers of neighbouring columns share cache lines), the P1 pressure
   // operator as sliced-ELL; option spmv_layout forces one of them.
   const int la = opt.spmv_layout == 2 ? 1 : 0;
   const int lp = opt.spmv_layout == 1 ? 0 : 1;
+  if (la) ensure_sell_A();
+  if (opt.precision != 0) ensure_mixed_buffers();
   if (la != sellA_) lhs_dirty = true;
   if (lp != sellP_) lbc_dirty = true;
   const bool mixed = d_lhs32.n > 0;

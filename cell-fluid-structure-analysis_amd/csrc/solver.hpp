@@ -224,6 +224,7 @@ class NSSolver {
   DBuf<int> d_Psptr, d_Prlen, d_Pperm, d_Pscol, d_Psrc;
   DevCsr As_{}, Ps_{};
   int64_t storedA = 0, storedP = 0;
+  bool sellA_built_ = false;  // the momentum operator's sliced-ELL copy exists
   // operators (CSR slot order; the *S and *32 arrays are in sliced-ELL order)
   DBuf<double> d_M, d_K, d_lhs, d_L, d_Lbc, d_G3, d_ilm, d_invdA, d_invdL;
   DBuf<double> d_lhsS, d_LbcS;
@@ -233,6 +234,8 @@ class NSSolver {
   // the Krylov operators in their current layout (valid after prepare_operators)
   const DevCsr &momA() const { return sellA_ == 1 ? As_ : A_; }
   const double *momV() const { return sellA_ == 1 ? d_lhsS.p : d_lhs.p; }
+  // builds the length-sorted sliced-ELL copy of the momentum pattern when first used
+  void ensure_sell_A();
   const DevCsr &preA() const { return sellP_ == 1 ? Ps_ : P_; }
   const double *preV() const { return sellP_ == 1 ? d_LbcS.p : d_Lbc.p; }
   double lhs_dt = 0.0;
