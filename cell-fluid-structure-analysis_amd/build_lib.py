@@ -21,7 +21,8 @@ INCLUDE = PKG.parent / "include"
 LIB = LIBDIR / "libcfsa_ns.so"
 DRIVER = LIBDIR / "cfsa_simulation"
 
-HIP_SOURCES = ["kernels.hip", "solver.hip", "adjoint.hip", "forces.hip", "capi.cpp"]
+HIP_SOURCES = ["kernels.hip", "solver.hip", "adjoint.hip", "forces.hip", "noalloc.hip",
+               "capi.cpp"]
 HOST_SOURCES = ["mesh.cpp", "pattern.cpp", "vtk.cpp", "yaml_config.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

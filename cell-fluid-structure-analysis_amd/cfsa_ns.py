@@ -169,6 +169,7 @@ _sig("cfsa_ns_get_incidence", C.c_int, _p, _i32p, _i32p)
 _sig("cfsa_ns_compute_advection", C.c_int, _p, _f64p)
 _sig("cfsa_ns_compute_divergence", C.c_int, _p, _f64p, _f64p)
 _sig("cfsa_ns_apply_matrix", C.c_int, _p, C.c_int, _f64p, _f64p, C.c_int)
+_sig("cfsa_ns_apply_matrix_noalloc", C.c_int, _p, C.c_int, _f64p, _f64p, C.c_int)
 _sig("cfsa_ns_krylov_solve", C.c_int, _p, C.c_int, C.c_int, C.c_int, _f64p, _f64p, C.c_double,
      C.c_int, C.c_int, C.POINTER(SolveResult))
 _sig("cfsa_ns_time_momentum_spmv", C.c_int, _p, C.c_int, C.c_int, C.POINTER(C.c_double))

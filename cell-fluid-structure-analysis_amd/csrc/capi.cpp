@@ -651,6 +651,17 @@ int cfsa_ns_apply_matrix(cfsa_ns *h, int which, const double *x, double *y, int 
     return CFSA_OK;
   });
 }
+int cfsa_ns_apply_matrix_noalloc(cfsa_ns *h, int which, const double *x, double *y,
+                                 int nrhs) {
+  CHECK_PTR(h);
+  CHECK_PTR(x);
+  CHECK_PTR(y);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.apply_matrix_noalloc(which, x, y, nrhs);
+    return CFSA_OK;
+  });
+}
 int cfsa_ns_krylov_solve(cfsa_ns *h, int which, int method, int nrhs, const double *b,
                          double *x, double tol, int maxit, int precision,
                          cfsa_solve_result *res) {

@@ -56,6 +56,7 @@ NSSolver::NSSolver() {
   d_mask.alloc(3);
   d_flag.alloc(1);
   d_scalar.alloc(8);
+  scratch_ = noalloc_scratch_current_device();
 }
 
 NSSolver::~NSSolver() {
@@ -251,6 +252,9 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   }
   lhs_dirty = lbc_dirty = true;
   initialized = true;
+  // allocate and fill the Krylov operators' layout copies now, so later
+  // entry points (the allocation-free one included) find them in place
+  prepare_operators();
   CFSA_HIP(hipStreamSynchronize(stream_));
 }
 

@@ -23,6 +23,26 @@ enum Status {
   ST_IO = 7,
 };
 
+// Non-zero while an allocation-free entry point runs on this thread: any
+// device allocation or free a DBuf would make then throws instead.
+inline thread_local int g_no_alloc = 0;
+struct NoAllocScope {
+  NoAllocScope() { ++g_no_alloc; }
+  ~NoAllocScope() { --g_no_alloc; }
+  NoAllocScope(const NoAllocScope &) = delete;
+  NoAllocScope &operator=(const NoAllocScope &) = delete;
+};
+inline void check_may_allocate() {
+  if (g_no_alloc)
+    throw std::runtime_error(
+        "device allocation needed inside an allocation-free entry point (run a step, "
+        "build_sparse_matrices or an allocating entry point once first)");
+}
+
+// Device address of the allocation-free entry point's static scratch array
+// on the current device (noalloc.hip).
+double *noalloc_scratch_current_device();
+
 template <class T>
 struct DBuf {
   T *p = nullptr;
@@ -36,12 +56,16 @@ struct DBuf {
   }
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      check_may_allocate();
+      (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
   }
   void alloc(size_t count) {
     if (p && count == n) return;
+    check_may_allocate();
     release();
     n = count;
     if (count) CFSA_HIP(hipMalloc(&p, count * sizeof(T)));
@@ -180,6 +204,10 @@ class NSSolver {
   void compute_advection(double *out);             // computeAdvectionRHS
   void compute_divergence(const double *u, double *out);  // computeDivergence
   void apply_matrix(int which, const double *x, double *y, int nrhs);
+  // apply_matrix with x and y staged in the library's static device scratch
+  // array instead of per-call allocations (noalloc.hip); no device memory is
+  // allocated or freed
+  void apply_matrix_noalloc(int which, const double *x, double *y, int nrhs);
   SolveResult krylov_solve(int which, int method, int nrhs, const double *b, double *x,
                            double tol, int maxit, int precision);
   hipStream_t stream() const { return stream_; }
@@ -203,6 +231,8 @@ class NSSolver {
 
  private:
   hipStream_t stream_ = nullptr;
+  // this device's static scratch array (noalloc.hip), looked up at construction
+  double *scratch_ = nullptr;
   // mesh on device
   DBuf<int> d_tets, d_ten, d_order, d_incp, d_inct;
   DBuf<uint8_t> d_loc;       // row-gather slot map (assembly 0): local index per incidence

@@ -250,6 +250,18 @@ int cfsa_ns_compute_advection(cfsa_ns *h, double *out);
 int cfsa_ns_compute_divergence(cfsa_ns *h, const double *u, double *out);
 /* y = A x; nrhs 1 or 3 (interleaved); for which 5..7 y = [Gx x, Gy x, Gz x] */
 int cfsa_ns_apply_matrix(cfsa_ns *h, int which, const double *x, double *y, int nrhs);
+/* cfsa_ns_apply_matrix without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  x and y
+ * are staged in a static 64 MiB device array per GPU, so x and y together must
+ * fit in 8 Mi doubles (x padded to a multiple of 32; CFSA_ERR_INVALID beyond).
+ * Every call on one GPU shares that GPU's array, which keeps the last call's
+ * contents: calls that overlap on one GPU race.  The operators must not need
+ * a fresh device allocation: true after initialize unless the precision or
+ * spmv_layout option changed since (then one allocating call first; otherwise
+ * CFSA_ERR_INVALID).  Same results as cfsa_ns_apply_matrix, bit for bit. */
+int cfsa_ns_apply_matrix_noalloc(cfsa_ns *h, int which, const double *x, double *y,
+                                 int nrhs);
 /* solve matrix 2 (nrhs 1 or 3) or 4 (nrhs 1) with method 0 BiCGSTAB / 1 PCG */
 int cfsa_ns_krylov_solve(cfsa_ns *h, int which, int method, int nrhs, const double *b,
                          double *x, double tol, int maxit, int precision,
