@@ -40,7 +40,7 @@ class SolverOptions(C.Structure):
                 ("stop_rule", C.c_int), ("assembly", C.c_int),
                 ("spmv_layout", C.c_int), ("momentum_solver", C.c_int),
                 ("check_every_1", C.c_int), ("graphs", C.c_int), ("initial_guess", C.c_int),
-                ("reuse_pressure_correction", C.c_int)]
+                ("reuse_pressure_correction", C.c_int), ("restart_scale", C.c_double)]
 
 
 class StepStats(C.Structure):
@@ -156,6 +156,10 @@ for _n in ["velocity", "velocity_star", "pressure", "pressure_correction", "inv_
 _sig("cfsa_ns_set_velocity", C.c_int, _p, _f64p)
 _sig("cfsa_ns_set_pressure", C.c_int, _p, _f64p)
 _sig("cfsa_ns_get_velocity_at_node", C.c_int, _p, C.c_int64, _f64p)
+_sig("cfsa_ns_get_fields_async", C.c_int, _p, _p, _p)
+_sig("cfsa_ns_wait_fields", C.c_int, _p)
+_sig("cfsa_host_alloc", C.c_int, C.c_size_t, C.POINTER(_p))
+_sig("cfsa_host_free", None, _p)
 _sig("cfsa_ns_get_pressure_at_node", C.c_int, _p, C.c_int64, C.POINTER(C.c_double))
 _sig("cfsa_ns_get_scalar", C.c_int, _p, C.c_int, C.POINTER(C.c_double))
 _sig("cfsa_ns_get_reference_pressure_node", C.c_int, _p, C.POINTER(C.c_int))
@@ -219,6 +223,23 @@ def last_error() -> str:
 
 def library():
     return _lib
+
+
+class PinnedArray:
+    """A float64 numpy array in page-locked host memory (cfsa_host_alloc)."""
+
+    def __init__(self, shape):
+        n = int(np.prod(shape))
+        self._p = _p()
+        _check(_lib.cfsa_host_alloc(n * 8, C.byref(self._p)))
+        buf = (C.c_double * n).from_address(self._p.value) if n else (C.c_double * 0)()
+        self.array = np.frombuffer(buf, dtype=np.float64).reshape(shape)
+
+    def __del__(self):
+        p, self._p = getattr(self, "_p", None), None
+        if p and p.value and _lib is not None:
+            self.array = None
+            _lib.cfsa_host_free(p)
 
 
 # ---------------------------------------------------------------------------
@@ -488,6 +509,15 @@ class NavierStokesSolver:
 
     def getInvLumpedMass(self):
         return self._arr("inv_lumped_mass", (self.sizes()["nu"],))
+
+    def getFieldsAsync(self, u_out, p_out):
+        """Start copying u and p into caller arrays (pinned ones from
+        PinnedArray overlap later steps); waitFields() completes the copy."""
+        _check(_lib.cfsa_ns_get_fields_async(self._h, u_out.ctypes.data if u_out is not None else None,
+                                             p_out.ctypes.data if p_out is not None else None))
+
+    def waitFields(self):
+        _check(_lib.cfsa_ns_wait_fields(self._h))
 
     def getVelocityAtNode(self, i):
         out = np.empty(3, np.float64)

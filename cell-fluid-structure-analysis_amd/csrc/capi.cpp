@@ -72,6 +72,7 @@ cfsa::SolverOptions to_opts(const cfsa_solver_options &o) {
   s.graphs = o.graphs;
   s.initial_guess = o.initial_guess;
   s.reuse_correction = o.reuse_pressure_correction;
+  s.restart_scale = o.restart_scale;
   return s;
 }
 
@@ -460,7 +461,7 @@ int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt) {
       opt->spmv_layout < 0 || opt->spmv_layout > 2 || opt->momentum_solver < 0 ||
       opt->momentum_solver > 1 || opt->graphs < 0 || opt->graphs > 1 ||
       opt->initial_guess < 0 || opt->initial_guess > 1 || opt->reuse_pressure_correction < 0 ||
-      opt->reuse_pressure_correction > 1)
+      opt->reuse_pressure_correction > 1 || !(opt->restart_scale > 0))
     return fail(CFSA_ERR_INVALID, "invalid solver options");
   h->s.opt = to_opts(*opt);
   return CFSA_OK;
@@ -486,6 +487,7 @@ int cfsa_ns_get_options(const cfsa_ns *h, cfsa_solver_options *o) {
   o->graphs = s.graphs;
   o->initial_guess = s.initial_guess;
   o->reuse_pressure_correction = s.reuse_correction;
+  o->restart_scale = s.restart_scale;
   return CFSA_OK;
 }
 
@@ -529,6 +531,35 @@ int cfsa_ns_get_velocity_at_node(cfsa_ns *h, int64_t node, double out[3]) {
     h->s.get_velocity_at(node, out);
     return CFSA_OK;
   });
+}
+int cfsa_ns_get_fields_async(cfsa_ns *h, double *u_out, double *p_out) {
+  CHECK_PTR(h);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.fields_async(u_out, p_out);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_wait_fields(cfsa_ns *h) {
+  CHECK_PTR(h);
+  return guard([&]() -> int {
+    h->s.fields_wait();
+    return CFSA_OK;
+  });
+}
+int cfsa_host_alloc(size_t bytes, void **out) {
+  CHECK_PTR(out);
+  *out = nullptr;
+  if (bytes == 0) return CFSA_OK;
+  const hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return fail(CFSA_ERR_DEVICE, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  }
+  return CFSA_OK;
+}
+void cfsa_host_free(void *p) {
+  if (p) (void)hipHostFree(p);
 }
 int cfsa_ns_get_pressure_at_node(cfsa_ns *h, int64_t node, double *out) {
   CHECK_PTR(h);

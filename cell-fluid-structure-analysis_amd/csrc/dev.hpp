@@ -62,6 +62,7 @@ struct KrylovState {
   // PCG: the active flags as k_cg_xr saw them (what k_cg_p updates x for)
   int act_s[2][3];
   double tol;
+  double restart_scale;  // multiplier of eps^2 in the BiCGSTAB restart test
   int iters[3], restarts[3], status[3], active[3];
   int maxit, nrhs, any_active, pad;
 };

@@ -842,7 +842,7 @@ __global__ void k_bicg_s_begin(KrylovState *st, const double *partials, int G) {
     st->rho_old[c] = st->rho[c];
     const double rho = tot[2 * c + 1];
     st->rho[c] = rho;
-    if (fabs(rho) < eps2 * st->r0_sq[c]) {
+    if (fabs(rho) < st->restart_scale * eps2 * st->r0_sq[c]) {
       st->status[c] = KS_RESTART;
       st->active[c] = 0;
       continue;

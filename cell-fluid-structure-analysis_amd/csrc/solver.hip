@@ -16,8 +16,10 @@ namespace cfsa {
 
 namespace {
 
-__global__ void k_set_params(KrylovState *st, double tol, int maxit, int nrhs) {
+__global__ void k_set_params(KrylovState *st, double tol, int maxit, int nrhs,
+                             double restart_scale) {
   st->tol = tol;
+  st->restart_scale = restart_scale;
   st->maxit = maxit;
   st->nrhs = nrhs;
 }
@@ -61,6 +63,12 @@ NSSolver::NSSolver() {
 
 NSSolver::~NSSolver() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (copy_stream_) {
+    (void)hipStreamSynchronize(copy_stream_);
+    (void)hipStreamDestroy(copy_stream_);
+  }
+  if (ev_snap_) (void)hipEventDestroy(ev_snap_);
+  if (ev_copied_) (void)hipEventDestroy(ev_copied_);
   clear_graphs();
   for (auto e : ev_pool_) (void)hipEventDestroy(e);
   if (h_state) (void)hipHostFree(h_state);
@@ -71,6 +79,7 @@ NSSolver::~NSSolver() {
 // Setup: reference NavierStokesSolver::initialize (NavierStokes.cpp:10-68)
 // ---------------------------------------------------------------------------
 void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
+  fields_wait();  // a pending host copy reads the snapshot buffers
   clear_graphs();
   if (m.nv <= 0 || m.nt <= 0 || m.ne < 0 || !m.coords || !m.tets || !m.ten || !m.p1_bc ||
       (m.ne > 0 && !m.p2_bc))
@@ -455,7 +464,8 @@ void NSSolver::set_reference_node(int r) {
 // Krylov drivers
 // ---------------------------------------------------------------------------
 void NSSolver::set_state_params(double tol, int maxit, int nrhs) {
-  hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, stream_, d_state.p, tol, maxit, nrhs);
+  hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, stream_, d_state.p, tol, maxit, nrhs,
+                     opt.restart_scale);
   CFSA_LAUNCH_CHECK();
 }
 void NSSolver::set_mask(int a, int b, int c) {
@@ -954,6 +964,38 @@ int NSSolver::simple_step() {
 void NSSolver::get_velocity(double *out) {
   CFSA_HIP(hipStreamSynchronize(stream_));
   d_u.download(out, (size_t)nu * 3);
+}
+// The snapshot is two device-to-device copies on the solver's stream (about
+// 0.1 ms at 96^3), so later steps may overwrite u and p while the host copies
+// run; the next snapshot waits for the previous host copies first.
+void NSSolver::fields_async(double *u_out, double *p_out) {
+  if (!copy_stream_) {
+    CFSA_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
+    CFSA_HIP(hipEventCreateWithFlags(&ev_snap_, hipEventDisableTiming));
+    CFSA_HIP(hipEventCreateWithFlags(&ev_copied_, hipEventDisableTiming));
+  }
+  if (copy_pending_) CFSA_HIP(hipStreamWaitEvent(stream_, ev_copied_, 0));
+  d_usnap.alloc((size_t)nu * 3);
+  d_psnap.alloc((size_t)nv);
+  CFSA_HIP(hipMemcpyAsync(d_usnap.p, d_u.p, (size_t)nu * 3 * sizeof(double),
+                          hipMemcpyDeviceToDevice, stream_));
+  CFSA_HIP(hipMemcpyAsync(d_psnap.p, d_p.p, (size_t)nv * sizeof(double),
+                          hipMemcpyDeviceToDevice, stream_));
+  CFSA_HIP(hipEventRecord(ev_snap_, stream_));
+  CFSA_HIP(hipStreamWaitEvent(copy_stream_, ev_snap_, 0));
+  if (u_out)
+    CFSA_HIP(hipMemcpyAsync(u_out, d_usnap.p, (size_t)nu * 3 * sizeof(double),
+                            hipMemcpyDeviceToHost, copy_stream_));
+  if (p_out)
+    CFSA_HIP(hipMemcpyAsync(p_out, d_psnap.p, (size_t)nv * sizeof(double),
+                            hipMemcpyDeviceToHost, copy_stream_));
+  CFSA_HIP(hipEventRecord(ev_copied_, copy_stream_));
+  copy_pending_ = true;
+}
+void NSSolver::fields_wait() {
+  if (!copy_pending_) return;
+  CFSA_HIP(hipEventSynchronize(ev_copied_));
+  copy_pending_ = false;
 }
 void NSSolver::get_velocity_at(int64_t node, double out[3]) {
   CFSA_HIP(hipStreamSynchronize(stream_));

@@ -104,6 +104,8 @@ struct SolverOptions {
   // right-hand side do not change between correctors), so 1 reuses its p'
   // instead: the same bits, one pressure solve per step.  0 re-solves.
   int reuse_correction = 1;
+  // multiplier of eps^2 in BiCGSTAB's restart test (1 = Eigen's rule)
+  double restart_scale = 1.0;
 };
 
 struct SolveResult {
@@ -190,6 +192,11 @@ class NSSolver {
   // host <-> device field access
   void get_velocity(double *out);
   void get_velocity_at(int64_t node, double out[3]);  // one node's 3 values only
+  // Snapshot u and p on the device, then copy them to host buffers on a
+  // second stream (overlapping later work when the buffers are pinned);
+  // fields_wait blocks until the copies of the last fields_async are done.
+  void fields_async(double *u_out, double *p_out);
+  void fields_wait();
   void get_pressure_at(int64_t node, double *out);
   void set_velocity(const double *in);
   void get_velocity_star(double *out);
@@ -231,6 +238,11 @@ class NSSolver {
 
  private:
   hipStream_t stream_ = nullptr;
+  // field read-back: snapshot buffers, copy stream, snapshot / copied events
+  hipStream_t copy_stream_ = nullptr;
+  hipEvent_t ev_snap_ = nullptr, ev_copied_ = nullptr;
+  bool copy_pending_ = false;
+  DBuf<double> d_usnap, d_psnap;
   // this device's static scratch array (noalloc.hip), looked up at construction
   double *scratch_ = nullptr;
   // mesh on device

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define CFSA_ABI_VERSION 1
+#define CFSA_ABI_VERSION 2
 
 enum cfsa_status {
   CFSA_OK = 0,
@@ -87,6 +87,10 @@ typedef struct cfsa_solver_options {
                             input of that solve (u*, L_bc, right-hand side) is unchanged
                             between correctors, so the result is the same bits; p still
                             accumulates relax_p p' per corrector.  0: re-solve. */
+  double restart_scale;  /* 1 (Eigen's rule): BiCGSTAB restarts when |rho| < restart_scale
+                            eps^2 |r0|^2 (Eigen's restart branch, restated from its published
+                            algorithm).  Larger values make that branch reachable on small
+                            systems, for tests; > 0. */
 } cfsa_solver_options;
 
 typedef struct cfsa_step_stats {
@@ -215,6 +219,17 @@ int cfsa_ns_get_pressure(cfsa_ns *h, double *out);         /* nv */
 int cfsa_ns_set_pressure(cfsa_ns *h, const double *in);    /* nv */
 int cfsa_ns_get_pressure_correction(cfsa_ns *h, double *out);
 int cfsa_ns_get_velocity_at_node(cfsa_ns *h, int64_t node, double out[3]);
+/* Read-back of the reference's host-side fields (getVelocity / getPressure
+ * return host vectors, NavierStokes.hpp:188-189) without stalling the solver:
+ * u (nu x 3) and p (nv) are snapshotted on the device and copied to u_out /
+ * p_out (either may be NULL) on a second stream, overlapping later steps when
+ * the buffers are pinned (cfsa_host_alloc); cfsa_ns_wait_fields blocks until
+ * the copies are done.  The buffers must stay valid until then. */
+int cfsa_ns_get_fields_async(cfsa_ns *h, double *u_out, double *p_out);
+int cfsa_ns_wait_fields(cfsa_ns *h);
+/* pinned (page-locked) host memory for cfsa_ns_get_fields_async */
+int cfsa_host_alloc(size_t bytes, void **out);
+void cfsa_host_free(void *p);
 int cfsa_ns_get_pressure_at_node(cfsa_ns *h, int64_t node, double *out);
 /* which: 0 time, 1 dt, 2 viscosity, 3 density, 4 tolerance, 5 outlet pressure */
 int cfsa_ns_get_scalar(const cfsa_ns *h, int which, double *out);

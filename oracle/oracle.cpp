@@ -378,7 +378,13 @@ struct KResult {
   int iters = 0;
   double err = 0.0;
   bool ok = true;
+  int restarts = 0;
 };
+
+// Multiplier of Eigen's eps^2 in the BiCGSTAB restart test (1 = Eigen); a
+// test knob that makes the restart branch reachable on small systems.
+double g_restart_scale = 1.0;
+int g_last_restarts = 0;
 
 std::vector<double> inv_diag(const Csr &A) {
   std::vector<double> d(A.n, 1.0);
@@ -428,7 +434,7 @@ KResult bicgstab(const Csr &A, const std::vector<double> &rhs, std::vector<doubl
   while (stop_norm2(r, D, scaled) > tol2 && i < maxit) {
     double rho_old = rho;
     rho = edot(r0, r);
-    if (std::abs(rho) < eps2 * r0_sqnorm) {
+    if (std::abs(rho) < g_restart_scale * eps2 * r0_sqnorm) {
       std::vector<double> Axr = spmv(A, x);
       for (int k = 0; k < n; ++k) r[k] = rhs[k] - Axr[k];
       r0 = r;
@@ -455,6 +461,7 @@ KResult bicgstab(const Csr &A, const std::vector<double> &rhs, std::vector<doubl
   res.err = std::sqrt(stop_norm2(r, D, scaled) / b_stop);
   res.iters = i;
   res.ok = res.err <= tol;
+  res.restarts = restarts;
   return res;
 }
 
@@ -1389,8 +1396,12 @@ int orc_csr_krylov(int64_t n, const int32_t *rp, const int32_t *col, const doubl
   std::copy(sol.begin(), sol.end(), x);
   if (iters) *iters = r.iters;
   if (err) *err = r.err;
+  g_last_restarts = r.restarts;
   return r.ok ? 0 : 1;
 }
+
+void orc_set_restart_scale(double s) { g_restart_scale = s; }
+int orc_last_restarts(void) { return g_last_restarts; }
 
 int orc_ns_krylov_solve(orc_ns *h, int which, int method, const double *b, double *x,
                         double tol, int maxit, int32_t *iters, double *err) {
@@ -1400,6 +1411,7 @@ int orc_ns_krylov_solve(orc_ns *h, int which, int method, const double *b, doubl
   std::copy(sol.begin(), sol.end(), x);
   if (iters) *iters = r.iters;
   if (err) *err = r.err;
+  g_last_restarts = r.restarts;
   return r.ok ? 0 : 1;
 }
 

@@ -95,6 +95,11 @@ int orc_ns_krylov_solve(orc_ns *h, int which, int method, const double *b, doubl
                         double tol, int maxit, int32_t *iters, double *err);
 
 /* the same solvers on a caller-provided CSR matrix */
+/* multiplier of eps^2 in the BiCGSTAB restart test (1 = Eigen's rule; a test
+ * knob) and the restart count of the last BiCGSTAB solve of orc_csr_krylov or
+ * orc_ns_krylov_solve */
+void orc_set_restart_scale(double s);
+int orc_last_restarts(void);
 int orc_csr_krylov(int64_t n, const int32_t *rp, const int32_t *col, const double *val,
                    int method, const double *b, double *x, double tol, int maxit,
                    int32_t *iters, double *err);

@@ -116,6 +116,19 @@ _sig("orc_csr_krylov", C.c_int, C.c_int64, _i32p, _i32p, _f64p, C.c_int, _f64p, 
      C.c_double, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double))
 
 
+_sig("orc_set_restart_scale", None, C.c_double)
+_sig("orc_last_restarts", C.c_int)
+
+
+def set_restart_scale(s: float) -> None:
+    """Multiplier of eps^2 in the BiCGSTAB restart test (1 = Eigen's rule)."""
+    _lib.orc_set_restart_scale(float(s))
+
+
+def last_restarts() -> int:
+    return _lib.orc_last_restarts()
+
+
 def csr_krylov(rp, col, val, b, method=0, tol=1e-8, maxit=1000):
     """Eigen-form BiCGSTAB (0) or Jacobi CG (1) on a CSR matrix."""
     b = np.ascontiguousarray(b, np.float64)

@@ -204,6 +204,42 @@ def test_krylov_parity(P, O, case, layout, request):
     g.setOptions(spmv_layout=0)
 
 
+@pytest.mark.parametrize("scale", [1e27, 1e29])
+def test_bicgstab_restart_branch_parity(P, O, kuhn4, scale):
+    """Eigen's BiCGSTAB restart branch (|rho| < eps^2 |r0|^2: r = b - A x,
+    r0 = r, the iteration count reset on the first restart) with the
+    threshold raised on both sides so that a small solve passes through it:
+    the host-side restart of the batched 3-rhs and the 1-rhs device solves
+    takes the oracle's iterations and restarts and returns its solution."""
+    g, o, pm, om = kuhn4
+    sz = g.sizes()
+    rng = np.random.default_rng(11)
+    b3 = rng.uniform(-1, 1, (sz["nu"], 3))
+    bp = rng.uniform(-1, 1, sz["nv"])
+    g.setOptions(restart_scale=scale)
+    O.set_restart_scale(scale)
+    try:
+        total = 0
+        x, info = g.krylovSolve("lhs", b3, method=0, tol=1e-10)
+        for c in range(3):
+            xo, io = o.krylovSolve("lhs", b3[:, c], method=0, tol=1e-10)
+            rs = O.last_restarts()
+            assert (info["iters"][c], info["restarts"][c]) == (io["iters"], rs), (c, info, io, rs)
+            assert relerr(x[:, c], xo) <= 1e-12, (c, relerr(x[:, c], xo))
+            total += rs
+        xp, ip = g.krylovSolve("Lbc", bp, method=0, tol=1e-9)
+        xo, io = o.krylovSolve("Lbc", bp, method=0, tol=1e-9)
+        rs = O.last_restarts()
+        assert (ip["iters"][0], ip["restarts"][0]) == (io["iters"], rs), (ip, io, rs)
+        assert relerr(xp, xo) <= 1e-12
+        total += rs
+        print(f"restart scale {scale:g}: restarts lhs {info['restarts']} Lbc {rs}")
+        assert total >= 2
+    finally:
+        g.setOptions(restart_scale=1.0)
+        O.set_restart_scale(1.0)
+
+
 # ------------------------------------------------------------------ steps
 STEP_CASES = {
     "kuhn4": dict(kind="kuhn", n=4, steps=5),
