@@ -83,12 +83,58 @@ def run(P, g, steps, warmup, torch):
     return time.perf_counter() - t0, stats
 
 
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_whole_steps(P, O, workload, steps):
+    """Whole oracle pisoSteps (convection, right-hand sides, Eigen-form Krylov
+    solves, divergence, correction: everything the reference's step does), one
+    core, timed after initialize, on the cylinder workloads: config 1 (length
+    10, radius 1, 16 divisions, max element 0.1; CG pressure) and config 2 (the
+    README cylinder, 2 correctors).  The first step starts from rest."""
+    if workload == "config1":
+        pm = P.Mesh.cylinder(10.0, 1.0, 16, 0.1)
+        R, ncorr, psolver = 1.0, 1, 1
+    else:
+        pm = P.Mesh.cylinder(30.0, 2.0, 35, 0.03)
+        R, ncorr, psolver = 2.0, 2, 0
+    om = O.OracleMesh.fromArrays(pm.vertices(), pm.tets())
+    om.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+    t0 = time.perf_counter()
+    o = O.OracleNS(om, 1060.0, 0.004)
+    t_init = time.perf_counter() - t0
+    X = np.vstack([om.vertices(), om.edgeNodes()])
+    g = np.zeros((len(X), 3))
+    g[:, 0] = 0.5 * np.clip(1.0 - (X[:, 1] ** 2 + X[:, 2] ** 2) / R ** 2, 0.0, None)
+    o.setDt(0.01)
+    o.setNumberOfCorrections(ncorr)
+    o.setInletProfile(g)
+    o.setSolver(1e-6, 1e-8, 1000, psolver)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if o.pisoStep() != 0:
+            raise RuntimeError("oracle step failed")
+    t = time.perf_counter() - t0
+    sz = o.sizes()
+    return {"steps_per_s": steps / t, "steps": steps, "s_per_step": t / steps,
+            "initialize_s": t_init, "tets": sz["nt"], "velocity_dofs": 3 * sz["nu"],
+            "pressure_dofs": sz["nv"], "last_iters": o.stats()}
+
+
 def cpu_baseline(O, g, fp64_stats, sz, pressure_method):
     """Eigen-form BiCGSTAB / Jacobi CG of the oracle (one core, inner products
     in Eigen's packet order) timed for a few iterations on the workload's own
     momentum and pressure operators, then scaled by the iteration counts of
     the timed FP64 GPU steps (the oracle takes the same counts: see
-    tests/test_gpu_parity.py)."""
+    tests/test_gpu_parity.py).  An extrapolation: a whole oracle step at 96^3
+    takes minutes; whole steps are timed on the cylinders (cpu_whole_steps)."""
     O.set_dot_mode(1)
     try:
         return _cpu_baseline(O, g, fp64_stats, sz, pressure_method)
@@ -115,6 +161,7 @@ def _cpu_baseline(O, g, fp64_stats, sz, pressure_method):
     pre = np.mean([sum(s["pressure_iters"]) for s in fp64_stats])
     t_step = mom * t_mom + pre * t_p
     return {"value": 1.0 / t_step, "unit": "PISO steps/s", "cores": 1, "kind": "port",
+            "method": "extrapolated from timed Krylov iterations",
             "sample": (f"oracle Eigen-form BiCGSTAB, {k_mom} iterations on the {sz['nu']}-row "
                        f"momentum matrix ({t_mom:.3f} s/iter per component) and {k_p} "
                        f"{'Jacobi-CG' if pressure_method == 1 else 'BiCGSTAB'} on the "
@@ -122,14 +169,19 @@ def _cpu_baseline(O, g, fp64_stats, sz, pressure_method):
                        f"FP64 run's mean {mom:.1f} momentum + {pre:.1f} pressure iterations per "
                        f"step; element kernels (convection, divergence, right-hand sides) "
                        f"not counted, which favours the CPU"),
-            "cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+            "cpu": cpu_model(), "nproc": os.cpu_count()}
 
 
+# FP64 variants: (name, momentum solver); the reference's BiCGSTAB first
+FP64 = [("fp64", 0), ("fp64_cg", 1)]
 # mixed-precision variants: (name, precision, momentum solver)
 #   precision 1: FP32 inner solves for momentum and pressure, 2: momentum only
 #   momentum solver 0: BiCGSTAB (the reference's), 1: Jacobi CG (lhs is SPD)
 MIXED = [("mixed_bicgstab_all", 1, 0, 0), ("mixed_bicgstab", 2, 0, 0), ("mixed_cg", 2, 1, 0),
          ("mixed_cg_warm", 2, 1, 1)]
+# the FP64 variant with the same momentum solver as each mixed variant
+SAME_SOLVER = {"mixed_bicgstab_all": "fp64", "mixed_bicgstab": "fp64", "mixed_cg": "fp64_cg",
+               "mixed_cg_warm": "fp64_cg"}
 # variants that keep the reference's zero initial guess compete for the headline
 HEADLINE = {"mixed_bicgstab_all", "mixed_bicgstab", "mixed_cg"}
 VEL_ERR_MAX = 1e-7
@@ -279,6 +331,8 @@ def main():
                          "does, instead of reusing the first corrector's p' (same bits; "
                          "matters only with 2 or more correctors)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-cpu-steps", action="store_true",
+                    help="skip the whole-step oracle timing on the two cylinders (~1 min)")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--only", default=None, help="run only this mixed variant (profiling)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -335,31 +389,50 @@ def main():
         mesh_desc = f"unit cube {n}^3 Kuhn 6-tet"
     log(f"mesh {workload}: {mesh.sizes()} in {time.perf_counter() - t0:.1f}s")
 
-    t0 = time.perf_counter()
-    g64 = make_solver(P, mesh, prof, args, 0)
-    sz = g64.sizes()
-    log(f"fp64 solver ready in {time.perf_counter() - t0:.1f}s: {sz}")
-    el64, st64 = run(P, g64, args.steps, args.warmup, torch)
-    log(f"fp64: {args.steps} steps in {el64:.3f}s; last {st64[-1]}")
+    def spmv_avg(st):
+        return sum(s["spmv_ms"] for s in st) / max(1, sum(s["spmv_launches"] for s in st))
+
+    res = {}
+    for name, msolver in FP64:
+        if args.only and name != "fp64" and args.only != name:
+            continue
+        t0 = time.perf_counter()
+        g = make_solver(P, mesh, prof, args, 0, msolver)
+        sz = g.sizes()
+        log(f"{name} solver ready in {time.perf_counter() - t0:.1f}s: {sz}")
+        el, st = run(P, g, args.steps, args.warmup, torch)
+        log(f"{name}: {args.steps} steps in {el:.3f}s; last {st[-1]}")
+        res[name] = {"steps_per_s": args.steps / el, "ms_per_step": 1e3 * el / args.steps,
+                     "momentum_solver": "cg_jacobi" if msolver else "bicgstab_jacobi",
+                     "momentum_iters": st[-1]["momentum_iters"],
+                     "pressure_iters": st[-1]["pressure_iters"], "spmv_avg_ms": spmv_avg(st)}
+        if name == "fp64":
+            g64, el64, st64 = g, el, st
+        else:
+            del g
     u64, p64 = g64.getVelocity(), g64.getPressure()
     d64 = g64.computeDivergence(u64)
     # a caller that reads the fields back to the host after every step (the
-    # reference's getVelocity / getPressure return host vectors) also pays this
+    # reference's getVelocity / getPressure return host vectors) also pays
+    # this: pageable copies, and the library's snapshot + pinned copy started
+    # right after the step (no later work to overlap: the worst case)
     t0 = time.perf_counter()
     for _ in range(3):
         g64.getVelocity()
         g64.getPressure()
+    host_copy_pageable_ms = 1e3 * (time.perf_counter() - t0) / 3
+    pu, pp = P.PinnedArray((sz["nu"], 3)), P.PinnedArray((sz["nv"],))
+    g64.getFieldsAsync(pu.array, pp.array)
+    g64.waitFields()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        g64.getFieldsAsync(pu.array, pp.array)
+        g64.waitFields()
     host_copy_ms = 1e3 * (time.perf_counter() - t0) / 3
+    assert np.array_equal(pu.array, u64) and np.array_equal(pp.array, p64)
+    del pu, pp
     b64 = spmv_bytes(sz["nu"], sz["nnzA"], 8)
     b32 = spmv_bytes(sz["nu"], sz["nnzA"], 4)
-
-    def spmv_avg(st):
-        return sum(s["spmv_ms"] for s in st) / max(1, sum(s["spmv_launches"] for s in st))
-
-    res = {"fp64": {"steps_per_s": args.steps / el64, "ms_per_step": 1e3 * el64 / args.steps,
-                    "momentum_iters": st64[-1]["momentum_iters"],
-                    "pressure_iters": st64[-1]["pressure_iters"],
-                    "spmv_avg_ms": spmv_avg(st64)}}
     primary, best = "fp64", None
     if not args.fp64_only:
         for name, precision, msolver, guess in MIXED:
@@ -383,7 +456,10 @@ def main():
                          "pressure_iters": st[-1]["pressure_iters"],
                          "pressure_refinements": st[-1]["pressure_refinements"],
                          "spmv_avg_ms": spmv_avg(st),
-                         "speedup_vs_fp64": el64 / el, "error_vs_fp64": err}
+                         "speedup_vs_fp64_same_solver": (
+                             res[SAME_SOLVER[name]]["ms_per_step"] / (1e3 * el / args.steps)
+                             if SAME_SOLVER[name] in res else None),
+                         "speedup_vs_fp64_bicgstab": el64 / el, "error_vs_fp64": err}
             ok = err["velocity_rel_l2"] <= VEL_ERR_MAX and name in HEADLINE
             if ok and (best is None or el < best[1]):
                 best = (name, el)
@@ -430,6 +506,11 @@ def main():
         t0 = time.perf_counter()
         cpu = cpu_baseline(O, g64, st64, sz, args.pressure_solver)
         log(f"cpu baseline in {time.perf_counter() - t0:.1f}s")
+        if not args.skip_cpu_steps:
+            t0 = time.perf_counter()
+            cpu["whole_steps"] = {"config1_cylinder_L10": cpu_whole_steps(P, O, "config1", 5),
+                                  "config2_cylinder_L30": cpu_whole_steps(P, O, "config2", 2)}
+            log(f"cpu whole steps in {time.perf_counter() - t0:.1f}s: {cpu['whole_steps']}")
 
     value = res[primary]["steps_per_s"]
     out = {"metric": METRIC, "value": value, "unit": "PISO steps/s", "n_gpus": 1,
@@ -446,15 +527,22 @@ def main():
                       "pressure_tol": args.pressure_tol, "momentum_tol": args.momentum_tol,
                       "stop_rule": "jacobi_scaled" if args.stop_rule else "reference",
                       "pressure_solver": "cg_jacobi" if args.pressure_solver else "bicgstab_jacobi",
-                      "fp64_momentum_solver": "bicgstab_jacobi", "headline": primary,
+                      "fp64_momentum_solvers": {k: v["momentum_solver"] for k, v in res.items()
+                                                if k in dict(FP64)},
+                      "headline": primary,
                       "later_correctors": ("re-solve" if args.resolve_correctors else
                                            "reuse the first corrector's p' (identical inputs, "
                                            "same bits)") if args.corrections > 1 else None,
                       "velocity_error_bound": VEL_ERR_MAX, "parallelism": "single GPU"},
            "dofs_per_s": value * (3 * sz["nu"] + sz["nv"]),
            "host_readback_ms_per_step": host_copy_ms,
+           "host_readback_pageable_ms_per_step": host_copy_pageable_ms,
            "steps_per_s_with_host_readback": 1e3 / (res[primary]["ms_per_step"] + host_copy_ms),
-           "speedup_vs_fp64": res["fp64"]["ms_per_step"] / res[primary]["ms_per_step"],
+           "speedup_vs_fp64": (res[SAME_SOLVER[primary]]["ms_per_step"] /
+                               res[primary]["ms_per_step"])
+           if SAME_SOLVER.get(primary) in res else (1.0 if primary == "fp64" else None),
+           "speedup_vs_fp64_solver": SAME_SOLVER.get(primary, "fp64"),
+           "speedup_vs_fp64_bicgstab": res["fp64"]["ms_per_step"] / res[primary]["ms_per_step"],
            **res, "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 

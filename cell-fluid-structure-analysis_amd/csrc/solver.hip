@@ -377,7 +377,10 @@ void NSSolver::prepare_operators() {
   const int la = opt.spmv_layout == 2 ? 1 : 0;
   const int lp = opt.spmv_layout == 1 ? 0 : 1;
   if (la) ensure_sell_A();
-  if (opt.precision != 0) ensure_mixed_buffers();
+  // FP32 copies, once they exist, are kept in step with every layout the
+  // operators are filled into (the sliced-ELL copy stores padded entries, more
+  // than the CSR slots the FP32 arrays were first sized for)
+  if (opt.precision != 0 || d_lhs32.n > 0) ensure_mixed_buffers();
   if (la != sellA_) lhs_dirty = true;
   if (lp != sellP_) lbc_dirty = true;
   const bool mixed = d_lhs32.n > 0;
@@ -390,6 +393,8 @@ void NSSolver::prepare_operators() {
       sell_fill<double>(stream_, storedA, d_Asrc.p, d_lhs.p, d_lhsS.p);
     }
     if (mixed) {
+      if (d_lhs32.n < (size_t)(la ? storedA : nnzA))
+        throw std::logic_error("FP32 momentum operator smaller than its layout");
       if (la)
         sell_fill<float>(stream_, storedA, d_Asrc.p, d_lhs.p, d_lhs32.p);
       else
@@ -414,6 +419,8 @@ void NSSolver::prepare_operators() {
       sell_fill<double>(stream_, storedP, d_Psrc.p, d_Lbc.p, d_LbcS.p);
     }
     if (mixed) {
+      if (d_Lbc32.n < (size_t)(lp ? storedP : nnzP))
+        throw std::logic_error("FP32 pressure operator smaller than its layout");
       if (lp)
         sell_fill<float>(stream_, storedP, d_Psrc.p, d_Lbc.p, d_Lbc32.p);
       else

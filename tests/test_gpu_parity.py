@@ -204,6 +204,26 @@ def test_krylov_parity(P, O, case, layout, request):
     g.setOptions(spmv_layout=0)
 
 
+def test_layout_switch_after_mixed_buffers(P, O):
+    """FP32 operator copies made while the momentum operator is in CSR order
+    are resized when a later call switches to the (padded) sliced-ELL layout,
+    for FP64 and FP32 SpMV and mixed solves alike (the order a benchmark
+    that times both layouts takes)."""
+    g, o, pm, om = make_case(P, O, "kuhn", 4)
+    sz = g.sizes()
+    b = np.random.default_rng(4).uniform(-1, 1, (sz["nu"], 3))
+    g.setOptions(spmv_layout=1)
+    g.timeMomentumSpmv(2, 1)
+    x1, i1 = g.krylovSolve("lhs", b, tol=1e-10, precision=1)
+    g.setOptions(spmv_layout=2)
+    g.timeMomentumSpmv(2, 0)
+    g.timeMomentumSpmv(2, 1)
+    x2, i2 = g.krylovSolve("lhs", b, tol=1e-10, precision=1)
+    assert np.array_equal(x1, x2) and i1["iters"] == i2["iters"]
+    g.setOptions(spmv_layout=0)
+    assert np.array_equal(g.applyMatrix("lhs", b, 3), O.csr_spmv(*g.getMatrix("lhs"), b))
+
+
 @pytest.mark.parametrize("scale", [1e27, 1e29])
 def test_bicgstab_restart_branch_parity(P, O, kuhn4, scale):
     """Eigen's BiCGSTAB restart branch (|rho| < eps^2 |r0|^2: r = b - A x,
