@@ -466,6 +466,16 @@ def main():
             del g
         if best is not None:
             primary = best[0]
+    if cyl:
+        # io::exportToVtk of the fluid fields (VtkExport.cpp:238-268), the
+        # runner's output path, on the README cylinder: host timing of one write
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            t0 = time.perf_counter()
+            g64.exportVtk(os.path.join(td, "fluid.vtk"))
+            res["vtk_write_s"] = time.perf_counter() - t0
+            res["vtk_bytes"] = os.path.getsize(os.path.join(td, "fluid.vtk"))
+        log(f"vtk write {res['vtk_write_s']:.2f}s, {res['vtk_bytes']} bytes")
     # value-only re-assembly of M, K, L, G into the fixed patterns (what a coupled
     # FSI step pays after the mesh moves), host-timed including the sync
     g64.build_sparse_matrices()

@@ -306,6 +306,7 @@ int cfsa_ns_initialize(cfsa_ns *h, const cfsa_mesh *mesh, double density, double
     in.ten = m.tet_edge_nodes.data();
     in.p1_bc = m.p1_bc.data();
     in.p2_bc = m.p2_bc.data();
+    in.faces = &m.faces;
     h->s.initialize(in, density, viscosity);
     h->mesh = mesh;
     h->synced = mesh->version;

@@ -71,16 +71,25 @@ __global__ __launch_bounds__(kBlock) void k_vertex_forces(
 
 }  // namespace
 
-// Faces in sorted-triple order with the reference's normals, from the current
-// vertex positions (connectivity + geometry of the host mesh layer).
+// Faces in sorted-triple order with the reference's normals.  Their list and
+// vertex order (which fixes each normal's orientation) are set once per mesh:
+// taken from the mesh handle, or built with the reference's connectivity and
+// flip rule at the first use (Mesh.cpp:494-639).  After vertices move, only
+// normal, area and centre are recomputed from the stored vertex order, as the
+// reference's Mesh::updateGeometry does (Mesh.hpp:98-101), without flipping.
 void NSSolver::ensure_faces() {
   if (faces_ready_) return;
-  Mesh m;
+  Mesh &m = face_mesh_;
   m.vertices = h_coords;
-  m.tets = h_tets;
-  m.buildConnectivity();
-  m.computeGeometry();
-  m.ensureConsistentFaceNormals();
+  if (!face_topology_) {
+    m.tets = h_tets;
+    m.buildConnectivity();
+    m.computeGeometry();
+    m.ensureConsistentFaceNormals();
+    face_topology_ = true;
+  } else {
+    m.computeGeometry();
+  }
   all_faces_.clear();
   std::vector<int32_t> bv, bt;
   std::vector<double> bg;
@@ -132,11 +141,13 @@ void NSSolver::compute_forces(int which, double *out) {
 void NSSolver::face_pressure(int64_t fi, double *mean, double force[3]) {
   ensure_faces();
   if (fi < 0 || fi >= (int64_t)all_faces_.size()) throw std::invalid_argument("face out of range");
-  std::vector<double> p((size_t)nv);
-  get_pressure(p.data());
   const Face &f = all_faces_[(size_t)fi];
   double s = 0.0;
-  for (int a = 0; a < 3; ++a) s += p[(size_t)f.vertids[(size_t)a]];
+  for (int a = 0; a < 3; ++a) {
+    double pv = 0.0;  // only the face's three vertex pressures are read
+    get_pressure_at(f.vertids[(size_t)a], &pv);
+    s += pv;
+  }
   const double avg = s / 3.0;
   if (mean) *mean = avg;
   if (force)

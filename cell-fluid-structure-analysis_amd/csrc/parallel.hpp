@@ -19,20 +19,28 @@ inline int host_threads() {
 }
 
 // Runs the tasks task(0) .. task(count - 1) on one thread each and rethrows
-// the first exception any of them raised.
+// the first exception any of them raised.  If a thread cannot be started
+// (std::system_error), the tasks not yet started run on the calling thread,
+// after which every started thread is joined as usual.
 template <class F>
 void run_threads(int count, F &&task) {
   std::vector<std::exception_ptr> err((size_t)count);
+  auto run = [&task, &err](int t) {
+    try {
+      task(t);
+    } catch (...) {
+      err[(size_t)t] = std::current_exception();
+    }
+  };
   std::vector<std::thread> th;
   th.reserve((size_t)count);
-  for (int t = 0; t < count; ++t)
-    th.emplace_back([&task, &err, t] {
-      try {
-        task(t);
-      } catch (...) {
-        err[(size_t)t] = std::current_exception();
-      }
-    });
+  int started = 0;
+  try {
+    for (; started < count; ++started) th.emplace_back(run, started);
+  } catch (...) {
+    // no thread for task `started` and the rest: run them here
+    for (int t = started; t < count; ++t) run(t);
+  }
   for (auto &x : th) x.join();
   for (auto &e : err)
     if (e) std::rethrow_exception(e);

@@ -128,6 +128,14 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   d_grad.upload(grad.data(), grad.size());
   d_bc.upload(h_bc.data(), h_bc.size());
   d_order.upload(C.order.data(), C.order.size());
+  // boundary faces: the mesh's own list when given, else built at first use
+  faces_ready_ = false;
+  face_mesh_ = Mesh();
+  face_topology_ = m.faces != nullptr;
+  if (face_topology_) {
+    face_mesh_.faces = *m.faces;
+    face_mesh_.tets = h_tets;
+  }
   d_amap.release();  // the row-gather slot map belongs to the previous patterns
   d_pmap.release();
   d_loc.release();

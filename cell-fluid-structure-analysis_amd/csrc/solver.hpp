@@ -138,6 +138,9 @@ struct MeshInput {
   const int32_t *ten = nullptr;       // nt x 6 edge-node ids (0..ne-1)
   const uint8_t *p1_bc = nullptr;     // nv
   const uint8_t *p2_bc = nullptr;     // ne
+  // faces in sorted-triple order with the orientation the mesh chose when it
+  // was built (nullptr: build them from coords)
+  const std::vector<Face> *faces = nullptr;
 };
 
 template <class T>
@@ -315,6 +318,8 @@ class NSSolver {
   DBuf<int> d_bfv, d_bft, d_vfp, d_vff;
   DBuf<double> d_bfg, d_fp, d_fs, d_forces;
   std::vector<Face> all_faces_;
+  Mesh face_mesh_;              // face list (orientation fixed once) + current geometry
+  bool face_topology_ = false;  // face_mesh_ holds the faces of this mesh
   int nbf_ = 0;
   bool faces_ready_ = false;
   void ensure_faces();

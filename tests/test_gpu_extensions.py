@@ -151,3 +151,30 @@ def test_node_accessors(P):
         g.getVelocityAtNode(sz["nu"])
     with pytest.raises(Exception):
         g.getPressureAtNode(sz["nv"])
+
+
+def test_face_accessors_after_vertex_move(P, O):
+    """After the mesh moves, the face accessors use the face list and vertex
+    order fixed when the mesh was built and recompute normal, area and centre
+    from it (Mesh::updateGeometry, Mesh.hpp:98-101): the same faces as the
+    oracle mesh moved the same way, for internal and boundary faces."""
+    g, o, pm, om = make_case(P, O, "kuhn", 3)
+    assert g.pisoStep() and o.pisoStep() == 0
+    X = pm.vertices()
+    Xd = X.copy()
+    Xd[:, 0] += 0.04 * np.sin(2 * np.pi * X[:, 1]) * X[:, 0] * (1 - X[:, 0])
+    Xd[:, 1] += 0.03 * np.cos(np.pi * X[:, 2]) * X[:, 1] * (1 - X[:, 1])
+    pm.updateVertices(Xd)
+    om.updateVertices(Xd)
+    g.build_sparse_matrices()
+    p = np.random.default_rng(8).uniform(-1, 1, g.sizes()["nv"])
+    g.setPressure(p)
+    v, t, geo = om.faces()
+    vh, th, geoh = pm.faces()
+    assert np.array_equal(v, vh) and np.array_equal(geo, geoh)
+    assert g.nFaces() == len(v)
+    for fi in range(0, len(v), 7):
+        mp = ((p[v[fi, 0]] + p[v[fi, 1]]) + p[v[fi, 2]]) / 3.0
+        assert g.getMeanFacePressure(fi) == mp
+        np.testing.assert_array_equal(g.getPressureForceAtFace(fi),
+                                      (-geo[fi, :3] * geo[fi, 3]) * mp)
