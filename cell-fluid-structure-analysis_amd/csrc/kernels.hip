@@ -1411,7 +1411,8 @@ template void cg_s_init<1>(hipStream_t, KrylovState *, const double *, int);
 template void cg_s_init<3>(hipStream_t, KrylovState *, const double *, int);
 
 // ===========================================================================
-// Element assembly into the fixed patterns (graph-coloured, atomic-free)
+// Element assembly into the fixed patterns: row gather (default) and
+// graph-coloured, atomic-free element scatter
 // ===========================================================================
 __device__ __forceinline__ int find_slot(const int *__restrict__ rowptr,
                                          const int *__restrict__ col, int row, int c) {

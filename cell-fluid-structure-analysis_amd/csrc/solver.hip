@@ -291,8 +291,11 @@ void NSSolver::update_mesh(const double *coords, const uint8_t *p1_bc, const uin
   lhs_dirty = lbc_dirty = true;
 }
 
-// Coloured, atomic-free assembly of M, K (P2 clique pattern) and L, Gx/y/z
-// (P1 pattern); reference build order G, M, L, K (NavierStokes.hpp:97-102).
+// Assembly of M, K (P2 clique pattern) and L, Gx/y/z (P1 pattern) into the fixed
+// CSR patterns; reference build order G, M, L, K (NavierStokes.hpp:97-102).
+// assembly 0 (default): row gather through the slot map, six rows per wavefront
+// with LDS accumulators, bit-exact; 2: row gather with one thread per row,
+// bit-exact; 1: graph-coloured, atomic-free element scatter (colour-order sums).
 void NSSolver::assemble() {
   const double inv_mu = 1.0 / viscosity;  // NavierStokes.cpp:153
   if (opt.assembly != 1 && d_amap.n == 0) {
