@@ -128,21 +128,32 @@ def cpu_whole_steps(P, O, workload, steps):
             "pressure_dofs": sz["nv"], "last_iters": o.stats()}
 
 
-def cpu_baseline(O, g, fp64_stats, sz, pressure_method):
+def jacobi_pressure_iters(g, args):
+    """Jacobi-CG iterations of the last step's pressure system (rhs = -div(u*)/dt,
+    the outlet pressure being 0) on the GPU: what the oracle's CG would take,
+    when the timed steps used the AMG-preconditioned solve."""
+    rhs = -g.computeDivergence(g.getVelocityStar()) / args.dt
+    _, r = g.krylovSolve("Lbc", rhs, method=1, tol=args.pressure_tol, maxit=20000)
+    return r["iters"][0]
+
+
+def cpu_baseline(O, g, fp64_stats, sz, pressure_method, pressure_iters=None):
     """Eigen-form BiCGSTAB / Jacobi CG of the oracle (one core, inner products
     in Eigen's packet order) timed for a few iterations on the workload's own
     momentum and pressure operators, then scaled by the iteration counts of
     the timed FP64 GPU steps (the oracle takes the same counts: see
-    tests/test_gpu_parity.py).  An extrapolation: a whole oracle step at 96^3
-    takes minutes; whole steps are timed on the cylinders (cpu_whole_steps)."""
+    tests/test_gpu_parity.py; with the AMG pressure solve on the GPU, the
+    pressure count is the Jacobi-CG count of the same system, pressure_iters).
+    An extrapolation: a whole oracle step at 96^3 takes minutes; whole steps
+    are timed on the cylinders (cpu_whole_steps)."""
     O.set_dot_mode(1)
     try:
-        return _cpu_baseline(O, g, fp64_stats, sz, pressure_method)
+        return _cpu_baseline(O, g, fp64_stats, sz, pressure_method, pressure_iters)
     finally:
         O.set_dot_mode(0)
 
 
-def _cpu_baseline(O, g, fp64_stats, sz, pressure_method):
+def _cpu_baseline(O, g, fp64_stats, sz, pressure_method, pressure_iters=None):
     rp, col, val = g.getMatrix("lhs")
     rng = np.random.default_rng(0)
     k_mom = 30
@@ -158,7 +169,10 @@ def _cpu_baseline(O, g, fp64_stats, sz, pressure_method):
     O.csr_krylov(rp, col, val, bp, method=pressure_method, tol=0.0, maxit=k_p)
     t_p = (time.perf_counter() - t0) / k_p
     mom = np.mean([sum(s["momentum_iters"]) for s in fp64_stats])
-    pre = np.mean([sum(s["pressure_iters"]) for s in fp64_stats])
+    if pressure_iters is None:
+        pre = np.mean([sum(s["pressure_iters"]) for s in fp64_stats])
+    else:
+        pre = float(np.mean([len(s["pressure_iters"]) for s in fp64_stats]) * pressure_iters)
     t_step = mom * t_mom + pre * t_p
     return {"value": 1.0 / t_step, "unit": "PISO steps/s", "cores": 1, "kind": "port",
             "method": "extrapolated from timed Krylov iterations",
@@ -171,6 +185,8 @@ def _cpu_baseline(O, g, fp64_stats, sz, pressure_method):
                        f"not counted, which favours the CPU"),
             "cpu": cpu_model(), "nproc": os.cpu_count()}
 
+
+PRESSURE_SOLVERS = {0: "bicgstab_jacobi", 1: "cg_jacobi", 2: "cg_amg"}
 
 # FP64 variants: (name, momentum solver); the reference's BiCGSTAB first
 FP64 = [("fp64", 0), ("fp64_cg", 1)]
@@ -323,7 +339,8 @@ def main():
     ap.add_argument("--stop-rule", type=int, default=1,
                     help="0: |r| <= tol |b| (reference; its 1e10 inlet-penalty rows dominate "
                          "|b|, so the interior is not solved to tol), 1: Jacobi-scaled")
-    ap.add_argument("--pressure-solver", type=int, default=1, help="0 BiCGSTAB, 1 Jacobi CG")
+    ap.add_argument("--pressure-solver", type=int, default=2, choices=[0, 1, 2],
+                    help="0 BiCGSTAB, 1 Jacobi CG, 2 AMG-preconditioned CG")
     ap.add_argument("--adjoint-tol", type=float, default=1e-14,
                     help="Krylov tolerance (Jacobi-scaled rule) of the adjoint workload")
     ap.add_argument("--resolve-correctors", action="store_true",
@@ -514,7 +531,10 @@ def main():
     if not args.skip_cpu:
         from oracle import oracle as O
         t0 = time.perf_counter()
-        cpu = cpu_baseline(O, g64, st64, sz, args.pressure_solver)
+        if args.pressure_solver == 2:
+            cpu = cpu_baseline(O, g64, st64, sz, 1, jacobi_pressure_iters(g64, args))
+        else:
+            cpu = cpu_baseline(O, g64, st64, sz, args.pressure_solver)
         log(f"cpu baseline in {time.perf_counter() - t0:.1f}s")
         if not args.skip_cpu_steps:
             t0 = time.perf_counter()
@@ -536,7 +556,7 @@ def main():
                       "inlet_peak": U, "reynolds": None if cyl else args.re, "dt": args.dt,
                       "pressure_tol": args.pressure_tol, "momentum_tol": args.momentum_tol,
                       "stop_rule": "jacobi_scaled" if args.stop_rule else "reference",
-                      "pressure_solver": "cg_jacobi" if args.pressure_solver else "bicgstab_jacobi",
+                      "pressure_solver": PRESSURE_SOLVERS[args.pressure_solver],
                       "fp64_momentum_solvers": {k: v["momentum_solver"] for k, v in res.items()
                                                 if k in dict(FP64)},
                       "headline": primary,
@@ -545,6 +565,8 @@ def main():
                                            "same bits)") if args.corrections > 1 else None,
                       "velocity_error_bound": VEL_ERR_MAX, "parallelism": "single GPU"},
            "dofs_per_s": value * (3 * sz["nu"] + sz["nv"]),
+           "pressure_amg": ({k: v for k, v in g64.amgInfo().items() if k != "builds"}
+                            if args.pressure_solver == 2 else None),
            "host_readback_ms_per_step": host_copy_ms,
            "host_readback_pageable_ms_per_step": host_copy_pageable_ms,
            "steps_per_s_with_host_readback": 1e3 / (res[primary]["ms_per_step"] + host_copy_ms),

@@ -40,7 +40,8 @@ class SolverOptions(C.Structure):
                 ("stop_rule", C.c_int), ("assembly", C.c_int),
                 ("spmv_layout", C.c_int), ("momentum_solver", C.c_int),
                 ("check_every_1", C.c_int), ("graphs", C.c_int), ("initial_guess", C.c_int),
-                ("reuse_pressure_correction", C.c_int), ("restart_scale", C.c_double)]
+                ("reuse_pressure_correction", C.c_int), ("restart_scale", C.c_double),
+                ("amg_reuse", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -177,6 +178,20 @@ _sig("cfsa_ns_apply_matrix_noalloc", C.c_int, _p, C.c_int, _f64p, _f64p, C.c_int
 _sig("cfsa_ns_krylov_solve", C.c_int, _p, C.c_int, C.c_int, C.c_int, _f64p, _f64p, C.c_double,
      C.c_int, C.c_int, C.POINTER(SolveResult))
 _sig("cfsa_ns_time_momentum_spmv", C.c_int, _p, C.c_int, C.c_int, C.POINTER(C.c_double))
+_sig("cfsa_ns_amg_apply", C.c_int, _p, _f64p, _f64p)
+_sig("cfsa_ns_amg_info", C.c_int, _p, C.c_int, _p, _p, C.POINTER(C.c_double),
+     C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("cfsa_amg_build", C.c_int, C.c_int64, _i32p, _i32p, _f64p, C.c_double, C.c_int,
+     C.POINTER(_p))
+_sig("cfsa_amg_free", C.c_int, _p)
+_sig("cfsa_amg_num_levels", C.c_int, _p, C.POINTER(C.c_int), C.POINTER(C.c_int64),
+     C.POINTER(C.c_int))
+_sig("cfsa_amg_level_size", C.c_int, _p, C.c_int, C.c_int, C.POINTER(C.c_int64),
+     C.POINTER(C.c_int64), C.POINTER(C.c_int64))
+_sig("cfsa_amg_level_get", C.c_int, _p, C.c_int, C.c_int, _i32p, _i32p, _f64p)
+_sig("cfsa_amg_level_params", C.c_int, _p, C.c_int, C.POINTER(C.c_double),
+     C.POINTER(C.c_double), _f64p, _p)
+_sig("cfsa_amg_coarse_inverse", C.c_int, _p, _f64p)
 _sig("cfsa_ns_get_stream", C.c_int, _p, C.POINTER(_p))
 for _n in ["pressure_forces", "shear_stress", "total_fluid_forces"]:
     _sig(f"cfsa_ns_compute_{_n}", C.c_int, _p, _f64p)
@@ -634,6 +649,23 @@ class NavierStokesSolver:
                    "restarts": list(r.restarts)[:nrhs], "refinements": r.refinements,
                    "ok": bool(r.ok)}
 
+    def amgApply(self, r):
+        """One V-cycle z = M^-1 r of the pressure operator's AMG hierarchy."""
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty_like(r)
+        _check(_lib.cfsa_ns_amg_apply(self._h, r, z))
+        return z
+
+    def amgInfo(self):
+        cap = 32
+        rows, nnz = (C.c_int64 * cap)(), (C.c_int64 * cap)()
+        ms, builds, levels = C.c_double(), C.c_int(), C.c_int()
+        _check(_lib.cfsa_ns_amg_info(self._h, cap, C.cast(rows, _p), C.cast(nnz, _p),
+                                     C.byref(ms), C.byref(builds), C.byref(levels)))
+        L = levels.value
+        return {"levels": L, "rows": list(rows)[:L], "nnz": list(nnz)[:L],
+                "setup_ms": ms.value, "builds": builds.value}
+
     def timeMomentumSpmv(self, reps=20, precision=0):
         out = C.c_double()
         _check(_lib.cfsa_ns_time_momentum_spmv(self._h, int(reps), int(precision), C.byref(out)))
@@ -703,3 +735,55 @@ class NavierStokesSolver:
         _check(_lib.cfsa_ns_adjoint_gradient(self._h, int(nsteps), len(coef), basis, coef,
                                              C.byref(J), dc, C.byref(dmu)))
         return J.value, dc, dmu.value
+
+
+class HostAmg:
+    """The library's smoothed-aggregation set-up run on a host CSR matrix
+    (cfsa_amg_build): levels, operators, prolongations and coefficients, for
+    inspection and tests."""
+
+    def __init__(self, rowptr, col, val, theta=0.08, max_coarse=400):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        h = _p()
+        _check(_lib.cfsa_amg_build(len(rowptr) - 1, rowptr, col, val, float(theta),
+                                   int(max_coarse), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.cfsa_amg_free(self._h)
+            self._h = None
+
+    def levels(self):
+        L, nc, cj = C.c_int(), C.c_int64(), C.c_int()
+        _check(_lib.cfsa_amg_num_levels(self._h, C.byref(L), C.byref(nc), C.byref(cj)))
+        return L.value, nc.value, bool(cj.value)
+
+    def matrix(self, level, which):
+        """which: 'A', 'P' or 'R'; returns (rowptr, col, val, shape)."""
+        w = {"A": 0, "P": 1, "R": 2}[which]
+        r, c, z = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(_lib.cfsa_amg_level_size(self._h, int(level), w, C.byref(r), C.byref(c),
+                                        C.byref(z)))
+        rp = np.empty(r.value + 1, np.int32)
+        ci = np.empty(z.value, np.int32)
+        v = np.empty(z.value, np.float64)
+        _check(_lib.cfsa_amg_level_get(self._h, int(level), w, rp, ci, v))
+        return rp, ci, v, (r.value, c.value)
+
+    def params(self, level):
+        rho, lmax = C.c_double(), C.c_double()
+        coef = np.empty(3, np.float64)
+        n = self.matrix(level, "A")[3][0]
+        agg = np.full(n, -2, np.int32)
+        _check(_lib.cfsa_amg_level_params(self._h, int(level), C.byref(rho), C.byref(lmax), coef,
+                                          agg.ctypes.data_as(_p)))
+        return {"rho": rho.value, "lmax": lmax.value, "coef": coef, "agg": agg}
+
+    def coarseInverse(self):
+        _, nc, _ = self.levels()
+        inv = np.empty(nc * nc, np.float64)
+        _check(_lib.cfsa_amg_coarse_inverse(self._h, inv))
+        return inv.reshape(nc, nc)

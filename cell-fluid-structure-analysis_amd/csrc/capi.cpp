@@ -7,6 +7,7 @@
 #include <memory>
 #include <string>
 
+#include "amg.hpp"
 #include "mesh.hpp"
 #include "solver.hpp"
 #include "vtk.hpp"
@@ -15,6 +16,9 @@
 struct cfsa_mesh {
   cfsa::Mesh m;
   uint64_t version = 0;  // bumped whenever labels or vertices change
+};
+struct cfsa_amg {
+  cfsa::AmgHierarchy H;
 };
 struct cfsa_ns {
   cfsa::NSSolver s;
@@ -73,6 +77,7 @@ cfsa::SolverOptions to_opts(const cfsa_solver_options &o) {
   s.initial_guess = o.initial_guess;
   s.reuse_correction = o.reuse_pressure_correction;
   s.restart_scale = o.restart_scale;
+  s.amg_reuse = o.amg_reuse;
   return s;
 }
 
@@ -456,13 +461,13 @@ int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt) {
   CHECK_PTR(h);
   CHECK_PTR(opt);
   if (opt->max_iter < 0 || opt->check_every < 1 || opt->check_every_1 < 1 || opt->precision < 0 || opt->precision > 2 ||
-      opt->pressure_solver < 0 || opt->pressure_solver > 1 || !(opt->momentum_tol >= 0) ||
+      opt->pressure_solver < 0 || opt->pressure_solver > 2 || !(opt->momentum_tol >= 0) ||
       !(opt->pressure_tol >= 0) || !(opt->inner_tol > 0) || opt->max_refine < 0 ||
       opt->stop_rule < 0 || opt->stop_rule > 1 || opt->assembly < 0 || opt->assembly > 2 ||
       opt->spmv_layout < 0 || opt->spmv_layout > 2 || opt->momentum_solver < 0 ||
       opt->momentum_solver > 1 || opt->graphs < 0 || opt->graphs > 1 ||
       opt->initial_guess < 0 || opt->initial_guess > 1 || opt->reuse_pressure_correction < 0 ||
-      opt->reuse_pressure_correction > 1 || !(opt->restart_scale > 0))
+      opt->reuse_pressure_correction > 1 || !(opt->restart_scale > 0) || opt->amg_reuse < 0)
     return fail(CFSA_ERR_INVALID, "invalid solver options");
   h->s.opt = to_opts(*opt);
   return CFSA_OK;
@@ -489,6 +494,7 @@ int cfsa_ns_get_options(const cfsa_ns *h, cfsa_solver_options *o) {
   o->initial_guess = s.initial_guess;
   o->reuse_pressure_correction = s.reuse_correction;
   o->restart_scale = s.restart_scale;
+  o->amg_reuse = s.amg_reuse;
   return CFSA_OK;
 }
 
@@ -706,6 +712,122 @@ int cfsa_ns_krylov_solve(cfsa_ns *h, int which, int method, int nrhs, const doub
     if (res) to_result(r, res);
     return CFSA_OK;
   });
+}
+int cfsa_ns_amg_apply(cfsa_ns *h, const double *r, double *z) {
+  CHECK_PTR(h);
+  CHECK_PTR(r);
+  CHECK_PTR(z);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.amg_apply(r, z);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_amg_info(cfsa_ns *h, int cap, int64_t *rows, int64_t *nnz, double *setup_ms,
+                     int *builds, int *levels) {
+  CHECK_PTR(h);
+  if (cap < 0) return fail(CFSA_ERR_INVALID, "cap < 0");
+  return guard([&]() -> int {
+    const int L = h->s.amg_info(cap, rows, nnz, setup_ms, builds);
+    if (levels) *levels = L;
+    return CFSA_OK;
+  });
+}
+int cfsa_amg_build(int64_t n, const int32_t *rowptr, const int32_t *col, const double *val,
+                   double theta, int max_coarse, cfsa_amg **out) {
+  CHECK_PTR(rowptr);
+  CHECK_PTR(out);
+  *out = nullptr;
+  if (n < 1 || n >= INT32_MAX) return fail(CFSA_ERR_INVALID, "matrix size out of range");
+  if (!(theta >= 0) || max_coarse < 1) return fail(CFSA_ERR_INVALID, "invalid AMG parameters");
+  return guard([&]() -> int {
+    cfsa::HostCsr A;
+    A.n = A.m = n;
+    A.rp.assign(rowptr, rowptr + n + 1);
+    if (A.rp[0] != 0) throw std::invalid_argument("rowptr[0] must be 0");
+    for (int64_t i = 0; i < n; ++i)
+      if (A.rp[(size_t)i + 1] < A.rp[(size_t)i]) throw std::invalid_argument("rowptr decreases");
+    const int64_t nnz = A.rp[(size_t)n];
+    if (nnz > 0 && (!col || !val)) throw std::invalid_argument("null column or value array");
+    A.ci.assign(col, col + nnz);
+    A.v.assign(val, val + nnz);
+    for (int64_t i = 0; i < n; ++i)
+      for (int32_t k = A.rp[(size_t)i]; k < A.rp[(size_t)i + 1]; ++k) {
+        const int32_t j = A.ci[(size_t)k];
+        if (j < 0 || j >= n || (k > A.rp[(size_t)i] && j <= A.ci[(size_t)k - 1]))
+          throw std::invalid_argument("columns must be in range and ascending per row");
+      }
+    cfsa::AmgParams prm;
+    prm.theta = theta;
+    prm.max_coarse = max_coarse;
+    auto h = std::make_unique<cfsa_amg>();
+    h->H = cfsa::amg_build(A, prm);
+    *out = h.release();
+    return CFSA_OK;
+  });
+}
+int cfsa_amg_free(cfsa_amg *h) {
+  delete h;
+  return CFSA_OK;
+}
+namespace {
+const cfsa::HostCsr *amg_matrix(const cfsa_amg *h, int level, int which) {
+  if (level < 0 || level >= (int)h->H.levels.size()) return nullptr;
+  const cfsa::AmgLevel &L = h->H.levels[(size_t)level];
+  if (which == 0) return &L.A;
+  if (level == (int)h->H.levels.size() - 1) return nullptr;
+  return which == 1 ? &L.P : which == 2 ? &L.R : nullptr;
+}
+}  // namespace
+int cfsa_amg_num_levels(const cfsa_amg *h, int *levels, int64_t *coarse_n, int *coarse_jacobi) {
+  CHECK_PTR(h);
+  if (levels) *levels = (int)h->H.levels.size();
+  if (coarse_n) *coarse_n = h->H.coarse_n;
+  if (coarse_jacobi) *coarse_jacobi = h->H.coarse_jacobi ? 1 : 0;
+  return CFSA_OK;
+}
+int cfsa_amg_level_size(const cfsa_amg *h, int level, int which, int64_t *rows, int64_t *cols,
+                        int64_t *nnz) {
+  CHECK_PTR(h);
+  const cfsa::HostCsr *M = amg_matrix(h, level, which);
+  if (!M) return fail(CFSA_ERR_INVALID, "no such AMG level matrix");
+  if (rows) *rows = M->n;
+  if (cols) *cols = M->m;
+  if (nnz) *nnz = M->nnz();
+  return CFSA_OK;
+}
+int cfsa_amg_level_get(const cfsa_amg *h, int level, int which, int32_t *rowptr, int32_t *col,
+                       double *val) {
+  CHECK_PTR(h);
+  const cfsa::HostCsr *M = amg_matrix(h, level, which);
+  if (!M) return fail(CFSA_ERR_INVALID, "no such AMG level matrix");
+  if (rowptr) std::memcpy(rowptr, M->rp.data(), M->rp.size() * sizeof(int32_t));
+  if (col) std::memcpy(col, M->ci.data(), M->ci.size() * sizeof(int32_t));
+  if (val) std::memcpy(val, M->v.data(), M->v.size() * sizeof(double));
+  return CFSA_OK;
+}
+int cfsa_amg_level_params(const cfsa_amg *h, int level, double *rho, double *lmax,
+                          double coef[3], int32_t *agg) {
+  CHECK_PTR(h);
+  if (level < 0 || level >= (int)h->H.levels.size())
+    return fail(CFSA_ERR_INVALID, "no such AMG level");
+  const cfsa::AmgLevel &L = h->H.levels[(size_t)level];
+  if (rho) *rho = L.rho;
+  if (lmax) *lmax = L.lmax;
+  if (coef) {
+    coef[0] = L.c0;
+    coef[1] = L.c1;
+    coef[2] = L.c2;
+  }
+  if (agg && !L.agg.empty()) std::memcpy(agg, L.agg.data(), L.agg.size() * sizeof(int32_t));
+  return CFSA_OK;
+}
+int cfsa_amg_coarse_inverse(const cfsa_amg *h, double *inv) {
+  CHECK_PTR(h);
+  CHECK_PTR(inv);
+  if (h->H.coarse_jacobi) return fail(CFSA_ERR_INVALID, "the coarsest level has no dense inverse");
+  std::memcpy(inv, h->H.coarse_inv.data(), h->H.coarse_inv.size() * sizeof(double));
+  return CFSA_OK;
 }
 int cfsa_ns_time_momentum_spmv(cfsa_ns *h, int reps, int precision, double *ms) {
   CHECK_PTR(h);

@@ -2252,4 +2252,399 @@ void convert(hipStream_t s, int64_t n, const TI *in, TO *out) {
 template void convert<float, double>(hipStream_t, int64_t, const double *, float *);
 template void convert<double, float>(hipStream_t, int64_t, const float *, double *);
 
+// ===========================================================================
+// AMG-preconditioned CG for the pressure system (pressure_solver = 2): the
+// V-cycle of a smoothed-aggregation hierarchy (set up on the host, amg.cpp)
+// with degree-2 Chebyshev smoothing, and the CG iteration around it.  Every
+// smoothing step is one SpMV whose epilogue applies the polynomial update, so a
+// level costs four operator products plus one restriction and one
+// prolongation.  All kernels take the solve's `active` flag as their mask and
+// exit at once after the solve has stopped.
+// ===========================================================================
+struct OpAmgBase : Mask1Base {
+  const int *__restrict__ col;
+  const double *__restrict__ val;
+  const double *__restrict__ x;  // the gathered vector
+  struct Frag {
+    int j;
+    double a;
+  };
+  __device__ Frag fetch(int k) const { return Frag{col[k], val[k]}; }
+  __device__ void prod(const Frag &f, double *p) const { p[0] = f.a * x[f.j]; }
+};
+
+// pre-smoothing from x0 = 0, second step: Y = X + c1 X + c2 D^-1 (b - A X)
+// (the first step X = c0 D^-1 b is fused into the kernel that produced b)
+struct OpChebPre : OpAmgBase {
+  const double *__restrict__ b;
+  const double *__restrict__ dinv;
+  double *__restrict__ y;
+  double c1, c2;
+  struct Pre {
+    double b, x, d;
+  };
+  __device__ Pre pre(int row, int) const { return Pre{b[row], x[row], dinv[row]}; }
+  __device__ void epi(int row, int, double acc, const Pre &pr, DD *) const {
+    y[row] = (pr.x + c1 * pr.x) + c2 * (pr.d * (pr.b - acc));
+  }
+};
+
+// r = b - A Y
+struct OpAmgResidual : OpAmgBase {
+  const double *__restrict__ b;
+  double *__restrict__ r;
+  struct Pre {
+    double b;
+  };
+  __device__ Pre pre(int row, int) const { return Pre{b[row]}; }
+  __device__ void epi(int row, int, double acc, const Pre &pr, DD *) const { r[row] = pr.b - acc; }
+};
+
+// b_c = R r; X_c = c0 D_c^-1 b_c (first Chebyshev step of the coarse level)
+// unless the coarse level is solved directly (xc == nullptr)
+struct OpRestrict : OpAmgBase {
+  const double *__restrict__ dinvc;
+  double *__restrict__ bc;
+  double *__restrict__ xc;
+  double c0;
+  struct Pre {
+    double d;
+  };
+  __device__ Pre pre(int row, int) const { return Pre{xc ? dinvc[row] : 0.0}; }
+  __device__ void epi(int row, int, double acc, const Pre &pr, DD *) const {
+    bc[row] = acc;
+    if (xc) xc[row] = c0 * (pr.d * acc);
+  }
+};
+
+// Y += P e_c
+struct OpProlong : OpAmgBase {
+  double *__restrict__ y;
+  struct Pre {
+    double y;
+  };
+  __device__ Pre pre(int row, int) const { return Pre{y[row]}; }
+  __device__ void epi(int row, int, double acc, const Pre &pr, DD *) const { y[row] = pr.y + acc; }
+};
+
+// post-smoothing from x0 = Y, first step: X = Y + c0 D^-1 (b - A Y)
+struct OpChebPost1 : OpAmgBase {
+  const double *__restrict__ b;
+  const double *__restrict__ dinv;
+  double *__restrict__ xo;
+  double c0;
+  struct Pre {
+    double b, y, d;
+  };
+  __device__ Pre pre(int row, int) const { return Pre{b[row], x[row], dinv[row]}; }
+  __device__ void epi(int row, int, double acc, const Pre &pr, DD *) const {
+    xo[row] = pr.y + c0 * (pr.d * (pr.b - acc));
+  }
+};
+
+// second step: Y = X + c1 (X - Y) + c2 D^-1 (b - A X); on the finest level the
+// pairs of b . Y (= r . z of the CG iteration) are emitted
+struct OpChebPost2 : OpAmgBase {
+  const double *__restrict__ b;
+  const double *__restrict__ dinv;
+  double *__restrict__ y;
+  double c1, c2;
+  struct Pre {
+    double b, x, y, d;
+  };
+  __device__ Pre pre(int row, int) const { return Pre{b[row], x[row], y[row], dinv[row]}; }
+  __device__ void epi(int row, int, double acc, const Pre &pr, DD *d) const {
+    const double z = (pr.x + c1 * (pr.x - pr.y)) + c2 * (pr.d * (pr.b - acc));
+    y[row] = z;
+    dd_fma(d[0], pr.b, z);
+  }
+};
+
+// coarsest level: Y = C b with the dense (pseudo-)inverse C, one wavefront per
+// row, lane-strided products and a fixed shuffle tree
+__global__ __launch_bounds__(kBlock) void k_amg_dense(int nc, const double *__restrict__ cinv,
+                                                      const double *__restrict__ b,
+                                                      double *__restrict__ y,
+                                                      const int *__restrict__ mask) {
+  if (mask && !mask[0]) return;
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (row >= nc) return;
+  double s = 0.0;
+  for (int j = lane; j < nc; j += 64) s += cinv[(size_t)row * nc + j] * b[j];
+#pragma unroll
+  for (int st = 32; st > 0; st >>= 1) s += __shfl_down(s, st, 64);
+  if (lane == 0) y[row] = s;
+}
+
+// coarsest level without a dense inverse: one Jacobi step Y = D^-1 b
+__global__ __launch_bounds__(kBlock) void k_amg_jacobi(int n, const double *__restrict__ dinv,
+                                                       const double *__restrict__ b,
+                                                       double *__restrict__ y,
+                                                       const int *__restrict__ mask) {
+  if (mask && !mask[0]) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    y[i] = dinv[i] * b[i];
+}
+
+// pairs of b . y (a one-level hierarchy has no smoothing SpMV to carry them)
+__global__ __launch_bounds__(kBlock) void k_amg_dot(int n, const double *__restrict__ b,
+                                                    const double *__restrict__ y,
+                                                    const int *__restrict__ mask,
+                                                    double *__restrict__ partials) {
+  if (mask && !mask[0]) return;
+  DD acc[1];
+  dd_zero(acc[0]);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    dd_fma(acc[0], b[i], y[i]);
+  block_partials_binned<1, 1>(acc, partials);
+}
+
+template <class Op>
+static Op amg_op(const DevCsr &, const double *val, const double *x, const int *mask) {
+  Op op{};
+  op.mask = mask;
+  op.val = val;
+  op.x = x;
+  return op;
+}
+
+static void amg_level(hipStream_t s, const AmgLevelView *lv, int nlev, int l, const double *cinv,
+                      int nc, bool cjac, const int *mask, double *partials, int *grid) {
+  const AmgLevelView &L = lv[l];
+  const bool top = l == 0;
+  if (l == nlev - 1) {
+    if (cjac)
+      hipLaunchKernelGGL(k_amg_jacobi, dim3(vec_grid(L.n)), dim3(kBlock), 0, s, L.n, L.dinv, L.b,
+                         L.y, mask);
+    else
+      hipLaunchKernelGGL(k_amg_dense, dim3(div_up(nc, kBlock / 64)), dim3(kBlock), 0, s, nc,
+                         cinv, L.b, L.y, mask);
+    CFSA_LAUNCH_CHECK();
+    if (top) {
+      const int G = vec_grid(L.n);
+      *grid = G;
+      hipLaunchKernelGGL(k_amg_dot, dim3(G), dim3(kBlock), 0, s, L.n, L.b, L.y, mask, partials);
+      CFSA_LAUNCH_CHECK();
+    }
+    return;
+  }
+  const AmgLevelView &C = lv[l + 1];
+  {
+    OpChebPre op = amg_op<OpChebPre>(L.A, L.Av, L.x, mask);
+    op.col = L.A.col;
+    op.b = L.b;
+    op.dinv = L.dinv;
+    op.y = L.y;
+    op.c1 = L.c1;
+    op.c2 = L.c2;
+    launch_spmv<double, 1, 0>(s, L.A, op, nullptr);
+  }
+  {
+    OpAmgResidual op = amg_op<OpAmgResidual>(L.A, L.Av, L.y, mask);
+    op.col = L.A.col;
+    op.b = L.b;
+    op.r = L.r;
+    launch_spmv<double, 1, 0>(s, L.A, op, nullptr);
+  }
+  {
+    const bool cdirect = l + 1 == nlev - 1;
+    OpRestrict op = amg_op<OpRestrict>(L.R, L.Rv, L.r, mask);
+    op.col = L.R.col;
+    op.dinvc = C.dinv;
+    op.bc = C.b;
+    op.xc = cdirect ? nullptr : C.x;
+    op.c0 = C.c0;
+    launch_spmv<double, 1, 0>(s, L.R, op, nullptr);
+  }
+  amg_level(s, lv, nlev, l + 1, cinv, nc, cjac, mask, partials, grid);
+  {
+    OpProlong op = amg_op<OpProlong>(L.P, L.Pv, C.y, mask);
+    op.col = L.P.col;
+    op.y = L.y;
+    launch_spmv<double, 1, 0>(s, L.P, op, nullptr);
+  }
+  {
+    OpChebPost1 op = amg_op<OpChebPost1>(L.A, L.Av, L.y, mask);
+    op.col = L.A.col;
+    op.b = L.b;
+    op.dinv = L.dinv;
+    op.xo = L.x;
+    op.c0 = L.c0;
+    launch_spmv<double, 1, 0>(s, L.A, op, nullptr);
+  }
+  {
+    OpChebPost2 op = amg_op<OpChebPost2>(L.A, L.Av, L.x, mask);
+    op.col = L.A.col;
+    op.b = L.b;
+    op.dinv = L.dinv;
+    op.y = L.y;
+    op.c1 = L.c1;
+    op.c2 = L.c2;
+    if (top)
+      *grid = launch_spmv<double, 1, 1>(s, L.A, op, partials);
+    else
+      launch_spmv<double, 1, 0>(s, L.A, op, nullptr);
+  }
+}
+
+void amg_vcycle(hipStream_t s, const AmgLevelView *lv, int nlev, const double *cinv, int nc,
+                bool coarse_jacobi, const int *mask, double *partials, int *grid) {
+  if (nlev < 1) throw std::logic_error("AMG V-cycle without levels");
+  if (!coarse_jacobi && nc != lv[nlev - 1].n)
+    throw std::logic_error("AMG coarse inverse does not match the coarsest level");
+  amg_level(s, lv, nlev, 0, cinv, nc, coarse_jacobi, mask, partials, grid);
+}
+
+// x = 0, r = b, X0 = c0 D^-1 b (first Chebyshev step of the finest level), and
+// the pairs of the stopping norm |b|^2 (or |D^-1 b|^2)
+__global__ __launch_bounds__(kBlock) void k_amg_cg_init(int n, const double *__restrict__ b,
+                                                        const double *__restrict__ invd,
+                                                        double c0, double *__restrict__ x,
+                                                        double *__restrict__ r,
+                                                        double *__restrict__ x0, int scaled,
+                                                        double *__restrict__ partials) {
+  DD acc[1];
+  dd_zero(acc[0]);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double bi = b[i], d = invd[i];
+    x[i] = 0.0;
+    r[i] = bi;
+    x0[i] = c0 * (d * bi);
+    const double sn = scaled ? d * bi : bi;
+    dd_fma(acc[0], sn, sn);
+  }
+  block_partials_binned<1, 1>(acc, partials);
+}
+
+// state of the solve from |b|^2 (pa) and rho = b . z0 (pc); p = z0 is copied by
+// the host
+__global__ void k_amg_cg_s_init(KrylovState *st, const double *pa, int Ga, const double *pc,
+                                int Gc) {
+  __shared__ double bb[1], rz[1];
+  reduce_final<1>(pa, Ga, bb);
+  reduce_final<1>(pc, Gc, rz);
+  if (threadIdx.x != 0) return;
+  const double b2 = bb[0];
+  st->rhs_sq[0] = b2;
+  st->iters[0] = 0;
+  st->restarts[0] = 0;
+  st->active[0] = 0;
+  st->any_active = 0;
+  for (int c = 1; c < 3; ++c) {
+    st->status[c] = KS_DONE;
+    st->active[c] = 0;
+  }
+  if (b2 == 0.0) {
+    st->status[0] = KS_DONE;
+    st->err[0] = 0.0;
+    return;
+  }
+  const double thr = fmax(st->tol * st->tol * b2, DBL_MIN);
+  st->tol2[0] = thr;
+  st->rr[0] = b2;
+  if (b2 < thr) {
+    st->status[0] = KS_DONE;
+    st->err[0] = 1.0;
+    return;
+  }
+  st->rho[0] = rz[0];
+  st->rho_s[0][0] = rz[0];
+  st->it_s[0][0] = 0;
+  st->status[0] = st->maxit > 0 ? KS_RUNNING : KS_DONE;
+  st->err[0] = 1.0;
+  st->active[0] = st->maxit > 0;
+  st->any_active = st->active[0];
+}
+
+// alpha = rho / p.Ap; r -= alpha q; X0 = c0 D^-1 r for the V-cycle; pairs of
+// the stopping norm.  The snapshot of `active` is what k_amg_cg_p updates x by.
+__global__ __launch_bounds__(kBlock) void k_amg_cg_xr(int n, KrylovState *__restrict__ st,
+                                                      const double *__restrict__ pq_partials,
+                                                      int Gq, int par,
+                                                      const double *__restrict__ q,
+                                                      const double *__restrict__ invd, double c0,
+                                                      double *__restrict__ r,
+                                                      double *__restrict__ x0, int scaled,
+                                                      double *__restrict__ partials) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->act_s[par][0] = st->active[0];
+  if (!st->active[0]) return;  // grid-uniform; the pairs are not read then
+  double pq[1];
+  block_reduce_pairs<1>(pq_partials, Gq, pq);
+  const double al = st->rho_s[par][0] / pq[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->r0v[0] = pq[0];
+    st->alpha[0] = al;
+  }
+  DD acc[1];
+  dd_zero(acc[0]);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double ri = r[i] - al * q[i];
+    const double d = invd[i];
+    r[i] = ri;
+    x0[i] = c0 * (d * ri);
+    const double sn = scaled ? d * ri : ri;
+    dd_fma(acc[0], sn, sn);
+  }
+  block_partials_binned<1, 1>(acc, partials);
+}
+
+// Every block reduces the stopping pairs (pa) and r . z (pc) and decides as
+// Eigen's conjugate_gradient does (block 0 writes the state); then
+// x += alpha p and, if the solve continues, p = z + beta p.
+__global__ __launch_bounds__(kBlock) void k_amg_cg_p(int n, KrylovState *__restrict__ st,
+                                                     const double *__restrict__ pa, int Ga,
+                                                     const double *__restrict__ pc, int Gc,
+                                                     int par, const double *__restrict__ z,
+                                                     double *__restrict__ p,
+                                                     double *__restrict__ x) {
+  __shared__ double beta_s[1];
+  __shared__ int cont_s[1];
+  if (!st->act_s[par][0]) return;
+  double rn[1], rz[1];
+  block_reduce_pairs<1>(pa, Ga, rn);
+  block_reduce_pairs<1>(pc, Gc, rz);
+  if (threadIdx.x == 0) {
+    const double tot[2] = {rn[0], rz[0]};
+    cg_decide<1>(st, tot, par, blockIdx.x == 0, beta_s, cont_s);
+  }
+  __syncthreads();
+  const double al = st->alpha[0], be = beta_s[0];
+  const bool cont = cont_s[0] != 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double pi = p[i];
+    x[i] = x[i] + al * pi;
+    if (cont) p[i] = z[i] + be * pi;
+  }
+}
+
+void amg_cg_init(hipStream_t s, int n, const double *b, const double *invd, double c0, double *x,
+                 double *r, double *x0, int scaled, double *partials, int *grid) {
+  const int G = vec_grid(n);
+  *grid = G;
+  hipLaunchKernelGGL(k_amg_cg_init, dim3(G), dim3(kBlock), 0, s, n, b, invd, c0, x, r, x0, scaled,
+                     partials);
+  CFSA_LAUNCH_CHECK();
+}
+void amg_cg_s_init(hipStream_t s, KrylovState *st, const double *pa, int Ga, const double *pc,
+                   int Gc) {
+  hipLaunchKernelGGL(k_amg_cg_s_init, dim3(1), dim3(kBlock), 0, s, st, pa, Ga, pc, Gc);
+  CFSA_LAUNCH_CHECK();
+}
+void amg_cg_xr(hipStream_t s, int n, KrylovState *st, const double *pq, int Gq, int par,
+               const double *q, const double *invd, double c0, double *r, double *x0, int scaled,
+               double *partials, int *grid) {
+  const int G = vec_grid(n);
+  *grid = G;
+  hipLaunchKernelGGL(k_amg_cg_xr, dim3(G), dim3(kBlock), 0, s, n, st, pq, Gq, par, q, invd, c0, r,
+                     x0, scaled, partials);
+  CFSA_LAUNCH_CHECK();
+}
+void amg_cg_p(hipStream_t s, int n, KrylovState *st, const double *pa, int Ga, const double *pc,
+              int Gc, int par, const double *z, double *p, double *x) {
+  hipLaunchKernelGGL(k_amg_cg_p, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, st, pa, Ga, pc, Gc,
+                     par, z, p, x);
+  CFSA_LAUNCH_CHECK();
+}
+
 }  // namespace cfsa

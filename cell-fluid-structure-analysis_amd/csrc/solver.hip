@@ -81,6 +81,10 @@ NSSolver::~NSSolver() {
 void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   fields_wait();  // a pending host copy reads the snapshot buffers
   clear_graphs();
+  amg_built_ = false;  // a hierarchy belongs to the previous mesh's operator
+  amg_lbc_changed_ = true;
+  amg_lv_.clear();
+  amg_view_.clear();
   if (m.nv <= 0 || m.nt <= 0 || m.ne < 0 || !m.coords || !m.tets || !m.ten || !m.p1_bc ||
       (m.ne > 0 && !m.p2_bc))
     throw std::invalid_argument("initialize: incomplete mesh arrays");
@@ -440,6 +444,7 @@ void NSSolver::prepare_operators() {
     }
     sellP_ = lp;
     lbc_dirty = false;
+    amg_lbc_changed_ = true;
   }
 }
 
@@ -690,6 +695,61 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
   return res;
 }
 
+// CG preconditioned by one V-cycle of the pressure operator's AMG hierarchy
+// (amg.hpp), Eigen's conjugate_gradient step and stopping rule otherwise.  One
+// iteration: q = A p with p.q; alpha, r -= alpha q and the first smoothing step
+// of the finest level; the V-cycle z = M^-1 r with r.z; the convergence test,
+// x += alpha p and p = z + beta p.  Chunks of four iterations replay as graphs.
+SolveResult NSSolver::pcg_amg(const double *b, double *x, double tol, int maxit) {
+  ensure_amg();
+  hipStream_t s = stream_;
+  const int n = nv;
+  KrylovState *st = d_state.p;
+  const int *act = st->active;
+  KrylovWork<double> &w = w1_;
+  AmgLevelView &top = amg_view_[0];
+  top.A = preA();
+  top.Av = preV();
+  top.dinv = d_invdL.p;
+  top.b = w.r.p;
+  const int L = (int)amg_view_.size();
+  set_state_params(tol, maxit, 1);
+  int Ga = 0, Gb = 0, Gc = 0;
+  amg_cg_init(s, n, b, d_invdL.p, top.c0, x, w.r.p, top.x, opt.stop_scaled, d_part_a.p, &Ga);
+  amg_vcycle(s, amg_view_.data(), L, amg_cinv_.p, amg_nc_, amg_cjac_, nullptr, d_part_c.p, &Gc);
+  amg_cg_s_init(s, st, d_part_a.p, Ga, d_part_c.p, Gc);
+  CFSA_HIP(hipMemcpyAsync(w.p.p, top.y, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  constexpr int chunk = 4;  // even: every chunk starts on parity slot 0
+  const long max_chunks = (long)maxit / chunk + 16;
+  auto body = [&]() {
+    for (int j = 0; j < chunk; ++j) {
+      const int par = j & 1;
+      spmv1_dot<double>(s, top.A, top.Av, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
+      amg_cg_xr(s, n, st, d_part_b.p, Gb, par, w.v.p, d_invdL.p, top.c0, w.r.p, top.x,
+                opt.stop_scaled, d_part_a.p, &Ga);
+      amg_vcycle(s, amg_view_.data(), L, amg_cinv_.p, amg_nc_, amg_cjac_, act, d_part_c.p, &Gc);
+      amg_cg_p(s, n, st, d_part_a.p, Ga, d_part_c.p, Gc, par, top.y, w.p.p, x);
+    }
+  };
+  const std::vector<uintptr_t> key = {
+      3, (uintptr_t)n, (uintptr_t)chunk, (uintptr_t)top.A.rowptr, (uintptr_t)top.A.sptr,
+      (uintptr_t)top.Av, (uintptr_t)d_invdL.p, (uintptr_t)x, (uintptr_t)w.r.p,
+      (uintptr_t)w.p.p, (uintptr_t)w.v.p, (uintptr_t)st, (uintptr_t)d_part_a.p,
+      (uintptr_t)d_part_b.p, (uintptr_t)d_part_c.p, (uintptr_t)opt.stop_scaled,
+      (uintptr_t)amg_builds_, (uintptr_t)top.x, (uintptr_t)top.y};
+  for (long ch = 0;; ++ch) {
+    if (ch > max_chunks) throw std::runtime_error("AMG-PCG driver did not terminate");
+    read_state();
+    if (h_state->status[0] == KS_DONE) break;
+    run_chunk(key, body);
+  }
+  SolveResult res;
+  res.iters[0] = h_state->iters[0];
+  res.err[0] = h_state->err[0];
+  res.ok = h_state->err[0] <= tol;
+  return res;
+}
+
 // FP64 iterative refinement around FP32 inner Krylov solves: the residual
 // b - A x, its norm and the stopping test are FP64; matrix values, vectors and
 // SpMV accumulation of the inner solve are FP32 (inner scalars stay FP64).
@@ -799,6 +859,7 @@ SolveResult NSSolver::solve_momentum(const double *b, double *x) {
 }
 
 SolveResult NSSolver::solve_pressure(const double *b, double *x) {
+  if (opt.pressure_solver == 2) return pcg_amg(b, x, opt.pressure_tol, opt.max_iter);
   if (opt.precision == 1)
     return refine<1>(preA(), preV(), d_invdL.p, d_Lbc32.p, d_invdL32.p, b, x,
                      opt.pressure_tol, opt.max_iter, opt.pressure_solver);
@@ -1151,7 +1212,10 @@ SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double
   db.upload(b, (size_t)n * nrhs);
   dx.alloc((size_t)n * nrhs);
   SolveResult r;
-  if (precision == 1) {
+  if (method == 2) {
+    if (which != 4) throw std::invalid_argument("krylov_solve: method 2 (AMG-PCG) solves L_bc");
+    r = pcg_amg(db.p, dx.p, tol, maxit);
+  } else if (precision == 1) {
     if (nrhs == 3)
       r = refine<3>(A, momV(), d_invdA.p, d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit,
                     method);

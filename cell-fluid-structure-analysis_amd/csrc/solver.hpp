@@ -86,7 +86,8 @@ struct SolverOptions {
   double momentum_tol = 1e-6;  // NavierStokes.cpp:500
   double pressure_tol = 1e-8;  // NavierStokes.cpp:635
   int max_iter = 1000;         // NavierStokes.cpp:499,634
-  int pressure_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG
+  int pressure_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG, 2 AMG-preconditioned
+                               // CG (FP64 in every precision mode)
   int momentum_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG (lhs is SPD)
   int precision = 0;           // 0 FP64, 1 FP32 inner + FP64 refinement, 2 momentum only
   double inner_tol = 1e-4;     // relative tolerance of each FP32 inner solve
@@ -106,6 +107,10 @@ struct SolverOptions {
   int reuse_correction = 1;
   // multiplier of eps^2 in BiCGSTAB's restart test (1 = Eigen's rule)
   double restart_scale = 1.0;
+  // pressure_solver 2: changes of the L_bc values for which the AMG hierarchy is
+  // kept (the finest level always smooths with the current operator); 0
+  // rebuilds it whenever the values change
+  int amg_reuse = 0;
 };
 
 struct SolveResult {
@@ -220,6 +225,12 @@ class NSSolver {
   void apply_matrix_noalloc(int which, const double *x, double *y, int nrhs);
   SolveResult krylov_solve(int which, int method, int nrhs, const double *b, double *x,
                            double tol, int maxit, int precision);
+  // one V-cycle of the pressure operator's AMG hierarchy applied to r (host
+  // arrays of nv values), building the hierarchy if needed
+  void amg_apply(const double *r, double *z);
+  // levels of the AMG hierarchy (0 if none), rows and nonzeros per level (up to
+  // cap entries), set-up time of the last build in ms, builds so far
+  int amg_info(int cap, int64_t *rows, int64_t *nnz, double *setup_ms, int *builds);
   hipStream_t stream() const { return stream_; }
   // Momentum-matrix SpMV launches repeated `reps` times between two events.
   double time_momentum_spmv(int reps, int precision);
@@ -306,6 +317,22 @@ class NSSolver {
   void clear_graphs();
   template <class F>
   void run_chunk(const std::vector<uintptr_t> &key, F &&launch);
+  // AMG hierarchy of the pressure operator (pressure_solver 2; amg_solver.hip)
+  struct AmgDevLevel {
+    DBuf<int> Arp, Aci, Awin, Prp, Pci, Pwin, Rrp, Rci, Rwin;
+    DBuf<double> Av, Pv, Rv, dinv, b, x, y, r;
+  };
+  std::vector<AmgDevLevel> amg_lv_;
+  std::vector<AmgLevelView> amg_view_;
+  DBuf<double> amg_cinv_;
+  int amg_nc_ = 0;
+  bool amg_cjac_ = false, amg_built_ = false, amg_lbc_changed_ = true;
+  int amg_stale_ = 0, amg_builds_ = 0;
+  double amg_setup_ms_ = 0.0;
+  std::vector<double> amg_vals_;  // L_bc values the hierarchy was built from
+  std::vector<int64_t> amg_rows_, amg_nnz_;
+  void ensure_amg();
+  SolveResult pcg_amg(const double *b, double *x, double tol, int maxit);
   // adjoint state
   DBuf<int> d_Ptslot;  // slot of (j, i) for each slot (i, j) of the P1 pattern
   DBuf<double> d_G3T, d_lu, d_lus, d_lb, d_lg, d_lgp, d_lpc, d_lrp, d_lp, d_lp_new, d_contrib,

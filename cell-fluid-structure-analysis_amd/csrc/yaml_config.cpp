@@ -266,6 +266,7 @@ NavierStokesConfig parse_ns(const YamlNode *node) {
   const std::string ps = get_or(*node, "pressure_solver", std::string("bicgstab"));
   if (ps == "bicgstab") c.pressure_solver = 0;
   else if (ps == "cg") c.pressure_solver = 1;
+  else if (ps == "amg") c.pressure_solver = 2;
   else throw std::runtime_error("Unknown pressure_solver: " + ps);
   const std::string pr = get_or(*node, "precision", std::string("fp64"));
   if (pr == "fp64") c.precision = 0;

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define CFSA_ABI_VERSION 2
+#define CFSA_ABI_VERSION 3
 
 enum cfsa_status {
   CFSA_OK = 0,
@@ -53,7 +53,9 @@ typedef struct cfsa_solver_options {
   double momentum_tol;   /* 1e-6 */
   double pressure_tol;   /* 1e-8 */
   int max_iter;          /* 1000 */
-  int pressure_solver;   /* 0 BiCGSTAB (reference), 1 Jacobi PCG */
+  int pressure_solver;   /* 0 BiCGSTAB (reference), 1 Jacobi PCG, 2 CG preconditioned by one
+                            V-cycle of a smoothed-aggregation AMG hierarchy of L_bc (built on
+                            the host from the current values; FP64 in every precision mode) */
   int precision;         /* 0 FP64, 1 FP32 inner solves in FP64 iterative refinement for the
                             momentum and pressure solves, 2 for the momentum solve only */
   double inner_tol;      /* relative tolerance of each FP32 inner solve, 1e-4 */
@@ -91,6 +93,10 @@ typedef struct cfsa_solver_options {
                             eps^2 |r0|^2 (Eigen's restart branch, restated from its published
                             algorithm).  Larger values make that branch reachable on small
                             systems, for tests; > 0. */
+  int amg_reuse;         /* pressure_solver 2: changes of the L_bc values (moved mesh, new
+                            labels) for which the AMG hierarchy is kept, the finest level
+                            always smoothing with the current operator; 0 (default) rebuilds
+                            on every change.  Re-assembly to identical values never rebuilds. */
 } cfsa_solver_options;
 
 typedef struct cfsa_step_stats {
@@ -277,10 +283,44 @@ int cfsa_ns_apply_matrix(cfsa_ns *h, int which, const double *x, double *y, int 
  * CFSA_ERR_INVALID).  Same results as cfsa_ns_apply_matrix, bit for bit. */
 int cfsa_ns_apply_matrix_noalloc(cfsa_ns *h, int which, const double *x, double *y,
                                  int nrhs);
-/* solve matrix 2 (nrhs 1 or 3) or 4 (nrhs 1) with method 0 BiCGSTAB / 1 PCG */
+/* solve matrix 2 (nrhs 1 or 3) or 4 (nrhs 1) with method 0 BiCGSTAB / 1 PCG /
+ * 2 AMG-preconditioned CG (matrix 4 only) */
 int cfsa_ns_krylov_solve(cfsa_ns *h, int which, int method, int nrhs, const double *b,
                          double *x, double tol, int maxit, int precision,
                          cfsa_solve_result *res);
+/* AMG preconditioner of the pressure operator (extension, no counterpart in the
+ * reference): one V-cycle z = M^-1 r (nv values each), building the hierarchy
+ * from the current L_bc if needed */
+int cfsa_ns_amg_apply(cfsa_ns *h, const double *r, double *z);
+/* the solver's current hierarchy: *levels (0 if none built), rows and nonzeros
+ * of up to cap levels, set-up time of the last build (ms), builds so far; any
+ * output pointer may be NULL */
+int cfsa_ns_amg_info(cfsa_ns *h, int cap, int64_t *rows, int64_t *nnz, double *setup_ms,
+                     int *builds, int *levels);
+/* Host-side hierarchy of any square CSR matrix (sorted columns per row), for
+ * inspection and tests: the set-up the solver runs (strength threshold theta,
+ * at most max_coarse rows on the coarsest level) */
+typedef struct cfsa_amg cfsa_amg;
+int cfsa_amg_build(int64_t n, const int32_t *rowptr, const int32_t *col, const double *val,
+                   double theta, int max_coarse, cfsa_amg **out);
+int cfsa_amg_free(cfsa_amg *h);
+/* levels (>= 1), rows of the coarsest level, 1 if the coarsest level is a
+ * Jacobi step instead of the dense inverse */
+int cfsa_amg_num_levels(const cfsa_amg *h, int *levels, int64_t *coarse_n, int *coarse_jacobi);
+/* which: 0 the level operator A, 1 the prolongation P (rows x cols = n x nc),
+ * 2 the restriction R = P^T; P and R exist on every level but the last */
+int cfsa_amg_level_size(const cfsa_amg *h, int level, int which, int64_t *rows, int64_t *cols,
+                        int64_t *nnz);
+int cfsa_amg_level_get(const cfsa_amg *h, int level, int which, int32_t *rowptr, int32_t *col,
+                       double *val);
+/* spectral radius estimate of D^-1 A, Chebyshev upper bound lmax, coefficients
+ * (c0, c1, c2) and, if agg is not NULL, the aggregate of each row (-1: none;
+ * empty on the last level) */
+int cfsa_amg_level_params(const cfsa_amg *h, int level, double *rho, double *lmax,
+                          double coef[3], int32_t *agg);
+/* dense (pseudo-)inverse of the coarsest operator, row-major coarse_n^2 */
+int cfsa_amg_coarse_inverse(const cfsa_amg *h, double *inv);
+
 /* mean duration (ms) of the momentum-matrix SpMV kernel over `reps` launches */
 int cfsa_ns_time_momentum_spmv(cfsa_ns *h, int reps, int precision, double *ms);
 /* the hipStream_t all solver work is issued on */
@@ -340,7 +380,7 @@ typedef struct cfsa_sim_config {
   int use_dirichlet_pressure;
   double dt, relax_p, relax_u;
   int n_corrections;   /* extension, default 1 */
-  int pressure_solver; /* extension: 0 bicgstab, 1 cg */
+  int pressure_solver; /* extension: 0 bicgstab, 1 cg, 2 amg (AMG-preconditioned CG) */
   int precision;       /* extension: 0 fp64, 1 mixed */
   double momentum_tol, pressure_tol; /* extension, 1e-6 / 1e-8 */
   int inlet_profile;   /* extension: 0 uniform, 1 parabolic */

@@ -75,6 +75,9 @@ def test_extension_keys(P):
     assert c["mesh_type"] == 3 and c["box_size"] == [2.0, 1.0, 1.0] and c["box_cells"] == [8, 4, 4]
     assert c["n_corrections"] == 2 and c["pressure_solver"] == 1 and c["precision"] == 1
     assert c["momentum_tol"] == 1e-8 and c["inlet_profile"] == 1
+    c = P.parseConfig("mesh:\n  type: cylinder\nnavier_stokes:\n  enabled: yes\n"
+                      "  pressure_solver: amg\n")
+    assert c["pressure_solver"] == 2
 
 
 @pytest.mark.parametrize("text,msg", [
