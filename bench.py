@@ -181,8 +181,11 @@ def _cpu_baseline(O, g, fp64_stats, sz, pressure_method, pressure_iters=None):
                        f"{'Jacobi-CG' if pressure_method == 1 else 'BiCGSTAB'} on the "
                        f"{sz['nv']}-row pressure matrix ({t_p * 1e3:.2f} ms/iter), scaled by the "
                        f"FP64 run's mean {mom:.1f} momentum + {pre:.1f} pressure iterations per "
-                       f"step; element kernels (convection, divergence, right-hand sides) "
-                       f"not counted, which favours the CPU"),
+                       f"step" + (" (the pressure count: Jacobi-CG on the last step's pressure "
+                                  "system, the GPU steps having used the AMG-preconditioned "
+                                  "solve)" if pressure_iters is not None else "") +
+                       "; element kernels (convection, divergence, right-hand sides) "
+                       "not counted, which favours the CPU"),
             "cpu": cpu_model(), "nproc": os.cpu_count()}
 
 

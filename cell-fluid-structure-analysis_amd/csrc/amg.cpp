@@ -224,29 +224,33 @@ std::vector<double> dense_inverse(const HostCsr &A) {
       a[(size_t)(i * n + j)] = s / l;
     }
   }
-  std::vector<double> inv((size_t)(n * n), 0.0), y((size_t)n);
-  for (int64_t c = 0; c < n; ++c) {
-    // L y = e_c, then L^T x = y (dropped pivots: zero)
-    for (int64_t i = 0; i < n; ++i) {
-      if (dropped[(size_t)i]) {
-        y[(size_t)i] = 0.0;
-        continue;
+  std::vector<double> inv((size_t)(n * n), 0.0);
+  // one column of the inverse per solve: L y = e_c, then L^T x = y (dropped
+  // pivots: zero); columns in parallel, each thread with its own work vector
+  parallel_chunks(n, [&](int, int64_t c0, int64_t c1) {
+    std::vector<double> y((size_t)n);
+    for (int64_t c = c0; c < c1; ++c) {
+      for (int64_t i = 0; i < n; ++i) {
+        if (dropped[(size_t)i]) {
+          y[(size_t)i] = 0.0;
+          continue;
+        }
+        double s = i == c ? 1.0 : 0.0;
+        for (int64_t k = 0; k < i; ++k) s -= a[(size_t)(i * n + k)] * y[(size_t)k];
+        y[(size_t)i] = s / a[(size_t)(i * n + i)];
       }
-      double s = i == c ? 1.0 : 0.0;
-      for (int64_t k = 0; k < i; ++k) s -= a[(size_t)(i * n + k)] * y[(size_t)k];
-      y[(size_t)i] = s / a[(size_t)(i * n + i)];
-    }
-    for (int64_t i = n - 1; i >= 0; --i) {
-      if (dropped[(size_t)i]) {
-        y[(size_t)i] = 0.0;
-        continue;
+      for (int64_t i = n - 1; i >= 0; --i) {
+        if (dropped[(size_t)i]) {
+          y[(size_t)i] = 0.0;
+          continue;
+        }
+        double s = y[(size_t)i];
+        for (int64_t k = i + 1; k < n; ++k) s -= a[(size_t)(k * n + i)] * y[(size_t)k];
+        y[(size_t)i] = s / a[(size_t)(i * n + i)];
       }
-      double s = y[(size_t)i];
-      for (int64_t k = i + 1; k < n; ++k) s -= a[(size_t)(k * n + i)] * y[(size_t)k];
-      y[(size_t)i] = s / a[(size_t)(i * n + i)];
+      for (int64_t i = 0; i < n; ++i) inv[(size_t)(i * n + c)] = y[(size_t)i];
     }
-    for (int64_t i = 0; i < n; ++i) inv[(size_t)(i * n + c)] = y[(size_t)i];
-  }
+  }, 64);
   return inv;
 }
 

@@ -47,7 +47,7 @@ struct HostCsr {
 
 struct AmgParams {
   double theta = 0.08;   // strength threshold
-  int max_coarse = 400;  // rows of the coarsest (dense) level at most
+  int max_coarse = 1000;  // rows of the coarsest (dense) level at most
   int max_levels = 12;
 };
 
