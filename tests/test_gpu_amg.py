@@ -9,7 +9,8 @@ GPU:
   the iteration count of the restated CG (within one);
 * PISO steps with the AMG pressure solve against the FP64 oracle (Jacobi-CG
   pressure) at tight tolerances, FP64 and mixed momentum;
-* graph replay and eager launches give the same bits; rebuild policy.
+* graph replay and eager launches give the same bits; rebuild policy;
+* the adjoint with AMG pressure solves equals the Jacobi-CG adjoint.
 """
 import numpy as np
 import pytest
@@ -156,3 +157,29 @@ def test_rebuild_policy(P):
         assert g.stepStats()["pressure_err"][0] <= 1e-8
         # reuse 0: rebuilt on the change; reuse 1: the first change keeps it
         assert g.amgInfo()["builds"] == 2, (reuse, g.amgInfo())
+
+
+def test_adjoint_with_amg_pressure(P):
+    """The adjoint's forward and transposed pressure solves through the AMG
+    preconditioner (L_bc is symmetric, so the same hierarchy serves both) give
+    the Jacobi-CG adjoint's J and gradient, at the tight tolerance gradients need."""
+    from test_gpu_adjoint import inlet_basis
+    pm = _mesh(P, "cyl")
+    X = np.vstack([pm.vertices(), pm.edgeNodes()])
+    B = inlet_basis(X, ax_lo=(0.0, 0.0), scale=1.0)
+    c = 0.5 + np.random.default_rng(0).normal(0, 0.05, 4)
+    out = []
+    for ps in (1, 2):
+        g = P.NavierStokesSolver()
+        g.initialize(pm, RHO, MU)
+        g.setDt(0.01)
+        g.setNumberOfCorrections(2)
+        g.setOptions(momentum_tol=1e-13, pressure_tol=1e-13, stop_rule=1, max_iter=20000,
+                     pressure_solver=ps)
+        out.append(g.adjointGradient(3, B, c))
+    (J1, dc1, dmu1), (J2, dc2, dmu2) = out
+    print(f"adjoint: J {J1:.12e} / {J2:.12e}, dc rel {relerr(dc2, dc1):.2e}, "
+          f"dmu rel {abs(dmu2 - dmu1) / abs(dmu1):.2e}", flush=True)
+    assert abs(J2 - J1) <= 1e-10 * abs(J1)
+    assert relerr(dc2, dc1) <= 1e-7
+    assert abs(dmu2 - dmu1) <= 1e-6 * abs(dmu1)
