@@ -1,6 +1,7 @@
 // AMG hierarchy of the pressure operator on the device (pressure_solver = 2):
-// host set-up from the current L_bc values (amg.cpp), upload of every level
-// as CSR-stream operators, and the parity entry points.
+// host set-up from the current L_bc values (amg.cpp), upload of every level's
+// operators (CSR-stream, or sliced-ELL), and the parity entry points.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -13,24 +14,47 @@ namespace cfsa {
 namespace {
 
 // uploads a host CSR as a CSR-stream operator (windows of up to 2048 nonzeros
-// and kStreamRows rows, one right-hand side)
-void upload_stream(const HostCsr &h, DBuf<int> &rp, DBuf<int> &ci, DBuf<int> &win,
-                   DBuf<double> &val, DevCsr &d) {
+// and kStreamRows rows, one right-hand side), or, when a row is longer than a
+// window holds (coarse operators of unstructured meshes grow dense), as
+// sliced-ELL (also when `sell` asks for it); the SpMV launchers pick the kernel
+// from the layout
+template <class M>
+void upload_level_matrix(const HostCsr &h, M &m, DevCsr &d, bool sell) {
   CsrPattern cp;
   cp.n = h.n;
   cp.rowptr = h.rp;
-  const std::vector<int32_t> w = stream_windows(cp, kStreamB<double, 1>, kStreamRows);
-  rp.upload(h.rp.data(), h.rp.size());
-  ci.upload(h.ci.data(), h.ci.size());
-  val.upload(h.v.data(), h.v.size());
-  win.upload(w.data(), w.size());
+  int32_t longest = 0;
+  for (int64_t i = 0; i < h.n; ++i)
+    longest = std::max(longest, h.rp[(size_t)i + 1] - h.rp[(size_t)i]);
   d = DevCsr{};
   d.n = (int)h.n;
   d.nnz = h.nnz();
-  d.rowptr = rp.p;
-  d.col = ci.p;
-  d.win1 = win.p;
-  d.nwin1 = (int)w.size() - 1;
+  if (!sell && longest <= kStreamB<double, 1>) {
+    const std::vector<int32_t> w = stream_windows(cp, kStreamB<double, 1>, kStreamRows);
+    m.rp.upload(h.rp.data(), h.rp.size());
+    m.ci.upload(h.ci.data(), h.ci.size());
+    m.v.upload(h.v.data(), h.v.size());
+    m.win.upload(w.data(), w.size());
+    d.rowptr = m.rp.p;
+    d.col = m.ci.p;
+    d.win1 = m.win.p;
+    d.nwin1 = (int)w.size() - 1;
+    return;
+  }
+  cp.col = h.ci;
+  const SellPattern S = build_sell(cp);
+  std::vector<double> sv((size_t)S.stored());
+  for (size_t e = 0; e < sv.size(); ++e) sv[e] = S.src[e] >= 0 ? h.v[(size_t)S.src[e]] : 0.0;
+  m.sptr.upload(S.sptr.data(), S.sptr.size());
+  m.rlen.upload(S.rlen.data(), S.rlen.size());
+  m.perm.upload(S.perm.data(), S.perm.size());
+  m.ci.upload(S.col.data(), S.col.size());
+  m.v.upload(sv.data(), sv.size());
+  d.col = m.ci.p;
+  d.sptr = m.sptr.p;
+  d.rlen = m.rlen.p;
+  d.perm = m.perm.p;
+  d.nslices = S.nslices;
 }
 
 }  // namespace
@@ -62,6 +86,9 @@ void NSSolver::ensure_amg() {
   A0.v = v;
   const AmgHierarchy H = amg_build(A0);
   const int L = (int)H.levels.size();
+  // coarse operators follow spmv_layout 2 (sliced-ELL); otherwise CSR-stream
+  // unless a row is too long for it
+  const bool sell = opt.spmv_layout == 2;
   clear_graphs();  // captured chunks hold the previous hierarchy's buffers
   amg_lv_.clear();
   amg_lv_.resize((size_t)L);
@@ -79,18 +106,18 @@ void NSSolver::ensure_amg() {
     amg_rows_[(size_t)l] = h.A.n;
     amg_nnz_[(size_t)l] = h.A.nnz();
     if (l > 0) {
-      upload_stream(h.A, d.Arp, d.Aci, d.Awin, d.Av, w.A);
-      w.Av = d.Av.p;
+      upload_level_matrix(h.A, d.A, w.A, sell);
+      w.Av = d.A.v.p;
       d.dinv.upload(h.dinv.data(), h.dinv.size());
       w.dinv = d.dinv.p;
       d.b.alloc((size_t)h.A.n);
       w.b = d.b.p;
     }
     if (l < L - 1) {
-      upload_stream(h.P, d.Prp, d.Pci, d.Pwin, d.Pv, w.P);
-      w.Pv = d.Pv.p;
-      upload_stream(h.R, d.Rrp, d.Rci, d.Rwin, d.Rv, w.R);
-      w.Rv = d.Rv.p;
+      upload_level_matrix(h.P, d.P, w.P, sell);
+      w.Pv = d.P.v.p;
+      upload_level_matrix(h.R, d.R, w.R, sell);
+      w.Rv = d.R.v.p;
     }
     d.x.alloc((size_t)h.A.n);
     d.y.alloc((size_t)h.A.n);

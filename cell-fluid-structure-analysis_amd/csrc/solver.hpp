@@ -318,9 +318,13 @@ class NSSolver {
   template <class F>
   void run_chunk(const std::vector<uintptr_t> &key, F &&launch);
   // AMG hierarchy of the pressure operator (pressure_solver 2; amg_solver.hip)
+  struct AmgDevMat {  // one operator: CSR-stream, or sliced-ELL for rows too long for it
+    DBuf<int> rp, ci, win, sptr, rlen, perm;
+    DBuf<double> v;
+  };
   struct AmgDevLevel {
-    DBuf<int> Arp, Aci, Awin, Prp, Pci, Pwin, Rrp, Rci, Rwin;
-    DBuf<double> Av, Pv, Rv, dinv, b, x, y, r;
+    AmgDevMat A, P, R;
+    DBuf<double> dinv, b, x, y, r;
   };
   std::vector<AmgDevLevel> amg_lv_;
   std::vector<AmgLevelView> amg_view_;

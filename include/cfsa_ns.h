@@ -72,7 +72,10 @@ typedef struct cfsa_solver_options {
                             (same bits as 0; 0 falls back to it for rows too long for
                             LDS) */
   int spmv_layout;       /* Krylov SpMV: 0 auto (momentum CSR-stream, pressure sliced-ELL),
-                            1 CSR-stream, 2 sliced-ELL; every layout gives the same bits */
+                            1 CSR-stream, 2 sliced-ELL; every layout gives the same bits.
+                            The AMG hierarchy's coarser operators (pressure_solver 2) are
+                            CSR-stream unless a row exceeds 2048 nonzeros, or sliced-ELL
+                            with 2, as set when the hierarchy is built */
   int momentum_solver;   /* 0 BiCGSTAB (reference), 1 Jacobi PCG (the momentum matrix
                             M/dt + K + penalty diagonal is symmetric positive definite) */
   int check_every_1;     /* the same for single right-hand-side (pressure) solves, 32: their

@@ -49,10 +49,12 @@ def _solver(P, pm, kind, **opts):
 CASES = ["kuhn08", "kuhn12n", "cyl"]
 
 
-@pytest.mark.parametrize("kind", CASES)
-def test_vcycle_matches_restatement(P, kind):
+@pytest.mark.parametrize("kind,layout", [(k, 0) for k in CASES] + [("cyl", 2)])
+def test_vcycle_matches_restatement(P, kind, layout):
+    """layout 2 puts every coarse operator, P and R in sliced-ELL (the layout
+    rows longer than a CSR-stream window use)."""
     pm = _mesh(P, kind)
-    g = _solver(P, pm, kind, pressure_solver=2)
+    g = _solver(P, pm, kind, pressure_solver=2, spmv_layout=layout)
     rp, col, val = g.getMatrix("Lbc")
     n = len(rp) - 1
     H = P.HostAmg(rp, col, val)

@@ -402,18 +402,32 @@ def test_nan_reported(P, O):
 
 
 def test_vtk_export_matches_fields(P, O, tmp_path):
+    """io::exportToVtk (VtkExport.cpp:11-16,49-84,238-268) after two PISO steps,
+    line by line against the text rendered here from the oracle's own state:
+    ostream's default formatting is printf's %g (6 significant digits), points
+    and velocity scaled, pressure not, labels as integers."""
     g, o, pm, om = make_case(P, O, "kuhn", 2)
-    assert g.pisoStep()
+    for _ in range(2):
+        assert g.pisoStep() and o.pisoStep() == 0
+    scale = 2.0
     path = tmp_path / "f.vtk"
-    g.exportVtk(path, scale=1.0)
-    txt = path.read_text().splitlines()
-    i = txt.index("SCALARS pressure double 1")
-    nv = pm.nVertices()
-    p = np.array([float(x) for x in txt[i + 2:i + 2 + nv]])
-    np.testing.assert_allclose(p, g.getPressure(), rtol=1e-5, atol=1e-12)
-    j = txt.index("VECTORS velocity double")
-    u = np.array([[float(v) for v in r.split()] for r in txt[j + 1:j + 1 + nv]])
-    np.testing.assert_allclose(u, g.getVelocity()[:nv], rtol=1e-5, atol=1e-12)
+    g.exportVtk(path, scale=scale)
+    got = path.read_text().split("\n")
+    X, T, bc = om.vertices(), om.tets(), om.p1BC()
+    u, p = o.getVelocity(), o.getPressure()
+    nv, nt = len(X), len(T)
+    f = lambda v: "%g" % v
+    want = ["# vtk DataFile Version 3.0", "Shape export", "ASCII", "DATASET UNSTRUCTURED_GRID",
+            f"POINTS {nv} double"]
+    want += [" ".join(f(c * scale) for c in x) for x in X]
+    want += [f"CELLS {nt} {5 * nt}"] + ["4 " + " ".join(str(int(i)) for i in t) for t in T]
+    want += [f"CELL_TYPES {nt}"] + ["10"] * nt
+    want += ["", f"POINT_DATA {nv}", "SCALARS FluidBoundaryType int 1", "LOOKUP_TABLE default"]
+    want += [str(int(b)) for b in bc]
+    want += ["", "SCALARS pressure double 1", "LOOKUP_TABLE default"] + [f(v) for v in p]
+    want += ["", "VECTORS velocity double"]
+    want += [" ".join(f(c * scale) for c in x) for x in u[:nv]] + [""]
+    assert got == want
 
 
 def test_boundary_forces_bit_exact(P, O):
