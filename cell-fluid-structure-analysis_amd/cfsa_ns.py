@@ -178,7 +178,7 @@ _sig("cfsa_ns_apply_matrix_noalloc", C.c_int, _p, C.c_int, _f64p, _f64p, C.c_int
 _sig("cfsa_ns_krylov_solve", C.c_int, _p, C.c_int, C.c_int, C.c_int, _f64p, _f64p, C.c_double,
      C.c_int, C.c_int, C.POINTER(SolveResult))
 _sig("cfsa_ns_time_momentum_spmv", C.c_int, _p, C.c_int, C.c_int, C.POINTER(C.c_double))
-_sig("cfsa_ns_amg_apply", C.c_int, _p, _f64p, _f64p)
+_sig("cfsa_ns_amg_apply", C.c_int, _p, _f64p, _f64p, C.c_int)
 _sig("cfsa_ns_amg_info", C.c_int, _p, C.c_int, _p, _p, C.POINTER(C.c_double),
      C.POINTER(C.c_int), C.POINTER(C.c_int))
 _sig("cfsa_amg_build", C.c_int, C.c_int64, _i32p, _i32p, _f64p, C.c_double, C.c_int,
@@ -649,11 +649,12 @@ class NavierStokesSolver:
                    "restarts": list(r.restarts)[:nrhs], "refinements": r.refinements,
                    "ok": bool(r.ok)}
 
-    def amgApply(self, r):
-        """One V-cycle z = M^-1 r of the pressure operator's AMG hierarchy."""
+    def amgApply(self, r, precision=0):
+        """One V-cycle z = M^-1 r of the pressure operator's AMG hierarchy
+        (precision 1: the FP32 V-cycle of the mixed mode)."""
         r = np.ascontiguousarray(r, np.float64)
         z = np.empty_like(r)
-        _check(_lib.cfsa_ns_amg_apply(self._h, r, z))
+        _check(_lib.cfsa_ns_amg_apply(self._h, r, z, int(precision)))
         return z
 
     def amgInfo(self):

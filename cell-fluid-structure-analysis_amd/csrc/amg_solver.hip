@@ -60,30 +60,38 @@ void upload_level_matrix(const HostCsr &h, M &m, DevCsr &d, bool sell) {
 }  // namespace
 
 // Builds (or keeps) the hierarchy for the current L_bc.  A hierarchy is rebuilt
-// when the values differ from those it was built from, unless amg_reuse allows
+// when the values differ from the last ones seen (a 64-bit fingerprint taken on
+// the device, so an unchanged operator costs no copy), unless amg_reuse allows
 // keeping it for that many changes; identical values (a re-assembly of an
 // unchanged mesh) never rebuild.
 void NSSolver::ensure_amg() {
   if (amg_built_ && !amg_lbc_changed_) return;
-  std::vector<double> v((size_t)nnzP);
-  CFSA_HIP(hipStreamSynchronize(stream_));
-  d_Lbc.download(v.data(), (size_t)nnzP);
   amg_lbc_changed_ = false;
+  d_print_.alloc(1);
+  fingerprint(stream_, nnzP, d_Lbc.p, d_print_.p);
+  uint64_t print = 0;
+  CFSA_HIP(hipMemcpyAsync(&print, d_print_.p, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          stream_));
+  CFSA_HIP(hipStreamSynchronize(stream_));
   if (amg_built_) {
-    if (v == amg_vals_) return;
+    if (print == amg_print_) return;
+    amg_print_ = print;
     if (amg_stale_ < opt.amg_reuse) {
       ++amg_stale_;
       return;
     }
   }
+  amg_print_ = print;
   const auto t0 = std::chrono::steady_clock::now();
+  std::vector<double> v((size_t)nnzP);
+  d_Lbc.download(v.data(), (size_t)nnzP);
   HostCsr A0;
   A0.n = A0.m = nv;
   A0.rp.resize((size_t)nv + 1);
   A0.ci.resize((size_t)nnzP);
   d_Prp.download(A0.rp.data(), (size_t)nv + 1);
   d_Pcol.download(A0.ci.data(), (size_t)nnzP);
-  A0.v = v;
+  A0.v = std::move(v);
   const AmgHierarchy H = amg_build(A0);
   const int L = (int)H.levels.size();
   // coarse operators follow spmv_layout 2 (sliced-ELL); otherwise CSR-stream
@@ -92,13 +100,14 @@ void NSSolver::ensure_amg() {
   clear_graphs();  // captured chunks hold the previous hierarchy's buffers
   amg_lv_.clear();
   amg_lv_.resize((size_t)L);
-  amg_view_.assign((size_t)L, AmgLevelView{});
+  amg_view_.assign((size_t)L, AmgLevelView<double>{});
+  amg32_built_ = false;
   amg_rows_.assign((size_t)L, 0);
   amg_nnz_.assign((size_t)L, 0);
   for (int l = 0; l < L; ++l) {
     const AmgLevel &h = H.levels[(size_t)l];
     AmgDevLevel &d = amg_lv_[(size_t)l];
-    AmgLevelView &w = amg_view_[(size_t)l];
+    AmgLevelView<double> &w = amg_view_[(size_t)l];
     w.n = (int)h.A.n;
     w.c0 = h.c0;
     w.c1 = h.c1;
@@ -132,7 +141,6 @@ void NSSolver::ensure_amg() {
     amg_cinv_.upload(H.coarse_inv.data(), H.coarse_inv.size());
   else
     amg_cinv_.release();
-  amg_vals_ = std::move(v);
   amg_stale_ = 0;
   amg_built_ = true;
   ++amg_builds_;
@@ -140,26 +148,120 @@ void NSSolver::ensure_amg() {
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void NSSolver::amg_apply(const double *r, double *z) {
-  if (!initialized) throw std::runtime_error("amg_apply needs an initialized solver");
-  prepare_operators();
-  ensure_amg();
-  AmgLevelView &top = amg_view_[0];
+// FP32 copies of every level's operators, diagonal and coarse inverse (device
+// conversions of the FP64 hierarchy) and FP32 work vectors; level 0 uses the
+// FP32 pressure operator the mixed mode keeps in the Krylov layout
+void NSSolver::ensure_amg32() {
+  if (amg32_built_) return;
+  const int L = (int)amg_lv_.size();
+  amg_view32_.assign((size_t)L, AmgLevelView<float>{});
+  auto conv = [&](const DBuf<double> &in, DBuf<float> &out) {
+    out.alloc(in.n);
+    if (in.n) convert<float, double>(stream_, (int64_t)in.n, in.p, out.p);
+    return out.p;
+  };
+  for (int l = 0; l < L; ++l) {
+    AmgDevLevel &d = amg_lv_[(size_t)l];
+    const AmgLevelView<double> &w = amg_view_[(size_t)l];
+    AmgLevelView<float> &f = amg_view32_[(size_t)l];
+    f.n = w.n;
+    f.c0 = w.c0;
+    f.c1 = w.c1;
+    f.c2 = w.c2;
+    f.A = w.A;
+    f.P = w.P;
+    f.R = w.R;
+    if (l > 0) {
+      f.Av = conv(d.A.v, d.A.v32);
+      f.dinv = conv(d.dinv, d.dinv32);
+      d.b32.alloc((size_t)w.n);
+      f.b = d.b32.p;
+    } else {
+      d.b32.alloc((size_t)w.n);  // FP32 copy of the CG residual
+      f.b = d.b32.p;
+    }
+    if (l < L - 1) {
+      f.Pv = conv(d.P.v, d.P.v32);
+      f.Rv = conv(d.R.v, d.R.v32);
+    }
+    d.x32.alloc((size_t)w.n);
+    d.y32.alloc((size_t)w.n);
+    d.r32.alloc((size_t)w.n);
+    f.x = d.x32.p;
+    f.y = d.y32.p;
+    f.r = d.r32.p;
+  }
+  if (!amg_cjac_) conv(amg_cinv_, amg_cinv32_);
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  amg32_built_ = true;
+}
+
+template <>
+AmgLevelView<double> *NSSolver::amg_views<double>() {
+  AmgLevelView<double> &top = amg_view_[0];
   top.A = preA();
   top.Av = preV();
   top.dinv = d_invdL.p;
+  top.b = w1_.r.p;  // the V-cycle reads the CG residual itself
+  return amg_view_.data();
+}
+template <>
+AmgLevelView<float> *NSSolver::amg_views<float>() {
+  if (d_Lbc32.n == 0) throw std::logic_error("FP32 V-cycle without the FP32 pressure operator");
+  ensure_amg32();
+  AmgLevelView<float> &top = amg_view32_[0];
+  top.A = preA();
+  top.Av = d_Lbc32.p;
+  top.dinv = d_invdL32.p;
+  return amg_view32_.data();
+}
+template <>
+const double *NSSolver::amg_cinv<double>() {
+  return amg_cinv_.p;
+}
+template <>
+const float *NSSolver::amg_cinv<float>() {
+  return amg_cinv32_.p;
+}
+
+void NSSolver::amg_apply(const double *r, double *z, int precision) {
+  if (!initialized) throw std::runtime_error("amg_apply needs an initialized solver");
+  if (precision != 0 && precision != 1) throw std::invalid_argument("amg_apply: precision 0 or 1");
+  const int saved = opt.precision;
+  opt.precision = precision == 1 ? 1 : saved;
+  try {
+    prepare_operators();
+  } catch (...) {
+    opt.precision = saved;
+    throw;
+  }
+  opt.precision = saved;
+  ensure_amg();
   DBuf<double> db;
   db.upload(r, (size_t)nv);
   w1_.alloc((size_t)nv);
-  top.b = w1_.r.p;
   int Ga = 0, Gc = 0;
-  // r is copied into the CG residual buffer with the first smoothing step;
+  const int L = (int)amg_view_.size();
+  // the first smoothing step goes with copying r into the CG residual buffer;
   // x goes to a scratch vector
-  amg_cg_init(stream_, nv, db.p, d_invdL.p, top.c0, w1_.s.p, w1_.r.p, top.x, 0, d_part_a.p, &Ga);
-  amg_vcycle(stream_, amg_view_.data(), (int)amg_view_.size(), amg_cinv_.p, amg_nc_, amg_cjac_,
-             nullptr, d_part_c.p, &Gc);
+  if (precision == 1) {
+    AmgLevelView<float> *lv = amg_views<float>();
+    amg_cg_init<float>(stream_, nv, db.p, d_invdL.p, lv[0].dinv, lv[0].c0, w1_.s.p, w1_.r.p,
+                       lv[0].b, lv[0].x, 0, d_part_a.p, &Ga);
+    amg_vcycle<float>(stream_, lv, L, amg_cinv<float>(), amg_nc_, amg_cjac_, w1_.r.p, nullptr,
+                      d_part_c.p, &Gc);
+    amg_cg_p0<float>(stream_, nv, lv[0].y, w1_.t.p);
+  } else {
+    AmgLevelView<double> *lv = amg_views<double>();
+    amg_cg_init<double>(stream_, nv, db.p, d_invdL.p, lv[0].dinv, lv[0].c0, w1_.s.p, w1_.r.p,
+                        nullptr, lv[0].x, 0, d_part_a.p, &Ga);
+    amg_vcycle<double>(stream_, lv, L, amg_cinv<double>(), amg_nc_, amg_cjac_, w1_.r.p, nullptr,
+                       d_part_c.p, &Gc);
+    CFSA_HIP(hipMemcpyAsync(w1_.t.p, lv[0].y, (size_t)nv * sizeof(double),
+                            hipMemcpyDeviceToDevice, stream_));
+  }
   CFSA_HIP(hipStreamSynchronize(stream_));
-  CFSA_HIP(hipMemcpy(z, top.y, (size_t)nv * sizeof(double), hipMemcpyDeviceToHost));
+  CFSA_HIP(hipMemcpy(z, w1_.t.p, (size_t)nv * sizeof(double), hipMemcpyDeviceToHost));
 }
 
 int NSSolver::amg_info(int cap, int64_t *rows, int64_t *nnz, double *setup_ms, int *builds) {

@@ -713,13 +713,14 @@ int cfsa_ns_krylov_solve(cfsa_ns *h, int which, int method, int nrhs, const doub
     return CFSA_OK;
   });
 }
-int cfsa_ns_amg_apply(cfsa_ns *h, const double *r, double *z) {
+int cfsa_ns_amg_apply(cfsa_ns *h, const double *r, double *z, int precision) {
   CHECK_PTR(h);
   CHECK_PTR(r);
   CHECK_PTR(z);
   if (int e = check_init(h)) return e;
+  if (precision != 0 && precision != 1) return fail(CFSA_ERR_INVALID, "precision 0 or 1");
   return guard([&]() -> int {
-    h->s.amg_apply(r, z);
+    h->s.amg_apply(r, z, precision);
     return CFSA_OK;
   });
 }

@@ -2258,119 +2258,133 @@ template void convert<double, float>(hipStream_t, int64_t, const float *, double
 // with degree-2 Chebyshev smoothing, and the CG iteration around it.  Every
 // smoothing step is one SpMV whose epilogue applies the polynomial update, so a
 // level costs four operator products plus one restriction and one
-// prolongation.  All kernels take the solve's `active` flag as their mask and
-// exit at once after the solve has stopped.
+// prolongation.  The V-cycle runs in T (double, or float for the mixed
+// precision = 1 mode: values, vectors and SpMV accumulation); the CG around it
+// is FP64 (iterate, residual, scalars, stopping test).  All kernels take the
+// solve's `active` flag as their mask and exit at once after the solve stopped.
 // ===========================================================================
+template <class T>
 struct OpAmgBase : Mask1Base {
   const int *__restrict__ col;
-  const double *__restrict__ val;
-  const double *__restrict__ x;  // the gathered vector
+  const T *__restrict__ val;
+  const T *__restrict__ x;  // the gathered vector
   struct Frag {
     int j;
-    double a;
+    T a;
   };
   __device__ Frag fetch(int k) const { return Frag{col[k], val[k]}; }
-  __device__ void prod(const Frag &f, double *p) const { p[0] = f.a * x[f.j]; }
+  __device__ void prod(const Frag &f, T *p) const { p[0] = f.a * x[f.j]; }
 };
 
 // pre-smoothing from x0 = 0, second step: Y = X + c1 X + c2 D^-1 (b - A X)
 // (the first step X = c0 D^-1 b is fused into the kernel that produced b)
-struct OpChebPre : OpAmgBase {
-  const double *__restrict__ b;
-  const double *__restrict__ dinv;
-  double *__restrict__ y;
-  double c1, c2;
+template <class T>
+struct OpChebPre : OpAmgBase<T> {
+  const T *__restrict__ b;
+  const T *__restrict__ dinv;
+  T *__restrict__ y;
+  T c1, c2;
   struct Pre {
-    double b, x, d;
+    T b, x, d;
   };
-  __device__ Pre pre(int row, int) const { return Pre{b[row], x[row], dinv[row]}; }
-  __device__ void epi(int row, int, double acc, const Pre &pr, DD *) const {
+  __device__ Pre pre(int row, int) const { return Pre{b[row], this->x[row], dinv[row]}; }
+  __device__ void epi(int row, int, T acc, const Pre &pr, DD *) const {
     y[row] = (pr.x + c1 * pr.x) + c2 * (pr.d * (pr.b - acc));
   }
 };
 
 // r = b - A Y
-struct OpAmgResidual : OpAmgBase {
-  const double *__restrict__ b;
-  double *__restrict__ r;
+template <class T>
+struct OpAmgResidual : OpAmgBase<T> {
+  const T *__restrict__ b;
+  T *__restrict__ r;
   struct Pre {
-    double b;
+    T b;
   };
   __device__ Pre pre(int row, int) const { return Pre{b[row]}; }
-  __device__ void epi(int row, int, double acc, const Pre &pr, DD *) const { r[row] = pr.b - acc; }
+  __device__ void epi(int row, int, T acc, const Pre &pr, DD *) const { r[row] = pr.b - acc; }
 };
 
 // b_c = R r; X_c = c0 D_c^-1 b_c (first Chebyshev step of the coarse level)
 // unless the coarse level is solved directly (xc == nullptr)
-struct OpRestrict : OpAmgBase {
-  const double *__restrict__ dinvc;
-  double *__restrict__ bc;
-  double *__restrict__ xc;
-  double c0;
+template <class T>
+struct OpRestrict : OpAmgBase<T> {
+  const T *__restrict__ dinvc;
+  T *__restrict__ bc;
+  T *__restrict__ xc;
+  T c0;
   struct Pre {
-    double d;
+    T d;
   };
-  __device__ Pre pre(int row, int) const { return Pre{xc ? dinvc[row] : 0.0}; }
-  __device__ void epi(int row, int, double acc, const Pre &pr, DD *) const {
+  __device__ Pre pre(int row, int) const { return Pre{xc ? dinvc[row] : T(0)}; }
+  __device__ void epi(int row, int, T acc, const Pre &pr, DD *) const {
     bc[row] = acc;
     if (xc) xc[row] = c0 * (pr.d * acc);
   }
 };
 
 // Y += P e_c
-struct OpProlong : OpAmgBase {
-  double *__restrict__ y;
+template <class T>
+struct OpProlong : OpAmgBase<T> {
+  T *__restrict__ y;
   struct Pre {
-    double y;
+    T y;
   };
   __device__ Pre pre(int row, int) const { return Pre{y[row]}; }
-  __device__ void epi(int row, int, double acc, const Pre &pr, DD *) const { y[row] = pr.y + acc; }
+  __device__ void epi(int row, int, T acc, const Pre &pr, DD *) const { y[row] = pr.y + acc; }
 };
 
 // post-smoothing from x0 = Y, first step: X = Y + c0 D^-1 (b - A Y)
-struct OpChebPost1 : OpAmgBase {
-  const double *__restrict__ b;
-  const double *__restrict__ dinv;
-  double *__restrict__ xo;
-  double c0;
+template <class T>
+struct OpChebPost1 : OpAmgBase<T> {
+  const T *__restrict__ b;
+  const T *__restrict__ dinv;
+  T *__restrict__ xo;
+  T c0;
   struct Pre {
-    double b, y, d;
+    T b, y, d;
   };
-  __device__ Pre pre(int row, int) const { return Pre{b[row], x[row], dinv[row]}; }
-  __device__ void epi(int row, int, double acc, const Pre &pr, DD *) const {
+  __device__ Pre pre(int row, int) const { return Pre{b[row], this->x[row], dinv[row]}; }
+  __device__ void epi(int row, int, T acc, const Pre &pr, DD *) const {
     xo[row] = pr.y + c0 * (pr.d * (pr.b - acc));
   }
 };
 
 // second step: Y = X + c1 (X - Y) + c2 D^-1 (b - A X); on the finest level the
-// pairs of b . Y (= r . z of the CG iteration) are emitted
-struct OpChebPost2 : OpAmgBase {
-  const double *__restrict__ b;
-  const double *__restrict__ dinv;
-  double *__restrict__ y;
-  double c1, c2;
+// pairs of r . Y (the CG's r . z, with the FP64 residual rdot) are emitted
+template <class T>
+struct OpChebPost2 : OpAmgBase<T> {
+  const T *__restrict__ b;
+  const T *__restrict__ dinv;
+  const double *__restrict__ rdot;
+  T *__restrict__ y;
+  T c1, c2;
   struct Pre {
-    double b, x, y, d;
+    T b, x, y, d;
+    double r;
   };
-  __device__ Pre pre(int row, int) const { return Pre{b[row], x[row], y[row], dinv[row]}; }
-  __device__ void epi(int row, int, double acc, const Pre &pr, DD *d) const {
-    const double z = (pr.x + c1 * (pr.x - pr.y)) + c2 * (pr.d * (pr.b - acc));
+  __device__ Pre pre(int row, int) const {
+    return Pre{b[row], this->x[row], y[row], dinv[row], rdot ? rdot[row] : 0.0};
+  }
+  __device__ void epi(int row, int, T acc, const Pre &pr, DD *d) const {
+    const T z = (pr.x + c1 * (pr.x - pr.y)) + c2 * (pr.d * (pr.b - acc));
     y[row] = z;
-    dd_fma(d[0], pr.b, z);
+    dd_fma(d[0], pr.r, (double)z);
   }
 };
 
 // coarsest level: Y = C b with the dense (pseudo-)inverse C, one wavefront per
 // row, lane-strided products and a fixed shuffle tree
-__global__ __launch_bounds__(kBlock) void k_amg_dense(int nc, const double *__restrict__ cinv,
-                                                      const double *__restrict__ b,
-                                                      double *__restrict__ y,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_amg_dense(int nc, const T *__restrict__ cinv,
+                                                      const T *__restrict__ b,
+                                                      T *__restrict__ y,
                                                       const int *__restrict__ mask) {
   if (mask && !mask[0]) return;
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (row >= nc) return;
-  double s = 0.0;
+  T s = T(0);
   for (int j = lane; j < nc; j += 64) s += cinv[(size_t)row * nc + j] * b[j];
 #pragma unroll
   for (int st = 32; st > 0; st >>= 1) s += __shfl_down(s, st, 64);
@@ -2378,131 +2392,167 @@ __global__ __launch_bounds__(kBlock) void k_amg_dense(int nc, const double *__re
 }
 
 // coarsest level without a dense inverse: one Jacobi step Y = D^-1 b
-__global__ __launch_bounds__(kBlock) void k_amg_jacobi(int n, const double *__restrict__ dinv,
-                                                       const double *__restrict__ b,
-                                                       double *__restrict__ y,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_amg_jacobi(int n, const T *__restrict__ dinv,
+                                                       const T *__restrict__ b,
+                                                       T *__restrict__ y,
                                                        const int *__restrict__ mask) {
   if (mask && !mask[0]) return;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     y[i] = dinv[i] * b[i];
 }
 
-// pairs of b . y (a one-level hierarchy has no smoothing SpMV to carry them)
-__global__ __launch_bounds__(kBlock) void k_amg_dot(int n, const double *__restrict__ b,
-                                                    const double *__restrict__ y,
+// pairs of r . y (a one-level hierarchy has no smoothing SpMV to carry them)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_amg_dot(int n, const double *__restrict__ r,
+                                                    const T *__restrict__ y,
                                                     const int *__restrict__ mask,
                                                     double *__restrict__ partials) {
   if (mask && !mask[0]) return;
   DD acc[1];
   dd_zero(acc[0]);
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-    dd_fma(acc[0], b[i], y[i]);
+    dd_fma(acc[0], r[i], (double)y[i]);
   block_partials_binned<1, 1>(acc, partials);
 }
 
-template <class Op>
-static Op amg_op(const DevCsr &, const double *val, const double *x, const int *mask) {
+template <class Op, class T>
+static Op amg_op(const DevCsr &A, const T *val, const T *x, const int *mask) {
   Op op{};
   op.mask = mask;
+  op.col = A.col;
   op.val = val;
   op.x = x;
   return op;
 }
 
-static void amg_level(hipStream_t s, const AmgLevelView *lv, int nlev, int l, const double *cinv,
-                      int nc, bool cjac, const int *mask, double *partials, int *grid) {
-  const AmgLevelView &L = lv[l];
+template <class T>
+static void amg_level(hipStream_t s, const AmgLevelView<T> *lv, int nlev, int l, const T *cinv,
+                      int nc, bool cjac, const double *rdot, const int *mask, double *partials,
+                      int *grid) {
+  const AmgLevelView<T> &L = lv[l];
   const bool top = l == 0;
   if (l == nlev - 1) {
     if (cjac)
-      hipLaunchKernelGGL(k_amg_jacobi, dim3(vec_grid(L.n)), dim3(kBlock), 0, s, L.n, L.dinv, L.b,
-                         L.y, mask);
+      hipLaunchKernelGGL((k_amg_jacobi<T>), dim3(vec_grid(L.n)), dim3(kBlock), 0, s, L.n, L.dinv,
+                         L.b, L.y, mask);
     else
-      hipLaunchKernelGGL(k_amg_dense, dim3(div_up(nc, kBlock / 64)), dim3(kBlock), 0, s, nc,
+      hipLaunchKernelGGL((k_amg_dense<T>), dim3(div_up(nc, kBlock / 64)), dim3(kBlock), 0, s, nc,
                          cinv, L.b, L.y, mask);
     CFSA_LAUNCH_CHECK();
     if (top) {
       const int G = vec_grid(L.n);
       *grid = G;
-      hipLaunchKernelGGL(k_amg_dot, dim3(G), dim3(kBlock), 0, s, L.n, L.b, L.y, mask, partials);
+      hipLaunchKernelGGL((k_amg_dot<T>), dim3(G), dim3(kBlock), 0, s, L.n, rdot, L.y, mask,
+                         partials);
       CFSA_LAUNCH_CHECK();
     }
     return;
   }
-  const AmgLevelView &C = lv[l + 1];
+  const AmgLevelView<T> &C = lv[l + 1];
   {
-    OpChebPre op = amg_op<OpChebPre>(L.A, L.Av, L.x, mask);
-    op.col = L.A.col;
+    auto op = amg_op<OpChebPre<T>>(L.A, L.Av, L.x, mask);
     op.b = L.b;
     op.dinv = L.dinv;
     op.y = L.y;
-    op.c1 = L.c1;
-    op.c2 = L.c2;
-    launch_spmv<double, 1, 0>(s, L.A, op, nullptr);
+    op.c1 = (T)L.c1;
+    op.c2 = (T)L.c2;
+    launch_spmv<T, 1, 0>(s, L.A, op, nullptr);
   }
   {
-    OpAmgResidual op = amg_op<OpAmgResidual>(L.A, L.Av, L.y, mask);
-    op.col = L.A.col;
+    auto op = amg_op<OpAmgResidual<T>>(L.A, L.Av, L.y, mask);
     op.b = L.b;
     op.r = L.r;
-    launch_spmv<double, 1, 0>(s, L.A, op, nullptr);
+    launch_spmv<T, 1, 0>(s, L.A, op, nullptr);
   }
   {
     const bool cdirect = l + 1 == nlev - 1;
-    OpRestrict op = amg_op<OpRestrict>(L.R, L.Rv, L.r, mask);
-    op.col = L.R.col;
+    auto op = amg_op<OpRestrict<T>>(L.R, L.Rv, L.r, mask);
     op.dinvc = C.dinv;
     op.bc = C.b;
     op.xc = cdirect ? nullptr : C.x;
-    op.c0 = C.c0;
-    launch_spmv<double, 1, 0>(s, L.R, op, nullptr);
+    op.c0 = (T)C.c0;
+    launch_spmv<T, 1, 0>(s, L.R, op, nullptr);
   }
-  amg_level(s, lv, nlev, l + 1, cinv, nc, cjac, mask, partials, grid);
+  amg_level(s, lv, nlev, l + 1, cinv, nc, cjac, rdot, mask, partials, grid);
   {
-    OpProlong op = amg_op<OpProlong>(L.P, L.Pv, C.y, mask);
-    op.col = L.P.col;
+    auto op = amg_op<OpProlong<T>>(L.P, L.Pv, C.y, mask);
     op.y = L.y;
-    launch_spmv<double, 1, 0>(s, L.P, op, nullptr);
+    launch_spmv<T, 1, 0>(s, L.P, op, nullptr);
   }
   {
-    OpChebPost1 op = amg_op<OpChebPost1>(L.A, L.Av, L.y, mask);
-    op.col = L.A.col;
+    auto op = amg_op<OpChebPost1<T>>(L.A, L.Av, L.y, mask);
     op.b = L.b;
     op.dinv = L.dinv;
     op.xo = L.x;
-    op.c0 = L.c0;
-    launch_spmv<double, 1, 0>(s, L.A, op, nullptr);
+    op.c0 = (T)L.c0;
+    launch_spmv<T, 1, 0>(s, L.A, op, nullptr);
   }
   {
-    OpChebPost2 op = amg_op<OpChebPost2>(L.A, L.Av, L.x, mask);
-    op.col = L.A.col;
+    auto op = amg_op<OpChebPost2<T>>(L.A, L.Av, L.x, mask);
     op.b = L.b;
     op.dinv = L.dinv;
+    op.rdot = top ? rdot : nullptr;
     op.y = L.y;
-    op.c1 = L.c1;
-    op.c2 = L.c2;
+    op.c1 = (T)L.c1;
+    op.c2 = (T)L.c2;
     if (top)
-      *grid = launch_spmv<double, 1, 1>(s, L.A, op, partials);
+      *grid = launch_spmv<T, 1, 1>(s, L.A, op, partials);
     else
-      launch_spmv<double, 1, 0>(s, L.A, op, nullptr);
+      launch_spmv<T, 1, 0>(s, L.A, op, nullptr);
   }
 }
 
-void amg_vcycle(hipStream_t s, const AmgLevelView *lv, int nlev, const double *cinv, int nc,
-                bool coarse_jacobi, const int *mask, double *partials, int *grid) {
+template <class T>
+void amg_vcycle(hipStream_t s, const AmgLevelView<T> *lv, int nlev, const T *cinv, int nc,
+                bool coarse_jacobi, const double *rdot, const int *mask, double *partials,
+                int *grid) {
   if (nlev < 1) throw std::logic_error("AMG V-cycle without levels");
   if (!coarse_jacobi && nc != lv[nlev - 1].n)
     throw std::logic_error("AMG coarse inverse does not match the coarsest level");
-  amg_level(s, lv, nlev, 0, cinv, nc, coarse_jacobi, mask, partials, grid);
+  if (!rdot) throw std::logic_error("AMG V-cycle without the residual of its dot");
+  amg_level(s, lv, nlev, 0, cinv, nc, coarse_jacobi, rdot, mask, partials, grid);
+}
+template void amg_vcycle<double>(hipStream_t, const AmgLevelView<double> *, int, const double *,
+                                 int, bool, const double *, const int *, double *, int *);
+template void amg_vcycle<float>(hipStream_t, const AmgLevelView<float> *, int, const float *,
+                                int, bool, const double *, const int *, double *, int *);
+
+// 64-bit fingerprint of an array of doubles: the sum (mod 2^64) of a mixed
+// hash of every (index, bit pattern), so it does not depend on the order the
+// atomic adds land in; used to tell whether an operator's values changed
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__global__ __launch_bounds__(kBlock) void k_fingerprint(int64_t n, const double *__restrict__ v,
+                                                        unsigned long long *out) {
+  uint64_t h = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    h += mix64((uint64_t)__double_as_longlong(v[i]) + 0x9e3779b97f4a7c15ull * (uint64_t)i);
+  for (int st = 32; st > 0; st >>= 1) h += __shfl_down(h, st, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)h);
+}
+void fingerprint(hipStream_t s, int64_t n, const double *v, uint64_t *out) {
+  CFSA_HIP(hipMemsetAsync(out, 0, sizeof(uint64_t), s));
+  hipLaunchKernelGGL(k_fingerprint, dim3(1024), dim3(kBlock), 0, s, n, v,
+                     reinterpret_cast<unsigned long long *>(out));
+  CFSA_LAUNCH_CHECK();
 }
 
-// x = 0, r = b, X0 = c0 D^-1 b (first Chebyshev step of the finest level), and
-// the pairs of the stopping norm |b|^2 (or |D^-1 b|^2)
+// x = 0, r = b, and for the V-cycle its finest right-hand side b0 = (T) b (not
+// written for T = double: the V-cycle reads r) and X0 = c0 D^-1 b; the pairs of
+// the stopping norm |b|^2 (or |D^-1 b|^2)
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_amg_cg_init(int n, const double *__restrict__ b,
                                                         const double *__restrict__ invd,
-                                                        double c0, double *__restrict__ x,
+                                                        const T *__restrict__ dinv0, double c0,
+                                                        double *__restrict__ x,
                                                         double *__restrict__ r,
-                                                        double *__restrict__ x0, int scaled,
+                                                        T *__restrict__ b0, T *__restrict__ x0,
+                                                        int scaled,
                                                         double *__restrict__ partials) {
   DD acc[1];
   dd_zero(acc[0]);
@@ -2510,15 +2560,17 @@ __global__ __launch_bounds__(kBlock) void k_amg_cg_init(int n, const double *__r
     const double bi = b[i], d = invd[i];
     x[i] = 0.0;
     r[i] = bi;
-    x0[i] = c0 * (d * bi);
+    const T bt = (T)bi;
+    if (b0) b0[i] = bt;
+    x0[i] = (T)c0 * (dinv0[i] * bt);
     const double sn = scaled ? d * bi : bi;
     dd_fma(acc[0], sn, sn);
   }
   block_partials_binned<1, 1>(acc, partials);
 }
 
-// state of the solve from |b|^2 (pa) and rho = b . z0 (pc); p = z0 is copied by
-// the host
+// state of the solve from |b|^2 (pa) and rho = b . z0 (pc); p = z0 is set by
+// k_amg_cg_p0
 __global__ void k_amg_cg_s_init(KrylovState *st, const double *pa, int Ga, const double *pc,
                                 int Gc) {
   __shared__ double bb[1], rz[1];
@@ -2557,15 +2609,26 @@ __global__ void k_amg_cg_s_init(KrylovState *st, const double *pa, int Ga, const
   st->any_active = st->active[0];
 }
 
-// alpha = rho / p.Ap; r -= alpha q; X0 = c0 D^-1 r for the V-cycle; pairs of
-// the stopping norm.  The snapshot of `active` is what k_amg_cg_p updates x by.
+// p = z0 (in FP64)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_amg_cg_p0(int n, const T *__restrict__ z,
+                                                      double *__restrict__ p) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    p[i] = (double)z[i];
+}
+
+// alpha = rho / p.Ap; r -= alpha q; b0 = (T) r and X0 = c0 D^-1 r for the
+// V-cycle; pairs of the stopping norm.  The snapshot of `active` is what
+// k_amg_cg_p updates x by.
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_amg_cg_xr(int n, KrylovState *__restrict__ st,
                                                       const double *__restrict__ pq_partials,
                                                       int Gq, int par,
                                                       const double *__restrict__ q,
-                                                      const double *__restrict__ invd, double c0,
-                                                      double *__restrict__ r,
-                                                      double *__restrict__ x0, int scaled,
+                                                      const double *__restrict__ invd,
+                                                      const T *__restrict__ dinv0, double c0,
+                                                      double *__restrict__ r, T *__restrict__ b0,
+                                                      T *__restrict__ x0, int scaled,
                                                       double *__restrict__ partials) {
   if (blockIdx.x == 0 && threadIdx.x == 0) st->act_s[par][0] = st->active[0];
   if (!st->active[0]) return;  // grid-uniform; the pairs are not read then
@@ -2582,7 +2645,9 @@ __global__ __launch_bounds__(kBlock) void k_amg_cg_xr(int n, KrylovState *__rest
     const double ri = r[i] - al * q[i];
     const double d = invd[i];
     r[i] = ri;
-    x0[i] = c0 * (d * ri);
+    const T rt = (T)ri;
+    if (b0) b0[i] = rt;
+    x0[i] = (T)c0 * (dinv0[i] * rt);
     const double sn = scaled ? d * ri : ri;
     dd_fma(acc[0], sn, sn);
   }
@@ -2592,10 +2657,11 @@ __global__ __launch_bounds__(kBlock) void k_amg_cg_xr(int n, KrylovState *__rest
 // Every block reduces the stopping pairs (pa) and r . z (pc) and decides as
 // Eigen's conjugate_gradient does (block 0 writes the state); then
 // x += alpha p and, if the solve continues, p = z + beta p.
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_amg_cg_p(int n, KrylovState *__restrict__ st,
                                                      const double *__restrict__ pa, int Ga,
                                                      const double *__restrict__ pc, int Gc,
-                                                     int par, const double *__restrict__ z,
+                                                     int par, const T *__restrict__ z,
                                                      double *__restrict__ p,
                                                      double *__restrict__ x) {
   __shared__ double beta_s[1];
@@ -2614,16 +2680,18 @@ __global__ __launch_bounds__(kBlock) void k_amg_cg_p(int n, KrylovState *__restr
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double pi = p[i];
     x[i] = x[i] + al * pi;
-    if (cont) p[i] = z[i] + be * pi;
+    if (cont) p[i] = (double)z[i] + be * pi;
   }
 }
 
-void amg_cg_init(hipStream_t s, int n, const double *b, const double *invd, double c0, double *x,
-                 double *r, double *x0, int scaled, double *partials, int *grid) {
+template <class T>
+void amg_cg_init(hipStream_t s, int n, const double *b, const double *invd, const T *dinv0,
+                 double c0, double *x, double *r, T *b0, T *x0, int scaled, double *partials,
+                 int *grid) {
   const int G = vec_grid(n);
   *grid = G;
-  hipLaunchKernelGGL(k_amg_cg_init, dim3(G), dim3(kBlock), 0, s, n, b, invd, c0, x, r, x0, scaled,
-                     partials);
+  hipLaunchKernelGGL((k_amg_cg_init<T>), dim3(G), dim3(kBlock), 0, s, n, b, invd, dinv0, c0, x,
+                     r, b0, x0, scaled, partials);
   CFSA_LAUNCH_CHECK();
 }
 void amg_cg_s_init(hipStream_t s, KrylovState *st, const double *pa, int Ga, const double *pc,
@@ -2631,20 +2699,38 @@ void amg_cg_s_init(hipStream_t s, KrylovState *st, const double *pa, int Ga, con
   hipLaunchKernelGGL(k_amg_cg_s_init, dim3(1), dim3(kBlock), 0, s, st, pa, Ga, pc, Gc);
   CFSA_LAUNCH_CHECK();
 }
+template <class T>
+void amg_cg_p0(hipStream_t s, int n, const T *z, double *p) {
+  hipLaunchKernelGGL((k_amg_cg_p0<T>), dim3(vec_grid(n)), dim3(kBlock), 0, s, n, z, p);
+  CFSA_LAUNCH_CHECK();
+}
+template <class T>
 void amg_cg_xr(hipStream_t s, int n, KrylovState *st, const double *pq, int Gq, int par,
-               const double *q, const double *invd, double c0, double *r, double *x0, int scaled,
-               double *partials, int *grid) {
+               const double *q, const double *invd, const T *dinv0, double c0, double *r, T *b0,
+               T *x0, int scaled, double *partials, int *grid) {
   const int G = vec_grid(n);
   *grid = G;
-  hipLaunchKernelGGL(k_amg_cg_xr, dim3(G), dim3(kBlock), 0, s, n, st, pq, Gq, par, q, invd, c0, r,
-                     x0, scaled, partials);
+  hipLaunchKernelGGL((k_amg_cg_xr<T>), dim3(G), dim3(kBlock), 0, s, n, st, pq, Gq, par, q, invd,
+                     dinv0, c0, r, b0, x0, scaled, partials);
   CFSA_LAUNCH_CHECK();
 }
+template <class T>
 void amg_cg_p(hipStream_t s, int n, KrylovState *st, const double *pa, int Ga, const double *pc,
-              int Gc, int par, const double *z, double *p, double *x) {
-  hipLaunchKernelGGL(k_amg_cg_p, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, st, pa, Ga, pc, Gc,
-                     par, z, p, x);
+              int Gc, int par, const T *z, double *p, double *x) {
+  hipLaunchKernelGGL((k_amg_cg_p<T>), dim3(vec_grid(n)), dim3(kBlock), 0, s, n, st, pa, Ga, pc,
+                     Gc, par, z, p, x);
   CFSA_LAUNCH_CHECK();
 }
+#define CFSA_INST_AMG(T)                                                                       \
+  template void amg_cg_init<T>(hipStream_t, int, const double *, const double *, const T *,     \
+                               double, double *, double *, T *, T *, int, double *, int *);     \
+  template void amg_cg_p0<T>(hipStream_t, int, const T *, double *);                            \
+  template void amg_cg_xr<T>(hipStream_t, int, KrylovState *, const double *, int, int,         \
+                             const double *, const double *, const T *, double, double *, T *,  \
+                             T *, int, double *, int *);                                        \
+  template void amg_cg_p<T>(hipStream_t, int, KrylovState *, const double *, int,               \
+                            const double *, int, int, const T *, double *, double *);
+CFSA_INST_AMG(double)
+CFSA_INST_AMG(float)
 
 }  // namespace cfsa

@@ -138,39 +138,55 @@ template <class T, int K>
 void cg_p(hipStream_t s, int n, KrylovState *st, const double *partials, int Gv, int par,
           const T *r, const T *invd, T *p, T *x);
 
-// ---- AMG-preconditioned CG (single right-hand side, FP64) ----
-// One level of the smoothed-aggregation hierarchy on the device.  Level 0's A
+// ---- AMG-preconditioned CG (single right-hand side) ----
+// One level of the smoothed-aggregation hierarchy on the device, in the V-cycle's
+// value type T (double; float for the mixed precision = 1 mode).  Level 0's A
 // is the pressure operator in its Krylov layout (sliced-ELL or CSR-stream), its
-// b the CG residual and its y the preconditioned residual z; coarser levels'
-// A, and every level's P (n x nc) and R (nc x n), are CSR-stream.
+// b the CG residual (T = double) or its T copy, and its y the preconditioned
+// residual z; coarser levels' A, and every level's P (n x nc) and R (nc x n),
+// are CSR-stream (sliced-ELL for rows too long for a window).
+template <class T>
 struct AmgLevelView {
   int n = 0;
   DevCsr A, P, R;
-  const double *Av = nullptr, *Pv = nullptr, *Rv = nullptr;
-  const double *dinv = nullptr;
-  double *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;
+  const T *Av = nullptr, *Pv = nullptr, *Rv = nullptr;
+  const T *dinv = nullptr;
+  T *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;
   double c0 = 0.0, c1 = 0.0, c2 = 0.0;
 };
 // y0 = V(b0): pre-smoothing (x0 = c0 D^-1 b0 must be in lv[0].x), residual,
 // restriction, coarser levels, prolongation, post-smoothing; the coarsest
 // level is the dense inverse cinv (nc x nc) or, with coarse_jacobi, D^-1.
-// Emits the pairs of b0 . y0 into partials (*grid per value).  mask: the
-// solve's active flag (nullptr: always run).
-void amg_vcycle(hipStream_t s, const AmgLevelView *lv, int nlev, const double *cinv, int nc,
-                bool coarse_jacobi, const int *mask, double *partials, int *grid);
-// x = 0, r = b, x0 = c0 D^-1 b and the stopping-norm pairs of b
-void amg_cg_init(hipStream_t s, int n, const double *b, const double *invd, double c0, double *x,
-                 double *r, double *x0, int scaled, double *partials, int *grid);
+// Emits the pairs of rdot . y0 (rdot: the FP64 CG residual) into partials
+// (*grid per value).  mask: the solve's active flag (nullptr: always run).
+template <class T>
+void amg_vcycle(hipStream_t s, const AmgLevelView<T> *lv, int nlev, const T *cinv, int nc,
+                bool coarse_jacobi, const double *rdot, const int *mask, double *partials,
+                int *grid);
+// *out = order-independent 64-bit fingerprint of v[0..n) (not for use inside a
+// captured graph: it clears *out with a memset)
+void fingerprint(hipStream_t s, int64_t n, const double *v, uint64_t *out);
+// x = 0, r = b, b0 = (T) b (skipped when b0 is null), x0 = c0 D0^-1 b and the
+// stopping-norm pairs of b (invd: the FP64 Jacobi diagonal; dinv0: level 0's in T)
+template <class T>
+void amg_cg_init(hipStream_t s, int n, const double *b, const double *invd, const T *dinv0,
+                 double c0, double *x, double *r, T *b0, T *x0, int scaled, double *partials,
+                 int *grid);
 // state from the stopping norm of b (pa) and rho = b . z0 (pc)
 void amg_cg_s_init(hipStream_t s, KrylovState *st, const double *pa, int Ga, const double *pc,
                    int Gc);
-// alpha, r -= alpha q, x0 = c0 D^-1 r, stopping-norm pairs of r
+// p = z0
+template <class T>
+void amg_cg_p0(hipStream_t s, int n, const T *z, double *p);
+// alpha, r -= alpha q, b0 = (T) r, x0 = c0 D0^-1 r, stopping-norm pairs of r
+template <class T>
 void amg_cg_xr(hipStream_t s, int n, KrylovState *st, const double *pq, int Gq, int par,
-               const double *q, const double *invd, double c0, double *r, double *x0, int scaled,
-               double *partials, int *grid);
+               const double *q, const double *invd, const T *dinv0, double c0, double *r, T *b0,
+               T *x0, int scaled, double *partials, int *grid);
 // convergence test, x += alpha p, p = z + beta p
+template <class T>
 void amg_cg_p(hipStream_t s, int n, KrylovState *st, const double *pa, int Ga, const double *pc,
-              int Gc, int par, const double *z, double *p, double *x);
+              int Gc, int par, const T *z, double *p, double *x);
 
 // ---- FEM element / node kernels ----
 void assemble_colour(hipStream_t s, int ntc, const int *order, const int *tets,

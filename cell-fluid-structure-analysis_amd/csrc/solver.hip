@@ -3,6 +3,8 @@
 // reads back Krylov convergence flags every few iterations.
 #include "solver.hpp"
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -699,7 +701,10 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
 // (amg.hpp), Eigen's conjugate_gradient step and stopping rule otherwise.  One
 // iteration: q = A p with p.q; alpha, r -= alpha q and the first smoothing step
 // of the finest level; the V-cycle z = M^-1 r with r.z; the convergence test,
-// x += alpha p and p = z + beta p.  Chunks of four iterations replay as graphs.
+// x += alpha p and p = z + beta p.  The CG is FP64; the V-cycle runs in T
+// (float in the mixed precision = 1 mode).  Chunks of four iterations replay as
+// graphs.
+template <class T>
 SolveResult NSSolver::pcg_amg(const double *b, double *x, double tol, int maxit) {
   ensure_amg();
   hipStream_t s = stream_;
@@ -707,36 +712,40 @@ SolveResult NSSolver::pcg_amg(const double *b, double *x, double tol, int maxit)
   KrylovState *st = d_state.p;
   const int *act = st->active;
   KrylovWork<double> &w = w1_;
-  AmgLevelView &top = amg_view_[0];
-  top.A = preA();
-  top.Av = preV();
-  top.dinv = d_invdL.p;
-  top.b = w.r.p;
+  AmgLevelView<T> *lv = amg_views<T>();
+  const T *cinv = amg_cinv<T>();
+  AmgLevelView<T> &top = lv[0];
+  const DevCsr &A = preA();
+  const double *Av = preV();
+  // T = double: the V-cycle's right-hand side is the CG residual itself
+  T *b0 = std::is_same<T, double>::value ? nullptr : top.b;
   const int L = (int)amg_view_.size();
   set_state_params(tol, maxit, 1);
   int Ga = 0, Gb = 0, Gc = 0;
-  amg_cg_init(s, n, b, d_invdL.p, top.c0, x, w.r.p, top.x, opt.stop_scaled, d_part_a.p, &Ga);
-  amg_vcycle(s, amg_view_.data(), L, amg_cinv_.p, amg_nc_, amg_cjac_, nullptr, d_part_c.p, &Gc);
+  amg_cg_init<T>(s, n, b, d_invdL.p, top.dinv, top.c0, x, w.r.p, b0, top.x, opt.stop_scaled,
+                 d_part_a.p, &Ga);
+  amg_vcycle<T>(s, lv, L, cinv, amg_nc_, amg_cjac_, w.r.p, nullptr, d_part_c.p, &Gc);
   amg_cg_s_init(s, st, d_part_a.p, Ga, d_part_c.p, Gc);
-  CFSA_HIP(hipMemcpyAsync(w.p.p, top.y, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  amg_cg_p0<T>(s, n, top.y, w.p.p);
   constexpr int chunk = 4;  // even: every chunk starts on parity slot 0
   const long max_chunks = (long)maxit / chunk + 16;
   auto body = [&]() {
     for (int j = 0; j < chunk; ++j) {
       const int par = j & 1;
-      spmv1_dot<double>(s, top.A, top.Av, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
-      amg_cg_xr(s, n, st, d_part_b.p, Gb, par, w.v.p, d_invdL.p, top.c0, w.r.p, top.x,
-                opt.stop_scaled, d_part_a.p, &Ga);
-      amg_vcycle(s, amg_view_.data(), L, amg_cinv_.p, amg_nc_, amg_cjac_, act, d_part_c.p, &Gc);
-      amg_cg_p(s, n, st, d_part_a.p, Ga, d_part_c.p, Gc, par, top.y, w.p.p, x);
+      spmv1_dot<double>(s, A, Av, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
+      amg_cg_xr<T>(s, n, st, d_part_b.p, Gb, par, w.v.p, d_invdL.p, top.dinv, top.c0, w.r.p, b0,
+                   top.x, opt.stop_scaled, d_part_a.p, &Ga);
+      amg_vcycle<T>(s, lv, L, cinv, amg_nc_, amg_cjac_, w.r.p, act, d_part_c.p, &Gc);
+      amg_cg_p<T>(s, n, st, d_part_a.p, Ga, d_part_c.p, Gc, par, top.y, w.p.p, x);
     }
   };
   const std::vector<uintptr_t> key = {
-      3, (uintptr_t)n, (uintptr_t)chunk, (uintptr_t)top.A.rowptr, (uintptr_t)top.A.sptr,
-      (uintptr_t)top.Av, (uintptr_t)d_invdL.p, (uintptr_t)x, (uintptr_t)w.r.p,
-      (uintptr_t)w.p.p, (uintptr_t)w.v.p, (uintptr_t)st, (uintptr_t)d_part_a.p,
-      (uintptr_t)d_part_b.p, (uintptr_t)d_part_c.p, (uintptr_t)opt.stop_scaled,
-      (uintptr_t)amg_builds_, (uintptr_t)top.x, (uintptr_t)top.y};
+      3, sizeof(T), (uintptr_t)n, (uintptr_t)chunk, (uintptr_t)A.rowptr, (uintptr_t)A.sptr,
+      (uintptr_t)Av, (uintptr_t)top.Av, (uintptr_t)d_invdL.p, (uintptr_t)top.dinv,
+      (uintptr_t)x, (uintptr_t)w.r.p, (uintptr_t)w.p.p, (uintptr_t)w.v.p, (uintptr_t)st,
+      (uintptr_t)d_part_a.p, (uintptr_t)d_part_b.p, (uintptr_t)d_part_c.p,
+      (uintptr_t)opt.stop_scaled, (uintptr_t)amg_builds_, (uintptr_t)top.b, (uintptr_t)top.x,
+      (uintptr_t)top.y};
   for (long ch = 0;; ++ch) {
     if (ch > max_chunks) throw std::runtime_error("AMG-PCG driver did not terminate");
     read_state();
@@ -749,6 +758,8 @@ SolveResult NSSolver::pcg_amg(const double *b, double *x, double tol, int maxit)
   res.ok = h_state->err[0] <= tol;
   return res;
 }
+template SolveResult NSSolver::pcg_amg<double>(const double *, double *, double, int);
+template SolveResult NSSolver::pcg_amg<float>(const double *, double *, double, int);
 
 // FP64 iterative refinement around FP32 inner Krylov solves: the residual
 // b - A x, its norm and the stopping test are FP64; matrix values, vectors and
@@ -859,7 +870,10 @@ SolveResult NSSolver::solve_momentum(const double *b, double *x) {
 }
 
 SolveResult NSSolver::solve_pressure(const double *b, double *x) {
-  if (opt.pressure_solver == 2) return pcg_amg(b, x, opt.pressure_tol, opt.max_iter);
+  // AMG: the CG is FP64; the mixed mode for both solves runs the V-cycle in FP32
+  if (opt.pressure_solver == 2)
+    return opt.precision == 1 ? pcg_amg<float>(b, x, opt.pressure_tol, opt.max_iter)
+                              : pcg_amg<double>(b, x, opt.pressure_tol, opt.max_iter);
   if (opt.precision == 1)
     return refine<1>(preA(), preV(), d_invdL.p, d_Lbc32.p, d_invdL32.p, b, x,
                      opt.pressure_tol, opt.max_iter, opt.pressure_solver);
@@ -1214,7 +1228,8 @@ SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double
   SolveResult r;
   if (method == 2) {
     if (which != 4) throw std::invalid_argument("krylov_solve: method 2 (AMG-PCG) solves L_bc");
-    r = pcg_amg(db.p, dx.p, tol, maxit);
+    r = precision == 1 ? pcg_amg<float>(db.p, dx.p, tol, maxit)
+                       : pcg_amg<double>(db.p, dx.p, tol, maxit);
   } else if (precision == 1) {
     if (nrhs == 3)
       r = refine<3>(A, momV(), d_invdA.p, d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit,

@@ -226,8 +226,9 @@ class NSSolver {
   SolveResult krylov_solve(int which, int method, int nrhs, const double *b, double *x,
                            double tol, int maxit, int precision);
   // one V-cycle of the pressure operator's AMG hierarchy applied to r (host
-  // arrays of nv values), building the hierarchy if needed
-  void amg_apply(const double *r, double *z);
+  // arrays of nv values), building the hierarchy if needed; precision 1: the
+  // FP32 V-cycle of the mixed mode
+  void amg_apply(const double *r, double *z, int precision);
   // levels of the AMG hierarchy (0 if none), rows and nonzeros per level (up to
   // cap entries), set-up time of the last build in ms, builds so far
   int amg_info(int cap, int64_t *rows, int64_t *nnz, double *setup_ms, int *builds);
@@ -321,21 +322,33 @@ class NSSolver {
   struct AmgDevMat {  // one operator: CSR-stream, or sliced-ELL for rows too long for it
     DBuf<int> rp, ci, win, sptr, rlen, perm;
     DBuf<double> v;
+    DBuf<float> v32;  // FP32 copy for the mixed-mode V-cycle
   };
   struct AmgDevLevel {
     AmgDevMat A, P, R;
     DBuf<double> dinv, b, x, y, r;
+    DBuf<float> dinv32, b32, x32, y32, r32;
   };
   std::vector<AmgDevLevel> amg_lv_;
-  std::vector<AmgLevelView> amg_view_;
+  std::vector<AmgLevelView<double>> amg_view_;
+  std::vector<AmgLevelView<float>> amg_view32_;
   DBuf<double> amg_cinv_;
+  DBuf<float> amg_cinv32_;
+  bool amg32_built_ = false;  // FP32 copies of the current hierarchy exist
   int amg_nc_ = 0;
   bool amg_cjac_ = false, amg_built_ = false, amg_lbc_changed_ = true;
   int amg_stale_ = 0, amg_builds_ = 0;
   double amg_setup_ms_ = 0.0;
-  std::vector<double> amg_vals_;  // L_bc values the hierarchy was built from
+  uint64_t amg_print_ = 0;          // fingerprint of the L_bc values last seen
+  DBuf<uint64_t> d_print_;
   std::vector<int64_t> amg_rows_, amg_nnz_;
   void ensure_amg();
+  void ensure_amg32();  // FP32 copies of the hierarchy (device conversions)
+  template <class T>
+  AmgLevelView<T> *amg_views();
+  template <class T>
+  const T *amg_cinv();
+  template <class T>
   SolveResult pcg_amg(const double *b, double *x, double tol, int maxit);
   // adjoint state
   DBuf<int> d_Ptslot;  // slot of (j, i) for each slot (i, j) of the P1 pattern
@@ -383,5 +396,15 @@ class NSSolver {
   SolveResult solve_momentum(const double *b, double *x);
   SolveResult solve_pressure(const double *b, double *x);
 };
+
+// the V-cycle views and coarse inverses per value type (amg_solver.hip)
+template <>
+AmgLevelView<double> *NSSolver::amg_views<double>();
+template <>
+AmgLevelView<float> *NSSolver::amg_views<float>();
+template <>
+const double *NSSolver::amg_cinv<double>();
+template <>
+const float *NSSolver::amg_cinv<float>();
 
 }  // namespace cfsa

@@ -55,7 +55,8 @@ typedef struct cfsa_solver_options {
   int max_iter;          /* 1000 */
   int pressure_solver;   /* 0 BiCGSTAB (reference), 1 Jacobi PCG, 2 CG preconditioned by one
                             V-cycle of a smoothed-aggregation AMG hierarchy of L_bc (built on
-                            the host from the current values; FP64 in every precision mode) */
+                            the host from the current values).  The CG is FP64 in every
+                            precision mode; with precision 1 the V-cycle runs in FP32 */
   int precision;         /* 0 FP64, 1 FP32 inner solves in FP64 iterative refinement for the
                             momentum and pressure solves, 2 for the momentum solve only */
   double inner_tol;      /* relative tolerance of each FP32 inner solve, 1e-4 */
@@ -293,8 +294,9 @@ int cfsa_ns_krylov_solve(cfsa_ns *h, int which, int method, int nrhs, const doub
                          cfsa_solve_result *res);
 /* AMG preconditioner of the pressure operator (extension, no counterpart in the
  * reference): one V-cycle z = M^-1 r (nv values each), building the hierarchy
- * from the current L_bc if needed */
-int cfsa_ns_amg_apply(cfsa_ns *h, const double *r, double *z);
+ * from the current L_bc if needed; precision 0 FP64, 1 the FP32 V-cycle of the
+ * mixed precision = 1 mode */
+int cfsa_ns_amg_apply(cfsa_ns *h, const double *r, double *z, int precision);
 /* the solver's current hierarchy: *levels (0 if none built), rows and nonzeros
  * of up to cap levels, set-up time of the last build (ms), builds so far; any
  * output pointer may be NULL */

@@ -67,14 +67,19 @@ def test_vcycle_matches_restatement(P, kind, layout):
         e = relerr(z, zw)
         print(f"{kind}: levels {H.levels()[0]}, V-cycle rel err {e:.2e}", flush=True)
         assert e <= 1e-12, e
+        # the FP32 V-cycle of the mixed mode (precision 1): FP32 rounding
+        e32 = relerr(g.amgApply(r, precision=1), zw)
+        print(f"{kind}: FP32 V-cycle rel err {e32:.2e}", flush=True)
+        assert e32 <= 1e-5, e32
     info = g.amgInfo()
     L = H.levels()[0]
     assert info["levels"] == L and info["builds"] == 1
     assert info["rows"] == [H.matrix(l, "A")[3][0] for l in range(L)]
 
 
-@pytest.mark.parametrize("kind", CASES)
-def test_amg_pcg_solves(P, kind):
+@pytest.mark.parametrize("kind,precision", [(k, 0) for k in CASES] + [("cyl", 1)])
+def test_amg_pcg_solves(P, kind, precision):
+    """precision 1: FP64 CG around the FP32 V-cycle."""
     pm = _mesh(P, kind)
     g = _solver(P, pm, kind, stop_rule=1)
     rp, col, val = g.getMatrix("Lbc")
@@ -83,7 +88,7 @@ def test_amg_pcg_solves(P, kind):
     d = A.diagonal()
     b = A @ np.random.default_rng(5).standard_normal(n)
     tol = 1e-10
-    xa, ra = g.krylovSolve("Lbc", b, method=2, tol=tol, maxit=1000)
+    xa, ra = g.krylovSolve("Lbc", b, method=2, tol=tol, maxit=1000, precision=precision)
     xj, rj = g.krylovSolve("Lbc", b, method=1, tol=tol, maxit=5000)
     assert ra["ok"] and rj["ok"]
     res = np.linalg.norm((b - A @ xa) / d) / np.linalg.norm(b / d)
@@ -92,12 +97,12 @@ def test_amg_pcg_solves(P, kind):
           f"{rj['iters'][0]}; scaled residual {res:.2e}, vs Jacobi {relerr(xa, xj):.2e}",
           flush=True)
     assert res <= 1.01 * tol
-    assert abs(ra["iters"][0] - itw) <= 1
+    assert abs(ra["iters"][0] - itw) <= (1 if precision == 0 else 3)
     assert 4 * ra["iters"][0] <= rj["iters"][0]
     assert relerr(xa, xj) <= 1e-6
 
 
-@pytest.mark.parametrize("kind,precision", [("kuhn06", 0), ("cyl", 0), ("cyl", 2)])
+@pytest.mark.parametrize("kind,precision", [("kuhn06", 0), ("cyl", 0), ("cyl", 2), ("cyl", 1)])
 def test_piso_steps_vs_oracle(P, O, kind, precision):
     """pressure_solver 2 against the FP64 oracle (Jacobi-CG pressure), both at
     momentum 1e-10 / pressure 1e-10 on the Jacobi-scaled rule, 3 steps."""
@@ -127,11 +132,13 @@ def test_piso_steps_vs_oracle(P, O, kind, precision):
         assert eu <= 1e-7 and ep <= 1e-6, (step, eu, ep)
 
 
-def test_graphs_and_repeats_bit_identical(P):
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graphs_and_repeats_bit_identical(P, precision):
     pm = _mesh(P, "cyl")
     outs = []
     for graphs in (1, 0, 1):
-        g = _solver(P, pm, "cyl", pressure_solver=2, stop_rule=1, graphs=graphs)
+        g = _solver(P, pm, "cyl", pressure_solver=2, stop_rule=1, graphs=graphs,
+                    precision=precision)
         for _ in range(2):
             assert g.pisoStep(), P.last_error()
         outs.append((g.getVelocity(), g.getPressure(), g.stepStats()["pressure_iters"]))
