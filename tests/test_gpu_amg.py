@@ -102,10 +102,13 @@ def test_amg_pcg_solves(P, kind, precision):
     assert relerr(xa, xj) <= 1e-6
 
 
-@pytest.mark.parametrize("kind,precision", [("kuhn06", 0), ("cyl", 0), ("cyl", 2), ("cyl", 1)])
-def test_piso_steps_vs_oracle(P, O, kind, precision):
+@pytest.mark.parametrize("kind,precision,simple",
+                         [("kuhn06", 0, False), ("cyl", 0, False), ("cyl", 2, False),
+                          ("cyl", 1, False), ("kuhn10n", 0, False), ("kuhn08", 0, True)])
+def test_piso_steps_vs_oracle(P, O, kind, precision, simple):
     """pressure_solver 2 against the FP64 oracle (Jacobi-CG pressure), both at
-    momentum 1e-10 / pressure 1e-10 on the Jacobi-scaled rule, 3 steps."""
+    momentum 1e-10 / pressure 1e-10 on the Jacobi-scaled rule, 3 steps (kuhn10n:
+    Neumann outlet with the reference node pinned; simple: simpleStep)."""
     pm = _mesh(P, kind)
     ncorr = 2 if kind == "cyl" else 1
     g = _solver(P, pm, kind, stop_rule=1, momentum_tol=1e-10, pressure_tol=1e-10,
@@ -117,12 +120,19 @@ def test_piso_steps_vs_oracle(P, O, kind, precision):
     o.setDt(0.01)
     o.setMeanInletVelocity((0.5, 0.0, 0.0))
     o.setNumberOfCorrections(ncorr)
+    if kind.endswith("n"):
+        o.setOutlet(1, 0.0)
+        o.setReferencePressureNode(5)
     o.setSolver(1e-10, 1e-10, 5000, 1)
     o.setMomentumSolver(1)
     o.setStopScaled(True)
     for step in range(3):
-        assert g.pisoStep(), P.last_error()
-        assert o.pisoStep() == 0
+        if simple:
+            assert g.simpleStep(), P.last_error()
+            assert o.simpleStep() == 0
+        else:
+            assert g.pisoStep(), P.last_error()
+            assert o.pisoStep() == 0
         st, so = g.stepStats(), o.stats()
         eu = relerr(g.getVelocity(), o.getVelocity())
         ep = relerr(g.getPressure(), o.getPressure())
