@@ -436,6 +436,30 @@ class NavierStokesIntegrator {
   int step_count_ = 0;
 };
 
+// NavierStokesSimpleSolver (NavierStokes.hpp:201-251): simpleStep() time
+// stepping over a shared solver; its time is the solver's own
+class NavierStokesSimpleSolver {
+ public:
+  explicit NavierStokesSimpleSolver(std::shared_ptr<NavierStokesSolver> ns_solver)
+      : solver_(std::move(ns_solver)) {}
+  bool step() { return solver_ ? solver_->simpleStep() : false; }
+  bool advanceSteps(int n) {
+    for (int i = 0; i < n; ++i)
+      if (!step()) return false;
+    return true;
+  }
+  bool runUntil(double target_time) {
+    while (solver_->getTime() < target_time)
+      if (!step()) return false;
+    return true;
+  }
+  double getTime() const { return solver_->getTime(); }
+  std::shared_ptr<NavierStokesSolver> getSolver() { return solver_; }
+
+ private:
+  std::shared_ptr<NavierStokesSolver> solver_;
+};
+
 }  // namespace api
 
 namespace io {

@@ -7,7 +7,9 @@ same scenario:
   (lines 43-124);
 * tests/cxx/coupled_fluid_step.cpp: the fluid side of CoupledSolver
   (CoupledSolver.cpp:28-48, 86-97, 113-129, 141-175, 186-189), including
-  boundary relabelling between steps and the force hand-off.
+  boundary relabelling between steps and the force hand-off;
+* tests/cxx/simple_solver.cpp: NavierStokesSimpleSolver
+  (NavierStokes.hpp:201-251) against the oracle's simpleStep.
 """
 import subprocess
 from pathlib import Path
@@ -87,3 +89,18 @@ def test_cxx_coupled_fluid_step_matches_oracle(P, O, tmp_path):
         got = out(name).reshape(-1, 3)
         assert relerr(got, want) <= 1e-8, name
     assert relerr(out("div"), o.computeDivergence(u)) <= 1e-12
+
+
+def test_cxx_simple_solver_matches_oracle(O, tmp_path):
+    out = _build_and_run("simple_solver", tmp_path)
+    om = O.OracleMesh.structuredRectangularPrism(1.0, 1.0, 1.0, 4, 4, 4)
+    om.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+    o = O.OracleNS(om, 1060.0, 0.004)
+    o.setDt(0.01)
+    o.setMeanInletVelocity((0.5, 0.0, 0.0))
+    o.setOutlet(0, 2.0)
+    for _ in range(3):
+        assert o.simpleStep() == 0
+    u, p = out("u").reshape(-1, 3), out("p")
+    assert relerr(u, o.getVelocity()) <= 1e-8 and relerr(p, o.getPressure()) <= 1e-8
+    assert abs(out("t")[0] - 0.03) <= 1e-15
