@@ -179,6 +179,10 @@ _sig("cfsa_ns_krylov_solve", C.c_int, _p, C.c_int, C.c_int, C.c_int, _f64p, _f64
      C.c_int, C.c_int, C.POINTER(SolveResult))
 _sig("cfsa_ns_time_momentum_spmv", C.c_int, _p, C.c_int, C.c_int, C.POINTER(C.c_double))
 _sig("cfsa_ns_amg_apply", C.c_int, _p, _f64p, _f64p, C.c_int)
+for _n in ["solve_momentum_predictor", "solve_pressure_poisson", "correct_velocity",
+           "normalize_pressure"]:
+    _sig(f"cfsa_ns_{_n}", C.c_int, _p)
+_sig("cfsa_ns_solve_pressure_bicgstab", C.c_int, _p, _f64p, _f64p, C.POINTER(SolveResult))
 _sig("cfsa_ns_amg_info", C.c_int, _p, C.c_int, _p, _p, C.POINTER(C.c_double),
      C.POINTER(C.c_int), C.POINTER(C.c_int))
 _sig("cfsa_amg_build", C.c_int, C.c_int64, _i32p, _i32p, _f64p, C.c_double, C.c_int,
@@ -438,6 +442,31 @@ class NavierStokesSolver:
 
     def simpleStep(self) -> bool:
         return _lib.cfsa_ns_simple_step(self._h) == 0
+
+    # --- the parts of the step (NavierStokes.cpp:416-524, 586-668, 702-771, 804-813) ---
+    def solveMomentumPredictor(self) -> bool:
+        return _lib.cfsa_ns_solve_momentum_predictor(self._h) == 0
+
+    def solvePressurePoisson(self) -> bool:
+        return _lib.cfsa_ns_solve_pressure_poisson(self._h) == 0
+
+    def correctVelocity(self) -> bool:
+        return _lib.cfsa_ns_correct_velocity(self._h) == 0
+
+    def normalizePressire(self) -> bool:
+        return _lib.cfsa_ns_normalize_pressure(self._h) == 0
+
+    def solvePressureBiCGSTAB(self, rhs):
+        """NavierStokes.cpp:773-802; returns (ok, x, iterations)."""
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        if rhs.shape != (self.sizes()["nv"],):
+            raise ValueError("solvePressureBiCGSTAB: rhs has one entry per vertex")
+        x = np.empty_like(rhs)
+        r = SolveResult()
+        st = _lib.cfsa_ns_solve_pressure_bicgstab(self._h, rhs, x, C.byref(r))
+        if st not in (0, 3):
+            _check(st)
+        return st == 0, x, r.iters[0]
 
     # --- setters ---
     def setMeanInletVelocity(self, v):

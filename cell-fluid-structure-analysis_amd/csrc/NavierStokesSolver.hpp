@@ -256,6 +256,29 @@ class NavierStokesSolver {
     after_step();
     return st == CFSA_OK;
   }
+  // the parts of the step (NavierStokes.cpp:416-524, 586-668, 702-771, 804-813)
+  bool solveMomentumPredictor() { return part(cfsa_ns_solve_momentum_predictor); }
+  bool solvePressurePoisson() { return part(cfsa_ns_solve_pressure_poisson); }
+  bool correctVelocity() { return part(cfsa_ns_correct_velocity); }
+  bool normalizePressire() { return part(cfsa_ns_normalize_pressure); }
+  // NavierStokes.cpp:773-802: x = L^-1 rhs (BiCGSTAB, 1e-8, 1000 iterations)
+  bool solvePressureBiCGSTAB(const std::vector<double> &rhs, std::vector<double> &x) {
+    if (rhs.size() != nv_) throw std::invalid_argument("solvePressureBiCGSTAB: rhs has nv entries");
+    x.assign(nv_, 0.0);
+    const int st = cfsa_ns_solve_pressure_bicgstab(h_, rhs.data(), x.data(), nullptr);
+    if (st == CFSA_ERR_PRESSURE_NOCONV) {
+      std::cerr << "[NS] BiCGSTAB failed" << std::endl;
+      return false;
+    }
+    check(st);
+    return true;
+  }
+  // NavierStokes.hpp:92-95: the GPU assembles the four operators in one pass,
+  // so each of these re-assembles all of them (the same values)
+  void buildGradientMatrices() { build_sparse_matrices(); }
+  void buildMassMatrix() { build_sparse_matrices(); }
+  void buildStiffnessMatrix() { build_sparse_matrices(); }
+  void buildPoissonMatrix() { build_sparse_matrices(); }
   bool hasNans() {
     int out = 0;
     check(cfsa_ns_has_nans(h_, &out));
@@ -353,6 +376,15 @@ class NavierStokesSolver {
     Field1 p;
   };
   size_t cur() const { return step_ & 1; }
+  bool part(int (*f)(cfsa_ns *)) {
+    if (mesh_) mesh_->sync();
+    const int st = f(h_);
+    if (st != CFSA_OK && st != CFSA_ERR_MOMENTUM_NOCONV && st != CFSA_ERR_PRESSURE_NOCONV &&
+        st != CFSA_ERR_NOT_INIT)
+      throw std::runtime_error(cfsa_last_error());
+    after_step();  // the fields changed: host copies are stale
+    return st == CFSA_OK;
+  }
   void after_step() {
     ++step_;
     state_[cur()] = Buf::kStale;

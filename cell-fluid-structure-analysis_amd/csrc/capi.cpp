@@ -386,6 +386,56 @@ int cfsa_ns_simple_step(cfsa_ns *h) {
   });
 }
 
+namespace {
+int substep(cfsa_ns *h, int (cfsa::NSSolver::*f)()) {
+  CHECK_PTR(h);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    sync_mesh(h);
+    const int st = (h->s.*f)();
+    if (st != CFSA_OK) g_last_error = h->s.last_error;
+    return st;
+  });
+}
+}  // namespace
+int cfsa_ns_solve_momentum_predictor(cfsa_ns *h) {
+  return substep(h, &cfsa::NSSolver::momentum_predictor_step);
+}
+int cfsa_ns_solve_pressure_poisson(cfsa_ns *h) {
+  return substep(h, &cfsa::NSSolver::pressure_poisson_step);
+}
+int cfsa_ns_correct_velocity(cfsa_ns *h) {
+  CHECK_PTR(h);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    sync_mesh(h);
+    h->s.correct_velocity_step();
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_normalize_pressure(cfsa_ns *h) {
+  CHECK_PTR(h);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.normalize_pressure_step();
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_solve_pressure_bicgstab(cfsa_ns *h, const double *rhs, double *x,
+                                    cfsa_solve_result *res) {
+  CHECK_PTR(h);
+  CHECK_PTR(rhs);
+  CHECK_PTR(x);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    sync_mesh(h);
+    const cfsa::SolveResult r = h->s.solve_poisson_bicgstab(rhs, x);
+    if (res) to_result(r, res);
+    if (!r.ok) return fail(CFSA_ERR_PRESSURE_NOCONV, "[NS] BiCGSTAB failed");
+    return CFSA_OK;
+  });
+}
+
 int cfsa_ns_set_mean_inlet_velocity(cfsa_ns *h, const double v[3]) {
   CHECK_PTR(h);
   CHECK_PTR(v);

@@ -977,6 +977,57 @@ int NSSolver::finish_step() {
   return has_nans() ? ST_NAN : ST_OK;
 }
 
+int NSSolver::momentum_predictor_step() {
+  if (!initialized) return ST_NOT_INIT;
+  prepare_operators();
+  const int st = momentum_predictor();
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  return st;
+}
+
+int NSSolver::pressure_poisson_step() {
+  if (!initialized) return ST_NOT_INIT;
+  prepare_operators();
+  const int st = pressure_poisson();
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  return st;
+}
+
+void NSSolver::correct_velocity_step() {
+  if (!initialized) throw std::runtime_error("[NS] Error: solver not initialized");
+  prepare_operators();
+  correct_velocity();
+  CFSA_HIP(hipStreamSynchronize(stream_));
+}
+
+void NSSolver::normalize_pressure_step() {
+  if (!initialized) throw std::runtime_error("[NS] Error: solver not initialized");
+  pressure_normalize(stream_, nv, d_p.p, reference_node, d_scalar.p);
+  CFSA_HIP(hipStreamSynchronize(stream_));
+}
+
+SolveResult NSSolver::solve_poisson_bicgstab(const double *rhs, double *x) {
+  if (!initialized) throw std::runtime_error("[NS] Error: solver not initialized");
+  prepare_operators();
+  d_bsb.upload(rhs, (size_t)nv);
+  d_bsx.alloc((size_t)nv);
+  d_bsinvd.alloc((size_t)nv);
+  inv_diag(stream_, nv, d_Pdiag.p, d_L.p, d_bsinvd.p);
+  const int saved = opt.stop_scaled;
+  opt.stop_scaled = 0;  // Eigen's |r| <= tol |b|
+  SolveResult r;
+  try {
+    r = bicgstab<double, 1>(P_, d_L.p, d_bsinvd.p, d_bsb.p, d_bsx.p, 1e-8, 1000, w1_);
+  } catch (...) {
+    opt.stop_scaled = saved;
+    throw;
+  }
+  opt.stop_scaled = saved;
+  CFSA_HIP(hipStreamSynchronize(stream_));
+  d_bsx.download(x, (size_t)nv);
+  return r;
+}
+
 int NSSolver::piso_step() {
   if (!initialized) {
     last_error = "[NS] Error: solver not initialized";

@@ -188,6 +188,16 @@ class NSSolver {
   void build_sparse_matrices();
   int piso_step();
   int simple_step();
+  // the step's parts as the reference exposes them (NavierStokes.cpp:416-524,
+  // 586-668, 702-771, 804-813): status codes as piso_step's
+  int momentum_predictor_step();
+  int pressure_poisson_step();
+  void correct_velocity_step();
+  void normalize_pressure_step();
+  // solvePressureBiCGSTAB (NavierStokes.cpp:773-802): BiCGSTAB with Jacobi on
+  // the Poisson matrix L itself (no penalty rows), tolerance 1e-8, 1000
+  // iterations, the reference's stopping rule; x (nv) gets the solution
+  SolveResult solve_poisson_bicgstab(const double *rhs, double *x);
   bool has_nans();
 
   // setters with side effects on cached operators
@@ -302,6 +312,7 @@ class NSSolver {
   DBuf<int> d_flag, d_mask;
   // Krylov
   KrylovWork<double> w3_, w1_;
+  DBuf<double> d_bsb, d_bsx, d_bsinvd;  // solve_poisson_bicgstab operands (kept: graph keys)
   KrylovWork<float> f3_, f1_;
   DBuf<float> d_b32, d_x32;
   DBuf<double> d_r64, d_part_a, d_part_b, d_part_c, d_part_r;

@@ -201,6 +201,22 @@ int cfsa_ns_build_sparse_matrices(cfsa_ns *h);
 int cfsa_ns_piso_step(cfsa_ns *h);
 /* NavierStokesSolver::simpleStep (NavierStokes.cpp:968-1009) */
 int cfsa_ns_simple_step(cfsa_ns *h);
+/* the parts of the step, public in the reference: solveMomentumPredictor
+ * (NavierStokes.cpp:416-524: u* from u and p), solvePressurePoisson (:586-668:
+ * p' from u*, p += relax_p p', outlet reset, normalisation), correctVelocity
+ * (:702-771: u from u* and p'), normalizePressire (:804-813).  Status codes as
+ * cfsa_ns_piso_step's; the time does not advance. */
+int cfsa_ns_solve_momentum_predictor(cfsa_ns *h);
+int cfsa_ns_solve_pressure_poisson(cfsa_ns *h);
+int cfsa_ns_correct_velocity(cfsa_ns *h);
+int cfsa_ns_normalize_pressure(cfsa_ns *h);
+/* NavierStokesSolver::solvePressureBiCGSTAB (NavierStokes.cpp:773-802): x = L^-1
+ * rhs by Jacobi BiCGSTAB on the Poisson matrix L (no penalty rows), tolerance
+ * 1e-8, 1000 iterations, from zero; CFSA_ERR_PRESSURE_NOCONV when it does not
+ * converge (the reference returns false); rhs and x hold nv values; res may be
+ * NULL */
+int cfsa_ns_solve_pressure_bicgstab(cfsa_ns *h, const double *rhs, double *x,
+                                    cfsa_solve_result *res);
 
 /* setters, NavierStokes.hpp:167-186 */
 int cfsa_ns_set_mean_inlet_velocity(cfsa_ns *h, const double v[3]);

@@ -9,7 +9,9 @@ same scenario:
   (CoupledSolver.cpp:28-48, 86-97, 113-129, 141-175, 186-189), including
   boundary relabelling between steps and the force hand-off;
 * tests/cxx/simple_solver.cpp: NavierStokesSimpleSolver
-  (NavierStokes.hpp:201-251) against the oracle's simpleStep.
+  (NavierStokes.hpp:201-251) against the oracle's simpleStep;
+* tests/cxx/step_parts.cpp: the public parts of the step
+  (NavierStokes.hpp:101-112) compose pisoStep bit for bit.
 """
 import subprocess
 from pathlib import Path
@@ -104,3 +106,7 @@ def test_cxx_simple_solver_matches_oracle(O, tmp_path):
     u, p = out("u").reshape(-1, 3), out("p")
     assert relerr(u, o.getVelocity()) <= 1e-8 and relerr(p, o.getPressure()) <= 1e-8
     assert abs(out("t")[0] - 0.03) <= 1e-15
+
+
+def test_cxx_step_parts(tmp_path):
+    _build_and_run("step_parts", tmp_path)
