@@ -21,9 +21,33 @@ double diag_of(const HostCsr &A, int64_t i) {
   return 0.0;
 }
 
+// A without its stored zeros.  The Kuhn meshes' P1 Laplacian stores half of its
+// entries as exact zeros (the cotangent weight of an edge opposite a right
+// dihedral angle), and products carry them on: every operator of the hierarchy
+// is kept free of them.  A row sum that skips zero terms is the same number
+// (s + 0 = s), so this changes no value, only the work per product.
+HostCsr drop_zeros(const HostCsr &A) {
+  HostCsr B;
+  B.n = A.n;
+  B.m = A.m;
+  B.rp.assign((size_t)A.n + 1, 0);
+  B.ci.reserve(A.ci.size());
+  B.v.reserve(A.v.size());
+  for (int64_t i = 0; i < A.n; ++i) {
+    for (int32_t k = A.rp[(size_t)i]; k < A.rp[(size_t)i + 1]; ++k)
+      if (A.v[(size_t)k] != 0.0) {
+        B.ci.push_back(A.ci[(size_t)k]);
+        B.v.push_back(A.v[(size_t)k]);
+      }
+    B.rp[(size_t)i + 1] = (int32_t)B.ci.size();
+  }
+  return B;
+}
+
 // C = A B, rows in parallel.  Row i accumulates a_ik b_kj over k in A's column
-// order and, inside, j in B's column order; columns are emitted sorted.  The
-// result does not depend on the thread count.
+// order and, inside, j in B's column order; columns are emitted sorted, entries
+// that come out exactly zero are not stored.  The result does not depend on
+// the thread count.
 HostCsr spgemm(const HostCsr &A, const HostCsr &B) {
   if (A.m != B.n) throw std::logic_error("spgemm: inner sizes differ");
   HostCsr C;
@@ -60,11 +84,14 @@ HostCsr spgemm(const HostCsr &A, const HostCsr &B) {
         }
       }
       std::sort(touched.begin(), touched.end());
-      C.rp[(size_t)i + 1] = (int32_t)touched.size();
-      for (int32_t j : touched) {
-        oc.push_back(j);
-        ov.push_back(acc[(size_t)j]);
-      }
+      int32_t kept = 0;
+      for (int32_t j : touched)
+        if (acc[(size_t)j] != 0.0) {
+          oc.push_back(j);
+          ov.push_back(acc[(size_t)j]);
+          ++kept;
+        }
+      C.rp[(size_t)i + 1] = kept;
     }
   });
   for (int64_t i = 0; i < A.n; ++i) {
@@ -301,7 +328,7 @@ AmgHierarchy amg_build(const HostCsr &A0, const AmgParams &prm) {
   if (A0.n != A0.m) throw std::invalid_argument("AMG: the operator must be square");
   AmgHierarchy H;
   H.levels.emplace_back();
-  H.levels.back().A = A0;
+  H.levels.back().A = drop_zeros(A0);
   for (;;) {
     AmgLevel &L = H.levels.back();
     finish_level(L);
@@ -356,6 +383,7 @@ AmgHierarchy amg_build(const HostCsr &A0, const AmgParams &prm) {
           v = tv + v;
           placed = true;
         }
+        if (v == 0.0) continue;
         P.ci.push_back(j);
         P.v.push_back(v);
       }

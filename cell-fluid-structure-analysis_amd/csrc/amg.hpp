@@ -27,6 +27,10 @@
 // and after the coarse correction; the polynomial is symmetric in the A inner
 // product, so the V-cycle is a symmetric positive definite preconditioner.
 //
+// Entries that are exactly zero (half of the Kuhn meshes' P1 Laplacian, and
+// what products make of them) are not stored on any level: a sum that skips a
+// zero term is the same number, so values are unchanged and products cost less.
+//
 // Every loop is deterministic: sparse products run row-parallel with a fixed
 // per-row accumulation order and sorted columns, so the hierarchy does not
 // depend on the thread count.

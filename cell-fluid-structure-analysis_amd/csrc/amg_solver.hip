@@ -78,6 +78,8 @@ void NSSolver::ensure_amg() {
     amg_print_ = print;
     if (amg_stale_ < opt.amg_reuse) {
       ++amg_stale_;
+      amg_top_fresh_ = false;  // level 0 reads the current L_bc from here on
+      amg32_built_ = false;
       return;
     }
   }
@@ -114,7 +116,14 @@ void NSSolver::ensure_amg() {
     w.c2 = h.c2;
     amg_rows_[(size_t)l] = h.A.n;
     amg_nnz_[(size_t)l] = h.A.nnz();
-    if (l > 0) {
+    if (l == 0) {
+      // the zero-free fine operator, in the pressure Krylov operator's layout
+      upload_level_matrix(h.A, d.A, amg_top_A_, preA().sptr != nullptr);
+      // the CG's product with its fused inner product uses the Krylov
+      // operator's window scratch (this operator has no more windows than it)
+      amg_top_A_.wpart = preA().wpart;
+      amg_top_A_.wpart_cap = preA().wpart_cap;
+    } else {
       upload_level_matrix(h.A, d.A, w.A, sell);
       w.Av = d.A.v.p;
       d.dinv.upload(h.dinv.data(), h.dinv.size());
@@ -142,6 +151,7 @@ void NSSolver::ensure_amg() {
   else
     amg_cinv_.release();
   amg_stale_ = 0;
+  amg_top_fresh_ = true;
   amg_built_ = true;
   ++amg_builds_;
   amg_setup_ms_ =
@@ -179,6 +189,7 @@ void NSSolver::ensure_amg32() {
     } else {
       d.b32.alloc((size_t)w.n);  // FP32 copy of the CG residual
       f.b = d.b32.p;
+      if (amg_top_fresh_) conv(d.A.v, d.A.v32);
     }
     if (l < L - 1) {
       f.Pv = conv(d.P.v, d.P.v32);
@@ -199,8 +210,8 @@ void NSSolver::ensure_amg32() {
 template <>
 AmgLevelView<double> *NSSolver::amg_views<double>() {
   AmgLevelView<double> &top = amg_view_[0];
-  top.A = preA();
-  top.Av = preV();
+  top.A = amg_fine_A();
+  top.Av = amg_fine_v();
   top.dinv = d_invdL.p;
   top.b = w1_.r.p;  // the V-cycle reads the CG residual itself
   return amg_view_.data();
@@ -210,8 +221,8 @@ AmgLevelView<float> *NSSolver::amg_views<float>() {
   if (d_Lbc32.n == 0) throw std::logic_error("FP32 V-cycle without the FP32 pressure operator");
   ensure_amg32();
   AmgLevelView<float> &top = amg_view32_[0];
-  top.A = preA();
-  top.Av = d_Lbc32.p;
+  top.A = amg_fine_A();
+  top.Av = amg_top_fresh_ ? amg_lv_[0].A.v32.p : d_Lbc32.p;
   top.dinv = d_invdL32.p;
   return amg_view32_.data();
 }

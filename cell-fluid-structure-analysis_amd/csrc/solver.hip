@@ -84,6 +84,7 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   fields_wait();  // a pending host copy reads the snapshot buffers
   clear_graphs();
   amg_built_ = false;  // a hierarchy belongs to the previous mesh's operator
+  amg_top_fresh_ = false;
   amg_lbc_changed_ = true;
   amg_lv_.clear();
   amg_view_.clear();
@@ -715,8 +716,8 @@ SolveResult NSSolver::pcg_amg(const double *b, double *x, double tol, int maxit)
   AmgLevelView<T> *lv = amg_views<T>();
   const T *cinv = amg_cinv<T>();
   AmgLevelView<T> &top = lv[0];
-  const DevCsr &A = preA();
-  const double *Av = preV();
+  const DevCsr &A = amg_fine_A();
+  const double *Av = amg_fine_v();
   // T = double: the V-cycle's right-hand side is the CG residual itself
   T *b0 = std::is_same<T, double>::value ? nullptr : top.b;
   const int L = (int)amg_view_.size();

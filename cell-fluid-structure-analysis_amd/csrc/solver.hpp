@@ -353,6 +353,14 @@ class NSSolver {
   uint64_t amg_print_ = 0;          // fingerprint of the L_bc values last seen
   DBuf<uint64_t> d_print_;
   std::vector<int64_t> amg_rows_, amg_nnz_;
+  // level 0's own copy of L_bc without its stored zeros (amg_lv_[0].A), in the
+  // layout of the Krylov pressure operator; used by the V-cycle and the CG's
+  // product while it holds the current values (not after a change the
+  // hierarchy is kept through, amg_reuse)
+  DevCsr amg_top_A_{};
+  bool amg_top_fresh_ = false;
+  const DevCsr &amg_fine_A() { return amg_top_fresh_ ? amg_top_A_ : preA(); }
+  const double *amg_fine_v() { return amg_top_fresh_ ? amg_lv_[0].A.v.p : preV(); }
   void ensure_amg();
   void ensure_amg32();  // FP32 copies of the hierarchy (device conversions)
   template <class T>
