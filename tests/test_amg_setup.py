@@ -118,6 +118,25 @@ def test_levels_and_galerkin_products(case):
         A = Ah
 
 
+def test_no_stored_zeros(case):
+    """Every operator of the hierarchy keeps only nonzero entries; level 0 holds
+    exactly the nonzero entries of the input (the Kuhn Laplacian stores about
+    half of its entries as exact zeros), in the input's order."""
+    kind, A0, H = case
+    L, _, _ = H.levels()
+    for l in range(L):
+        for w in ("A", "P", "R") if l + 1 < L else ("A",):
+            rp, ci, v, shape = H.matrix(l, w)
+            assert np.all(v != 0.0), (kind, l, w)
+            assert np.all(np.diff(rp) >= 0)
+    rp, ci, v, _ = H.matrix(0, "A")
+    keep = A0.data != 0.0
+    np.testing.assert_array_equal(v, A0.data[keep])
+    np.testing.assert_array_equal(ci, A0.indices[keep])
+    if kind.startswith("kuhn"):
+        assert keep.mean() < 0.6
+
+
 def test_spectral_bounds_and_chebyshev_polynomial(case):
     kind, A0, H = case
     L, _, _ = H.levels()
