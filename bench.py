@@ -64,7 +64,8 @@ def make_solver(P, mesh, prof, args, precision, momentum_solver=0, initial_guess
                  check_every=args.check_every, stop_rule=args.stop_rule,
                  pressure_solver=args.pressure_solver, inner_tol=args.inner_tol,
                  initial_guess=initial_guess,
-                 reuse_pressure_correction=0 if args.resolve_correctors else 1)
+                 reuse_pressure_correction=0 if args.resolve_correctors else 1,
+                 elem_spmv=args.elem_spmv if precision else 0)
     return g
 
 
@@ -356,6 +357,9 @@ def main():
                     help="skip the whole-step oracle timing on the two cylinders (~1 min)")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--only", default=None, help="run only this mixed variant (profiling)")
+    ap.add_argument("--elem-spmv", type=int, default=0, choices=[0, 1],
+                    help="mixed variants: FP32 momentum operator applied tet by tet (1) or "
+                         "from its assembled CSR copy (0)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()

@@ -22,8 +22,8 @@ LIB = LIBDIR / "libcfsa_ns.so"
 DRIVER = LIBDIR / "cfsa_simulation"
 
 HIP_SOURCES = ["kernels.hip", "solver.hip", "adjoint.hip", "forces.hip", "noalloc.hip",
-               "amg_solver.hip", "capi.cpp"]
-HOST_SOURCES = ["mesh.cpp", "pattern.cpp", "vtk.cpp", "yaml_config.cpp", "amg.cpp"]
+               "amg_solver.hip", "elem_solver.hip", "capi.cpp"]
+HOST_SOURCES = ["mesh.cpp", "pattern.cpp", "vtk.cpp", "yaml_config.cpp", "amg.cpp", "elem_patch.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")

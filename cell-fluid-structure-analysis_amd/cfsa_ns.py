@@ -29,7 +29,7 @@ WALL, INLET, OUTLET, INTERNAL, UNDEFINED = 0, 1, 2, 3, 4
 OUTLET_DIRICHLET, OUTLET_NEUMANN, OUTLET_UNDEFINED = 0, 1, 2
 STATUS = {0: "OK", 1: "NOT_INIT", 2: "MOMENTUM_NOCONV", 3: "PRESSURE_NOCONV", 4: "NAN",
           5: "INVALID", 6: "DEVICE", 7: "IO"}
-MATRIX = {"M": 0, "K": 1, "lhs": 2, "L": 3, "Lbc": 4, "Gx": 5, "Gy": 6, "Gz": 7}
+MATRIX = {"M": 0, "K": 1, "lhs": 2, "L": 3, "Lbc": 4, "Gx": 5, "Gy": 6, "Gz": 7, "lhs32": 8}
 
 
 class SolverOptions(C.Structure):
@@ -41,7 +41,7 @@ class SolverOptions(C.Structure):
                 ("spmv_layout", C.c_int), ("momentum_solver", C.c_int),
                 ("check_every_1", C.c_int), ("graphs", C.c_int), ("initial_guess", C.c_int),
                 ("reuse_pressure_correction", C.c_int), ("restart_scale", C.c_double),
-                ("amg_reuse", C.c_int)]
+                ("amg_reuse", C.c_int), ("elem_spmv", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -196,6 +196,11 @@ _sig("cfsa_amg_level_get", C.c_int, _p, C.c_int, C.c_int, _i32p, _i32p, _f64p)
 _sig("cfsa_amg_level_params", C.c_int, _p, C.c_int, C.POINTER(C.c_double),
      C.POINTER(C.c_double), _f64p, _p)
 _sig("cfsa_amg_coarse_inverse", C.c_int, _p, _f64p)
+_sig("cfsa_elem_patches_build", C.c_int, C.c_int64, C.c_int64, C.c_int64, _i32p, _i32p, _f64p,
+     C.POINTER(_p))
+_sig("cfsa_elem_patches_free", C.c_int, _p)
+_sig("cfsa_elem_patches_sizes", C.c_int, _p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"))
+_sig("cfsa_elem_patches_get", C.c_int, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p)
 _sig("cfsa_ns_get_stream", C.c_int, _p, C.POINTER(_p))
 for _n in ["pressure_forces", "shear_stress", "total_fluid_forces"]:
     _sig(f"cfsa_ns_compute_{_n}", C.c_int, _p, _f64p)
@@ -658,10 +663,10 @@ class NavierStokesSolver:
         w = MATRIX[which] if isinstance(which, str) else int(which)
         x = np.ascontiguousarray(x, np.float64)
         sz = self.sizes()
-        if w >= 5:
+        if 5 <= w <= 7:
             y = np.empty((sz["nv"], 3), np.float64)
         else:
-            n = sz["nu"] if w <= 2 else sz["nv"]
+            n = sz["nu"] if w <= 2 or w == 8 else sz["nv"]
             y = np.empty((n, nrhs) if nrhs == 3 else (n,), np.float64)
         _check(_lib.cfsa_ns_apply_matrix(self._h, w, x, y, nrhs))
         return y
@@ -817,3 +822,57 @@ class HostAmg:
         inv = np.empty(nc * nc, np.float64)
         _check(_lib.cfsa_amg_coarse_inverse(self._h, inv))
         return inv.reshape(nc, nc)
+
+
+class HostElemPatches:
+    """The element patches the FP32 momentum operator uses with elem_spmv = 1
+    (cfsa_elem_patches_build), built on the host from a mesh, for inspection
+    and tests."""
+
+    def __init__(self, nv, tets, ten, xyz):
+        tets = np.ascontiguousarray(tets, np.int32)
+        ten = np.ascontiguousarray(ten, np.int32)
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        ne = int(ten.max()) + 1 if ten.size else 0
+        h = _p()
+        _check(_lib.cfsa_elem_patches_build(int(nv), len(tets), ne, tets, ten, xyz, C.byref(h)))
+        self._h = h
+        sz = np.zeros(8, np.int64)
+        _check(_lib.cfsa_elem_patches_sizes(self._h, sz))
+        npatch, nvisit, nnodes, nrows, nstarts = (int(v) for v in sz[:5])
+        self.max_rows, self.max_nodes, self.max_slots = (int(v) for v in sz[5:])
+        self.node_off = np.empty(npatch + 1, np.int32)
+        self.nodes = np.empty(nnodes, np.int32)
+        self.row_off = np.empty(npatch + 1, np.int32)
+        self.rows = np.empty(nrows, np.int32)
+        self.starts = np.empty(nstarts, np.uint16)
+        self.visit_off = np.empty(npatch + 1, np.int32)
+        self.vtet = np.empty(nvisit, np.int32)
+        self.vstat = np.empty((nvisit, 8), np.uint32)
+        self.tet_order = np.empty(len(tets), np.int32)
+        arrs = [self.node_off, self.nodes, self.row_off, self.rows, self.starts, self.visit_off,
+                self.vtet, self.vstat, self.tet_order]
+        _check(_lib.cfsa_elem_patches_get(self._h, *[a.ctypes.data_as(_p) for a in arrs]))
+        self.npatch = npatch
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.cfsa_elem_patches_free(self._h)
+            self._h = None
+
+    def local_ids(self):
+        """visits x 10 local node ids (12 bits each at bit 12 l of words 0-3)"""
+        w = self.vstat[:, :4].astype(object)
+        big = w[:, 0] | (w[:, 1] << 32) | (w[:, 2] << 64) | (w[:, 3] << 96)
+        return np.array([[(int(v) >> (12 * l)) & 0xFFF for l in range(10)] for v in big],
+                        np.int64).reshape(-1, 10)
+
+    def ranks(self):
+        """visits x 10 slot ranks (255: the node is not owned by the patch)"""
+        w = self.vstat[:, 4:7]
+        b = np.stack([(w >> (8 * k)) & 0xFF for k in range(4)], 2).reshape(-1, 12)
+        return b[:, :10].astype(np.int64)
+
+    def tets(self):
+        """the mesh tet of each visit, from word 7 of its record (patch numbering)"""
+        return self.tet_order[self.vstat[:, 7].astype(np.int64)].astype(np.int64)

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "amg.hpp"
+#include "elem_patch.hpp"
 #include "mesh.hpp"
 #include "solver.hpp"
 #include "vtk.hpp"
@@ -19,6 +20,9 @@ struct cfsa_mesh {
 };
 struct cfsa_amg {
   cfsa::AmgHierarchy H;
+};
+struct cfsa_elem_patches {
+  cfsa::ElemPatchHost E;
 };
 struct cfsa_ns {
   cfsa::NSSolver s;
@@ -78,6 +82,7 @@ cfsa::SolverOptions to_opts(const cfsa_solver_options &o) {
   s.reuse_correction = o.reuse_pressure_correction;
   s.restart_scale = o.restart_scale;
   s.amg_reuse = o.amg_reuse;
+  s.elem_spmv = o.elem_spmv;
   return s;
 }
 
@@ -517,7 +522,8 @@ int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt) {
       opt->spmv_layout < 0 || opt->spmv_layout > 2 || opt->momentum_solver < 0 ||
       opt->momentum_solver > 1 || opt->graphs < 0 || opt->graphs > 1 ||
       opt->initial_guess < 0 || opt->initial_guess > 1 || opt->reuse_pressure_correction < 0 ||
-      opt->reuse_pressure_correction > 1 || !(opt->restart_scale > 0) || opt->amg_reuse < 0)
+      opt->reuse_pressure_correction > 1 || !(opt->restart_scale > 0) || opt->amg_reuse < 0 ||
+      opt->elem_spmv < 0 || opt->elem_spmv > 1)
     return fail(CFSA_ERR_INVALID, "invalid solver options");
   h->s.opt = to_opts(*opt);
   return CFSA_OK;
@@ -545,6 +551,7 @@ int cfsa_ns_get_options(const cfsa_ns *h, cfsa_solver_options *o) {
   o->reuse_pressure_correction = s.reuse_correction;
   o->restart_scale = s.restart_scale;
   o->amg_reuse = s.amg_reuse;
+  o->elem_spmv = s.elem_spmv;
   return CFSA_OK;
 }
 
@@ -732,7 +739,8 @@ int cfsa_ns_apply_matrix(cfsa_ns *h, int which, const double *x, double *y, int 
   CHECK_PTR(x);
   CHECK_PTR(y);
   if (int e = check_init(h)) return e;
-  if (which < 0 || which > 7 || (nrhs != 1 && nrhs != 3) || (which >= 3 && which <= 4 && nrhs != 1))
+  if (which < 0 || which > 8 || (nrhs != 1 && nrhs != 3) ||
+      (which >= 3 && which <= 4 && nrhs != 1) || (which == 8 && nrhs != 3))
     return fail(CFSA_ERR_INVALID, "apply_matrix: bad matrix id or nrhs");
   return guard([&]() -> int {
     h->s.apply_matrix(which, x, y, nrhs);
@@ -819,6 +827,64 @@ int cfsa_amg_build(int64_t n, const int32_t *rowptr, const int32_t *col, const d
 }
 int cfsa_amg_free(cfsa_amg *h) {
   delete h;
+  return CFSA_OK;
+}
+
+int cfsa_elem_patches_build(int64_t nv, int64_t nt, int64_t ne, const int32_t *tets,
+                            const int32_t *ten, const double *xyz, cfsa_elem_patches **out) {
+  CHECK_PTR(tets);
+  CHECK_PTR(ten);
+  CHECK_PTR(xyz);
+  CHECK_PTR(out);
+  *out = nullptr;
+  if (nv < 1 || nt < 1 || ne < 0 || nv + ne >= INT32_MAX || nt * 10 >= INT32_MAX)
+    return fail(CFSA_ERR_INVALID, "mesh sizes out of range");
+  return guard([&]() -> int {
+    for (int64_t i = 0; i < nt * 4; ++i)
+      if (tets[i] < 0 || tets[i] >= nv) throw std::invalid_argument("tet vertex out of range");
+    for (int64_t i = 0; i < nt * 6; ++i)
+      if (ten[i] < 0 || ten[i] >= ne) throw std::invalid_argument("edge node out of range");
+    auto h = std::make_unique<cfsa_elem_patches>();
+    h->E = cfsa::build_elem_patches(nv + ne, nv, nt, tets, ten, xyz);
+    *out = h.release();
+    return CFSA_OK;
+  });
+}
+int cfsa_elem_patches_free(cfsa_elem_patches *h) {
+  delete h;
+  return CFSA_OK;
+}
+int cfsa_elem_patches_sizes(const cfsa_elem_patches *h, int64_t out[8]) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  const cfsa::ElemPatchHost &E = h->E;
+  out[0] = E.npatch;
+  out[1] = E.nvisit;
+  out[2] = (int64_t)E.nodes.size();
+  out[3] = (int64_t)E.rows.size();
+  out[4] = (int64_t)E.starts.size();
+  out[5] = E.max_rows;
+  out[6] = E.max_nodes;
+  out[7] = E.max_slots;
+  return CFSA_OK;
+}
+int cfsa_elem_patches_get(const cfsa_elem_patches *h, int32_t *node_off, int32_t *nodes,
+                          int32_t *row_off, int32_t *rows, uint16_t *starts, int32_t *visit_off,
+                          int32_t *vtet, uint32_t *vstat, int32_t *tet_order) {
+  CHECK_PTR(h);
+  const cfsa::ElemPatchHost &E = h->E;
+  auto put = [](const auto &v, auto *o) {
+    if (o) std::copy(v.begin(), v.end(), o);
+  };
+  put(E.node_off, node_off);
+  put(E.nodes, nodes);
+  put(E.row_off, row_off);
+  put(E.rows, rows);
+  put(E.starts, starts);
+  put(E.visit_off, visit_off);
+  put(E.vtet, vtet);
+  put(E.vstat, vstat);
+  put(E.tet_order, tet_order);
   return CFSA_OK;
 }
 namespace {

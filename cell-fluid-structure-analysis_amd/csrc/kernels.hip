@@ -12,6 +12,7 @@
 #include <cfloat>
 
 #include "dd.hpp"
+#include "elem_patch.hpp"
 #include "pattern.hpp"
 
 namespace cfsa {
@@ -254,10 +255,288 @@ void sell_fill(hipStream_t s, int64_t stored, const int *src, const double *in, 
 template void sell_fill<double>(hipStream_t, int64_t, const int *, const double *, double *);
 template void sell_fill<float>(hipStream_t, int64_t, const int *, const double *, float *);
 
-// Launches the sliced-ELL kernel when A carries that layout, else CSR-stream;
-// returns the grid size (the number of partial pairs per value).
+// ===========================================================================
+// Element-patch SpMV (FP32 momentum operator, 3 right-hand sides)
+// ===========================================================================
+// One block per patch (elem_patch.hpp), in the XCD-chunked order of the
+// CSR-stream windows.  The patch's x values are staged in LDS first (one
+// gather per node instead of one per node and visit).  Phase 1: each thread
+// takes visits (tets) of the patch, reads x at the tet's 10 nodes and applies
+//   lhs_e = (rho V / dt) C + B A A^T B^T,  A = sqrt(V / mu) Phi,
+// writing the contribution to each owned node into that (row, tet) pair's LDS
+// slot.  Phase 2: the thread owning (row, component) sums the row's slots in
+// ascending tet order, adds the row's diagonal penalty and hands the sum to
+// the operator's epilogue (the same functors as the CSR kernels), so the
+// product and its fused inner products are deterministic.
+constexpr int kElemBlock = 512;
+
+template <int ND, class Op>
+__global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_elem_spmv(ElemPatchDev E, Op op,
+                                                          double *__restrict__ wpart) {
+  __shared__ float cnt[kElemSlots * 3];
+  __shared__ float xs[kElemNodes * 3];  // x at the patch's nodes
+  __shared__ unsigned short st[kElemRows + 1];
+  constexpr int NDs = ND > 0 ? ND : 1;
+  constexpr int NV = 3 * ND;
+  static_assert(sizeof(float) * kElemSlots * 3 >= sizeof(double) * 2 * NDs * kElemBlock,
+                "LDS reuse");
+  if (!op.any()) return;
+  const int q = (int)(blockIdx.x >> 3);
+  const int p = ((q / kStreamChunk) * 8 + (int)(blockIdx.x & 7)) * kStreamChunk + q % kStreamChunk;
+  if (p >= E.npatch) return;
+  const int tid = threadIdx.x;
+  const int n0 = E.node_off[p], nn = E.node_off[p + 1] - n0;
+  const int r0 = E.row_off[p], nr = E.row_off[p + 1] - r0;
+  const int v0 = E.visit_off[p], nvis = E.visit_off[p + 1] - v0;
+  const float *__restrict__ x = op.x;
+  // Everything a thread reads from global memory is issued up front, so the
+  // block's latencies overlap: the operands of its rows' epilogues, its first
+  // visit record, then the staging gathers; each later visit record is loaded
+  // while the previous visit is computed.
+  constexpr int RP = kElemBlock / 3;  // rows per pass of the row sums
+  constexpr int PMAX = (kElemRows + RP - 1) / RP;
+  const int myc = tid % 3;
+  const bool rowthr = tid < 3 * RP && op.active(myc);
+  typename Op::Pre pre2[PMAX];
+  float pen2[PMAX];
+#pragma unroll
+  for (int k = 0; k < PMAX; ++k) {
+    const int i = tid / 3 + k * RP;
+    pen2[k] = 0.0f;
+    if (rowthr && i < nr) {
+      pre2[k] = op.pre(r0 + i, myc);
+      pen2[k] = E.pen[r0 + i];
+    }
+  }
+  // visit records are stored as two planes (one 16-byte word per visit each),
+  // so every load of a wavefront reads one contiguous run; the geometry is
+  // per tet (three planes of one float4 per tet), gathered by the record's tet
+  const size_t nvt = (size_t)E.nvisit, ntt = (size_t)E.ntet;
+  const uint4 *vs = reinterpret_cast<const uint4 *>(E.vstat);
+  const float4 *vg = reinterpret_cast<const float4 *>(E.vgeo);
+  uint4 sa{}, sb{};
+  float4 ga{}, gb{}, gc{};
+  if (tid < nvis) {
+    const size_t vi = (size_t)(v0 + tid);
+    sa = vs[vi];
+    sb = vs[nvt + vi];
+  }
+  // the patch's x values, one gather per node (a node serves ~14 visits); all
+  // of a thread's node ids, then all of its gathers, are issued before use
+  static_assert(kElemNodes % kElemBlock == 0, "whole staging passes");
+  constexpr int NPT = kElemNodes / kElemBlock;
+  {
+    int jj[NPT];
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+      const int i = tid + u * kElemBlock;
+      jj[u] = i < nn ? E.nodes[n0 + i] : -1;
+    }
+    float xv[NPT][3];
+#pragma unroll
+    for (int u = 0; u < NPT; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xv[u][c] = jj[u] >= 0 ? x[(size_t)jj[u] * 3 + c] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < NPT; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xs[(tid + u * kElemBlock) * 3 + c] = xv[u][c];
+  }
+  if (tid < nvis) {
+    const size_t t = sb.w;
+    ga = vg[t];
+    gb = vg[ntt + t];
+    gc = vg[2 * ntt + t];
+  }
+  for (int i = tid; i <= nr; i += kElemBlock) st[i] = E.starts[r0 + p + i];
+  __syncthreads();
+  for (int v = tid; v < nvis; v += kElemBlock) {
+    const uint4 ca = sa, cb = sb;
+    const float4 gA = ga, gB = gb, gC = gc;
+    const bool more = v + kElemBlock < nvis;
+    if (more) {
+      const size_t vi = (size_t)(v0 + v + kElemBlock);
+      sa = vs[vi];
+      sb = vs[nvt + vi];
+    }
+    // 12-bit local ids at bit 12 l of words 0-3; 8-bit ranks in words 4-6
+    const uint64_t lo = (uint64_t)ca.y << 32 | ca.x, hi = (uint64_t)ca.w << 32 | ca.z;
+    const unsigned int w[3] = {cb.x, cb.y, cb.z};
+    int lid[10];
+#pragma unroll
+    for (int l = 0; l < 10; ++l) {
+      const int b = 12 * l;
+      const uint64_t f = b < 64 ? ((lo >> b) | (b + 12 > 64 ? hi << (64 - b) : 0)) : hi >> (b - 64);
+      lid[l] = (int)(f & 0xfffu);
+    }
+    float xe[10][3];
+#pragma unroll
+    for (int l = 0; l < 10; ++l) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xe[l][c] = xs[lid[l] * 3 + c];
+    }
+    const float m = gA.x;
+    const float a1[3] = {gA.y, gA.z, gA.w}, a2[3] = {gB.x, gB.y, gB.z}, a3[3] = {gB.w, gC.x, gC.y};
+    float ye[10][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float x0 = xe[0][c], x1 = xe[1][c], x2 = xe[2][c], x3 = xe[3][c];
+      const float e0 = xe[4][c], e1 = xe[5][c], e2 = xe[6][c], e3 = xe[7][c], e4 = xe[8][c],
+                  e5 = xe[9][c];
+      const float sv = (x0 + x1) + (x2 + x3);
+      const float se = ((e0 + e1) + (e2 + e3)) + (e4 + e5);
+      const float z0 = x0 + e0 + e1 + e2, z1 = x1 + e0 + e3 + e4, z2 = x2 + e1 + e3 + e5,
+                  z3 = x3 + e2 + e4 + e5;
+      const float d1 = z1 - z0, d2 = z2 - z0, d3 = z3 - z0;
+      float wv[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) wv[k] = d1 * a1[k] + d2 * a2[k] + d3 * a3[k];
+      const float s1 = a1[0] * wv[0] + a1[1] * wv[1] + a1[2] * wv[2];
+      const float s2 = a2[0] * wv[0] + a2[1] * wv[1] + a2[2] * wv[2];
+      const float s3 = a3[0] * wv[0] + a3[1] * wv[1] + a3[2] * wv[2];
+      const float s0 = -((s1 + s2) + s3);
+      // vertex rows: m (x_a / 20 + Sv / 20 + Se / 30); edge rows: m (x_k / 20 + Se / 60 + Sv / 30)
+      const float mv = m * (sv * (1.0f / 20.0f) + se * (1.0f / 30.0f));
+      const float me = m * (se * (1.0f / 60.0f) + sv * (1.0f / 30.0f));
+      const float m20 = m * (1.0f / 20.0f);
+      ye[0][c] = m20 * x0 + mv + s0;
+      ye[1][c] = m20 * x1 + mv + s1;
+      ye[2][c] = m20 * x2 + mv + s2;
+      ye[3][c] = m20 * x3 + mv + s3;
+      ye[4][c] = m20 * e0 + me + (s0 + s1);
+      ye[5][c] = m20 * e1 + me + (s0 + s2);
+      ye[6][c] = m20 * e2 + me + (s0 + s3);
+      ye[7][c] = m20 * e3 + me + (s1 + s2);
+      ye[8][c] = m20 * e4 + me + (s1 + s3);
+      ye[9][c] = m20 * e5 + me + (s2 + s3);
+    }
+#pragma unroll
+    for (int l = 0; l < 10; ++l) {
+      if (lid[l] < nr) {
+        const int rk = (int)((w[l / 4] >> (8 * (l % 4))) & 0xffu);
+        const int slot = (int)st[lid[l]] + rk;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cnt[slot * 3 + c] = ye[l][c];
+      }
+    }
+    if (more) {
+      const size_t t = sb.w;
+      ga = vg[t];
+      gb = vg[ntt + t];
+      gc = vg[2 * ntt + t];
+    }
+  }
+  __syncthreads();
+  DD dots[NDs];
+#pragma unroll
+  for (int d = 0; d < NDs; ++d) dd_zero(dots[d]);
+  if (rowthr) {
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k) {
+      const int i = tid / 3 + k * RP;
+      if (i < nr) {
+        float acc = 0.0f;
+        for (int sl = st[i]; sl < st[i + 1]; ++sl) acc += cnt[sl * 3 + myc];
+        // owned rows lead the node table; rows are numbered in patch order
+        if (pen2[k] != 0.0f) acc += pen2[k] * xs[i * 3 + myc];
+        op.epi(r0 + i, myc, acc, pre2[k], dots);
+      }
+    }
+  }
+  if constexpr (ND > 0) {
+    double *rh = reinterpret_cast<double *>(cnt);  // [ND][kElemBlock]
+    double *rl = rh + ND * kElemBlock;
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      rh[d * kElemBlock + tid] = dots[d].hi;
+      rl[d * kElemBlock + tid] = dots[d].lo;
+    }
+    __syncthreads();
+    // value v = (component c, product d) owns 32 lanes: lane l merges the
+    // threads c + 3 (l + 32 q), then a fixed shuffle tree
+    if (tid < NV * 32) {
+      const int v = tid / 32, l = tid % 32;
+      const int c = v / ND, d = v % ND;
+      DD s;
+      dd_zero(s);
+      for (int t = c + 3 * l; t < kElemBlock; t += 3 * 32)
+        dd_merge(s, rh[d * kElemBlock + t], rl[d * kElemBlock + t]);
+#pragma unroll
+      for (int sh = 16; sh > 0; sh >>= 1) {
+        const double oh = __shfl_down(s.hi, sh, 32);
+        const double ol = __shfl_down(s.lo, sh, 32);
+        dd_merge(s, oh, ol);
+      }
+      if (l == 0) {
+        wpart[2 * ((size_t)v * E.npatch + p)] = s.hi;
+        wpart[2 * ((size_t)v * E.npatch + p) + 1] = s.lo;
+      }
+    }
+  }
+}
+
+// Per-tet FP32 geometry (rho V / dt, sqrt(V / mu) grad phi_1..3 from the FP64
+// per-tet arrays; three planes of one float4 per tet) and each owned row's
+// diagonal penalty (Inlet / Wall).
+__global__ void k_elem_refresh(int64_t nt, int64_t nrows, const int *__restrict__ torder,
+                               const int *__restrict__ rows, const double *__restrict__ vol,
+                               const double *__restrict__ grad, double rho_dt, double inv_mu,
+                               const uint8_t *__restrict__ bc, float penalty,
+                               float4 *__restrict__ vgeo, float *__restrict__ pen) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < nt; j += stride) {
+    const int64_t t = torder[j];  // tet j of the patch numbering
+    const double V = vol[t];
+    const double sc = sqrt(V * inv_mu);
+    const double *g = grad + (size_t)t * 12;
+    vgeo[j] = make_float4((float)(rho_dt * V), (float)(sc * g[3]), (float)(sc * g[4]),
+                          (float)(sc * g[5]));
+    vgeo[nt + j] = make_float4((float)(sc * g[6]), (float)(sc * g[7]), (float)(sc * g[8]),
+                               (float)(sc * g[9]));
+    vgeo[2 * nt + j] = make_float4((float)(sc * g[10]), (float)(sc * g[11]), 0.0f, 0.0f);
+  }
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nrows; i += stride) {
+    const uint8_t b = bc[rows[i]];
+    pen[i] = (b == kInlet || b == kWall) ? penalty : 0.0f;
+  }
+}
+void elem_refresh(hipStream_t s, int64_t nt, int64_t nrows, const int *torder, const int *rows,
+                  const double *vol, const double *grad, double rho_dt, double inv_mu,
+                  const uint8_t *bc, double penalty, float *vgeo, float *pen) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_elem_refresh, dim3(4096), dim3(kBlock), 0, s, nt, nrows, torder, rows, vol,
+                     grad, rho_dt, inv_mu, bc, (float)penalty, reinterpret_cast<float4 *>(vgeo),
+                     pen);
+  CFSA_LAUNCH_CHECK();
+}
+
+// Launches the sliced-ELL kernel when A carries that layout, else CSR-stream
+// (or the element-patch kernel for an FP32 3-rhs operator that carries
+// patches); returns the grid size (the number of partial pairs per value).
 template <class T, int KO, int ND, class Op>
 static int launch_spmv(hipStream_t s, const DevCsr &A, const Op &op, double *partials) {
+  if constexpr (sizeof(T) == 4 && KO == 3) {
+    if (A.elem) {
+      const ElemPatchDev &E = *A.elem;
+      const int G = (E.npatch + kStreamGroup - 1) / kStreamGroup;
+      if (ND > 0 && (!A.wpart || (int64_t)KO * ND * E.npatch > A.wpart_cap))
+        throw std::logic_error("element-patch inner products need window scratch");
+      if (G > kMaxPartials)
+        throw std::runtime_error("element patches: too many for the partial buffers");
+      constexpr int round = 8 * kStreamChunk;
+      const int grid = (E.npatch + round - 1) / round * round;
+      hipLaunchKernelGGL((k_elem_spmv<ND, Op>), dim3(grid), dim3(kElemBlock), 0, s, E, op,
+                         A.wpart);
+      CFSA_LAUNCH_CHECK();
+      if constexpr (ND > 0) {
+        hipLaunchKernelGGL((k_window_reduce<KO * ND>), dim3(G), dim3(kBlock), 0, s, E.npatch,
+                           A.wpart, partials);
+        CFSA_LAUNCH_CHECK();
+      }
+      return G;
+    }
+  }
   if (A.sptr) {
     const int G = sell_grid(A.nslices);
     hipLaunchKernelGGL((k_sell<T, KO, ND, kSellUnroll, Op>), dim3(G), dim3(kBlock), 0, s, A.n,

@@ -10,6 +10,26 @@ namespace cfsa {
 // value arrays are in CSR slot order.  With sptr set it is the sliced-ELL copy
 // (col = per stored entry, sptr/rlen/perm as in SellPattern) and value arrays
 // are in stored-entry order; the SpMV launchers pick the kernel from that.
+// Element patches of the momentum operator on the device (elem_patch.hpp): an
+// FP32 3-rhs SpMV on a DevCsr that carries them is applied tet by tet.
+struct ElemPatchDev {
+  int npatch = 0;
+  const int *node_off = nullptr, *nodes = nullptr;
+  const int *row_off = nullptr, *rows = nullptr;
+  const unsigned short *starts = nullptr;
+  const int *visit_off = nullptr;
+  int64_t nvisit = 0, ntet = 0;
+  const unsigned int *vstat = nullptr;  // 2 planes of 4 words per visit
+  const float *vgeo = nullptr;          // 3 planes of 4 floats per tet
+  const float *pen = nullptr;           // diagonal penalty of each owned row
+};
+
+// per-tet FP32 geometry and owned-row penalties of the element patches from
+// the FP64 per-tet volumes and gradients (rho_dt = rho / dt, inv_mu = 1 / mu)
+void elem_refresh(hipStream_t s, int64_t nt, int64_t nrows, const int *torder, const int *rows,
+                  const double *vol, const double *grad, double rho_dt, double inv_mu,
+                  const uint8_t *bc, double penalty, float *vgeo, float *pen);
+
 struct DevCsr {
   int n = 0;
   int64_t nnz = 0;
@@ -26,6 +46,7 @@ struct DevCsr {
   // scratch of the CSR-stream inner products: pairs per (value, window)
   double *wpart = nullptr;
   int64_t wpart_cap = 0;  // values x windows the pair scratch holds
+  const ElemPatchDev *elem = nullptr;  // host struct; set: the element-patch SpMV
 };
 
 // Nonzeros staged per CSR-stream window (one block), measured best at 96^3

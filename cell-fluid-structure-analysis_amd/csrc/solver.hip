@@ -43,6 +43,39 @@ __global__ void k_accumulate(int64_t N, const float *__restrict__ d, double *__r
        i += (int64_t)gridDim.x * kBlock)
     x[i] = x[i] + (double)d[i];
 }
+// The element patches number the velocity nodes in patch order (node j of
+// that order is node perm[j]); the FP32 inner solves on them run in that order.
+// out[j][c] = masked ? float(in[perm[j]][c]) : 0
+__global__ void k_demote_perm(int n, const double *__restrict__ in, const int *__restrict__ mask,
+                              const int *__restrict__ perm, float *__restrict__ out) {
+  const int64_t N = (int64_t)n * 3;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * kBlock) {
+    const int c = (int)(i % 3);
+    out[i] = mask[c] ? (float)in[(int64_t)perm[i / 3] * 3 + c] : 0.0f;
+  }
+}
+// x[perm[j]][c] += d[j][c]
+__global__ void k_accumulate_perm(int n, const float *__restrict__ d, const int *__restrict__ perm,
+                                  double *__restrict__ x) {
+  const int64_t N = (int64_t)n * 3;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * kBlock) {
+    const int64_t o = (int64_t)perm[i / 3] * 3 + i % 3;
+    x[o] = x[o] + (double)d[i];
+  }
+}
+// out[j] = in[perm[j]] (K = 1), or its inverse with inverse = true
+template <class T>
+__global__ void k_permute(int n, const T *__restrict__ in, const int *__restrict__ perm,
+                          bool inverse, T *__restrict__ out) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    if (inverse)
+      out[perm[i]] = in[i];
+    else
+      out[i] = in[perm[i]];
+  }
+}
 constexpr double kMomentumPenalty = 1e10;  // NavierStokes.cpp:463
 constexpr double kPressurePenalty = 1e12;  // NavierStokes.cpp:611,619
 
@@ -85,6 +118,7 @@ void NSSolver::initialize(const MeshInput &m, double rho, double mu) {
   clear_graphs();
   amg_built_ = false;  // a hierarchy belongs to the previous mesh's operator
   amg_top_fresh_ = false;
+  elem_built_ = elem_fresh_ = false;  // so do the element patches
   amg_lbc_changed_ = true;
   amg_lv_.clear();
   amg_view_.clear();
@@ -410,6 +444,7 @@ void NSSolver::prepare_operators() {
       if (d_lhsS.n != (size_t)storedA) d_lhsS.alloc((size_t)storedA);
       sell_fill<double>(stream_, storedA, d_Asrc.p, d_lhs.p, d_lhsS.p);
     }
+    elem_fresh_ = false;  // the element patches' geometry and penalties follow the lhs
     if (mixed) {
       if (d_lhs32.n < (size_t)(la ? storedA : nnzA))
         throw std::logic_error("FP32 momentum operator smaller than its layout");
@@ -422,6 +457,15 @@ void NSSolver::prepare_operators() {
     sellA_ = la;
     lhs_dirty = false;
     lhs_dt = dt;
+  }
+  if (mixed && opt.elem_spmv && !elem_fresh_) {
+    ensure_elem();
+    elem_refresh(stream_, nt, elem_nrows_, d_etorder.p, d_erows.p, d_vol.p, d_grad.p,
+                 density / dt, 1.0 / viscosity, d_bc.p, kMomentumPenalty, d_evgeo.p, d_epen.p);
+    hipLaunchKernelGGL((k_permute<float>), dim3(1024), dim3(kBlock), 0, stream_, nu,
+                       d_invdA32.p, d_erows.p, false, d_einvd32.p);
+    CFSA_LAUNCH_CHECK();
+    elem_fresh_ = true;
   }
   if (lbc_dirty) {
     CFSA_HIP(hipMemcpyAsync(d_Lbc.p, d_L.p, (size_t)nnzP * sizeof(double),
@@ -601,7 +645,8 @@ SolveResult NSSolver::bicgstab(const DevCsr &A, const T *val, const T *invd, con
   };
   const std::vector<uintptr_t> key = {
       1, sizeof(T), K, (uintptr_t)n, (uintptr_t)chunk, (uintptr_t)A.rowptr,
-      (uintptr_t)A.col, (uintptr_t)A.sptr, (uintptr_t)val, (uintptr_t)invd, (uintptr_t)x,
+      (uintptr_t)A.col, (uintptr_t)A.sptr, (uintptr_t)A.elem, (uintptr_t)val, (uintptr_t)invd,
+      (uintptr_t)x,
       (uintptr_t)w.r.p, (uintptr_t)w.r0.p, (uintptr_t)w.p.p, (uintptr_t)w.v.p,
       (uintptr_t)w.y.p, (uintptr_t)w.s.p, (uintptr_t)w.z.p, (uintptr_t)w.t.p, (uintptr_t)sc,
       (uintptr_t)st, (uintptr_t)d_part_a.p, (uintptr_t)d_part_b.p, (uintptr_t)d_part_c.p,
@@ -673,7 +718,8 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
   };
   const std::vector<uintptr_t> key = {
       2, sizeof(T), K, (uintptr_t)n, (uintptr_t)chunk, (uintptr_t)A.rowptr,
-      (uintptr_t)A.col, (uintptr_t)A.sptr, (uintptr_t)val, (uintptr_t)invd, (uintptr_t)x,
+      (uintptr_t)A.col, (uintptr_t)A.sptr, (uintptr_t)A.elem, (uintptr_t)val, (uintptr_t)invd,
+      (uintptr_t)x,
       (uintptr_t)w.r.p, (uintptr_t)w.p.p, (uintptr_t)w.v.p,
       (uintptr_t)st, (uintptr_t)d_part_a.p, (uintptr_t)d_part_b.p,
       (uintptr_t)opt.stop_scaled, (uintptr_t)Gv};
@@ -773,6 +819,11 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double 
   const int n = A.n;
   KrylovWork<double> &w64 = (K == 3) ? w3_ : w1_;
   KrylovWork<float> &w32 = (K == 3) ? f3_ : f1_;
+  // the FP32 momentum operator: its CSR copy or the element patches (whose
+  // inner solves run in patch order, with the Jacobi scale in that order)
+  const DevCsr &A32 = (K == 3) ? momA32() : A;
+  const int *perm = A32.elem ? d_erows.p : nullptr;
+  if (perm) invd32 = d_einvd32.p;
   if (!keep_x) CFSA_HIP(hipMemsetAsync(x, 0, (size_t)n * K * sizeof(double), s));
   int Gr = 0;
   set_mask(1, K == 3, K == 3);
@@ -822,8 +873,12 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double 
       break;
     }
     set_mask(!conv[0], K == 3 && !conv[1], K == 3 && !conv[2]);
-    hipLaunchKernelGGL(k_demote_masked, dim3(2048), dim3(kBlock), 0, s, n, K, d_r64.p,
-                       d_mask.p, d_b32.p);
+    if (perm)
+      hipLaunchKernelGGL(k_demote_perm, dim3(2048), dim3(kBlock), 0, s, n, d_r64.p, d_mask.p,
+                         perm, d_b32.p);
+    else
+      hipLaunchKernelGGL(k_demote_masked, dim3(2048), dim3(kBlock), 0, s, n, K, d_r64.p,
+                         d_mask.p, d_b32.p);
     CFSA_LAUNCH_CHECK();
     // Inner tolerance: inner_tol, or looser when less than that is still needed
     // (0.7 x the reduction the least-converged active rhs needs: the FP32
@@ -835,12 +890,15 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double 
     const double tin = std::min(0.1, std::max(opt.inner_tol, 0.7 * need));
     SolveResult in;
     if (method == 1)
-      in = pcg<float, K>(A, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
+      in = pcg<float, K>(A32, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
     else
-      in = bicgstab<float, K>(A, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
+      in = bicgstab<float, K>(A32, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
     for (int c = 0; c < K; ++c) res.iters[c] += in.iters[c];
-    hipLaunchKernelGGL(k_accumulate, dim3(2048), dim3(kBlock), 0, s, (int64_t)n * K, d_x32.p,
-                       x);
+    if (perm)
+      hipLaunchKernelGGL(k_accumulate_perm, dim3(2048), dim3(kBlock), 0, s, n, d_x32.p, perm, x);
+    else
+      hipLaunchKernelGGL(k_accumulate, dim3(2048), dim3(kBlock), 0, s, (int64_t)n * K,
+                         d_x32.p, x);
     CFSA_LAUNCH_CHECK();
     set_mask(1, K == 3, K == 3);
     rr = residual();
@@ -1233,6 +1291,49 @@ void NSSolver::compute_divergence(const double *u, double *out) {
 // y = A x for matrix `which` (see get_matrix); nrhs 1 or 3 for P2 matrices,
 // and for G (5..7) x has nv entries and y is nv x 3 (all three components).
 void NSSolver::apply_matrix(int which, const double *x, double *y, int nrhs) {
+  if (which == 8) {
+    // the FP32 momentum operator of the mixed-precision inner solves (CSR copy
+    // or element patches), x and y converted on the device
+    if (nrhs != 3) throw std::invalid_argument("apply_matrix: the FP32 operator takes 3 rhs");
+    const int saved = opt.precision;
+    if (opt.precision == 0) opt.precision = 2;
+    try {
+      prepare_operators();
+    } catch (...) {
+      opt.precision = saved;
+      throw;
+    }
+    opt.precision = saved;
+    const size_t m = (size_t)nu * 3;
+    DBuf<double> dx, dy;
+    DBuf<float> fx, fy;
+    dx.upload(x, m);
+    dy.alloc(m);
+    fx.alloc(m);
+    fy.alloc(m);
+    const DevCsr &A32 = momA32();
+    if (A32.elem) {
+      // patch order in, node order out
+      DBuf<int> one;
+      const int ones[3] = {1, 1, 1};
+      one.upload(ones, 3);
+      hipLaunchKernelGGL(k_demote_perm, dim3(2048), dim3(kBlock), 0, stream_, nu, dx.p, one.p,
+                         d_erows.p, fx.p);
+      CFSA_LAUNCH_CHECK();
+      spmv3<float>(stream_, A32, d_lhs32.p, fx.p, fy.p, nullptr);
+      CFSA_HIP(hipMemsetAsync(dy.p, 0, m * sizeof(double), stream_));
+      hipLaunchKernelGGL(k_accumulate_perm, dim3(2048), dim3(kBlock), 0, stream_, nu, fy.p,
+                         d_erows.p, dy.p);
+      CFSA_LAUNCH_CHECK();
+    } else {
+      convert<float, double>(stream_, (int64_t)m, dx.p, fx.p);
+      spmv3<float>(stream_, A32, d_lhs32.p, fx.p, fy.p, nullptr);
+      convert<double, float>(stream_, (int64_t)m, fy.p, dy.p);
+    }
+    CFSA_HIP(hipStreamSynchronize(stream_));
+    dy.download(y, m);
+    return;
+  }
   prepare_operators();
   const bool p2 = which <= 2;
   const int n = p2 ? nu : nv;
@@ -1320,7 +1421,7 @@ double NSSolver::time_momentum_spmv(int reps, int precision) {
   CFSA_HIP(hipEventRecord(a, stream_));
   for (int i = 0; i < reps; ++i) {
     if (precision == 1)
-      spmv3<float>(stream_, momA(), d_lhs32.p, f3_.y.p, f3_.v.p, nullptr);
+      spmv3<float>(stream_, momA32(), d_lhs32.p, f3_.y.p, f3_.v.p, nullptr);
     else
       spmv3<double>(stream_, momA(), momV(), w3_.y.p, w3_.v.p, nullptr);
   }

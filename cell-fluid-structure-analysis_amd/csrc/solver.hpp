@@ -107,6 +107,9 @@ struct SolverOptions {
   int reuse_correction = 1;
   // multiplier of eps^2 in BiCGSTAB's restart test (1 = Eigen's rule)
   double restart_scale = 1.0;
+  // mixed precision: 1 applies the FP32 momentum operator tet by tet from the
+  // per-tet geometry (element patches) instead of its assembled CSR copy
+  int elem_spmv = 0;
   // pressure_solver 2: changes of the L_bc values for which the AMG hierarchy is
   // kept (the finest level always smooths with the current operator); 0
   // rebuilds it whenever the values change
@@ -300,6 +303,14 @@ class NSSolver {
   int sellA_ = -1, sellP_ = -1;  // layout the Krylov operator arrays hold (1 sliced-ELL)
   // the Krylov operators in their current layout (valid after prepare_operators)
   const DevCsr &momA() const { return sellA_ == 1 ? As_ : A_; }
+  // the FP32 momentum operator of the inner solves: momA(), or momA() with
+  // the element patches once they hold the current operator (elem_spmv)
+  const DevCsr &momA32() {
+    if (!(opt.elem_spmv && elem_built_ && elem_fresh_)) return momA();
+    elemA_ = momA();
+    elemA_.elem = &elem_dev_;
+    return elemA_;
+  }
   const double *momV() const { return sellA_ == 1 ? d_lhsS.p : d_lhs.p; }
   // builds the length-sorted sliced-ELL copy of the momentum pattern when first used
   void ensure_sell_A();
@@ -312,6 +323,17 @@ class NSSolver {
   DBuf<int> d_flag, d_mask;
   // Krylov
   KrylovWork<double> w3_, w1_;
+  // element patches of the momentum operator (elem_spmv; elem_solver.hip)
+  DBuf<int> d_enode_off, d_enodes, d_erow_off, d_erows, d_evisit_off, d_etorder;
+  DBuf<uint16_t> d_estarts;
+  DBuf<uint32_t> d_evstat;
+  DBuf<float> d_evgeo, d_epen, d_einvd32;  // d_einvd32: FP32 Jacobi scale in patch order
+  ElemPatchDev elem_dev_{};
+  DevCsr elemA_{};
+  bool elem_built_ = false, elem_fresh_ = false;
+  int64_t elem_nvisit_ = 0, elem_nrows_ = 0;
+  double elem_setup_ms_ = 0.0;
+  void ensure_elem();
   DBuf<double> d_bsb, d_bsx, d_bsinvd;  // solve_poisson_bicgstab operands (kept: graph keys)
   KrylovWork<float> f3_, f1_;
   DBuf<float> d_b32, d_x32;

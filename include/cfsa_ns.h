@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define CFSA_ABI_VERSION 3
+#define CFSA_ABI_VERSION 4
 
 enum cfsa_status {
   CFSA_OK = 0,
@@ -101,6 +101,10 @@ typedef struct cfsa_solver_options {
                             labels) for which the AMG hierarchy is kept, the finest level
                             always smoothing with the current operator; 0 (default) rebuilds
                             on every change.  Re-assembly to identical values never rebuilds. */
+  int elem_spmv;         /* precision 1 / 2: 1 applies the FP32 momentum operator of the inner
+                            solves tet by tet (patches of tets, from the per-tet volume and
+                            gradients, no assembled FP32 copy read); 0 (default) uses the
+                            assembled operator's FP32 copy.  The FP64 path is unaffected. */
 } cfsa_solver_options;
 
 typedef struct cfsa_step_stats {
@@ -289,7 +293,10 @@ int cfsa_ns_get_incidence(cfsa_ns *h, int32_t *ptr, int32_t *tet); /* nu+1, 10 n
 int cfsa_ns_compute_advection(cfsa_ns *h, double *out);
 /* computeDivergence(u, div) (NavierStokes.cpp:670-700); u is nu x 3 */
 int cfsa_ns_compute_divergence(cfsa_ns *h, const double *u, double *out);
-/* y = A x; nrhs 1 or 3 (interleaved); for which 5..7 y = [Gx x, Gy x, Gz x] */
+/* y = A x; nrhs 1 or 3 (interleaved); for which 5..7 y = [Gx x, Gy x, Gz x];
+ * which 8: the FP32 momentum operator of the mixed-precision inner solves
+ * (nrhs 3; the assembled operator's FP32 copy, or its element patches with
+ * elem_spmv = 1), x rounded to FP32 and y widened back */
 int cfsa_ns_apply_matrix(cfsa_ns *h, int which, const double *x, double *y, int nrhs);
 /* cfsa_ns_apply_matrix without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
@@ -339,6 +346,32 @@ int cfsa_amg_level_get(const cfsa_amg *h, int level, int which, int32_t *rowptr,
  * empty on the last level) */
 int cfsa_amg_level_params(const cfsa_amg *h, int level, double *rho, double *lmax,
                           double coef[3], int32_t *agg);
+
+/* ---- element patches of the momentum operator (elem_spmv), for inspection
+ * and tests: no counterpart in the reference, whose operator is only the
+ * assembled matrix (NavierStokes.cpp:460-495).  Rows (velocity nodes, the nv
+ * vertices then the ne edge nodes) are cut into patches in the Morton order of
+ * the node positions; a patch owns its rows and visits every tet touching one
+ * of them. ---- */
+typedef struct cfsa_elem_patches cfsa_elem_patches;
+/* tets nt x 4, tet edge nodes nt x 6 (0..ne-1), vertex coordinates nv x 3 */
+int cfsa_elem_patches_build(int64_t nv, int64_t nt, int64_t ne, const int32_t *tets,
+                            const int32_t *ten, const double *xyz, cfsa_elem_patches **out);
+int cfsa_elem_patches_free(cfsa_elem_patches *h);
+/* out: patches, visits, node-table entries, owned rows (= nv + ne), slot-start
+ * entries (owned rows + patches), largest owned rows / nodes / slots of a patch */
+int cfsa_elem_patches_sizes(const cfsa_elem_patches *h, int64_t out[8]);
+/* node_off / row_off / visit_off: patches + 1 offsets; nodes: each patch's node
+ * table (owned rows first); rows: owned rows; starts: per patch owned rows + 1
+ * slot starts; vtet: the tet of each visit; vstat: 8 words per visit, words
+ * 0-3 the 10 local node ids (12 bits each, id l at bit 12 l of the 128-bit
+ * little-endian value), words 4-6 the 10 slot ranks (8 bits each, 255 for a
+ * node the patch does not own), word 7 the tet in the patches' numbering;
+ * tet_order (nt): tet j of that numbering is mesh tet tet_order[j].  Any pointer
+ * may be NULL. */
+int cfsa_elem_patches_get(const cfsa_elem_patches *h, int32_t *node_off, int32_t *nodes,
+                          int32_t *row_off, int32_t *rows, uint16_t *starts, int32_t *visit_off,
+                          int32_t *vtet, uint32_t *vstat, int32_t *tet_order);
 /* dense (pseudo-)inverse of the coarsest operator, row-major coarse_n^2 */
 int cfsa_amg_coarse_inverse(const cfsa_amg *h, double *inv);
 

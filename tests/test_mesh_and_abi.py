@@ -149,7 +149,7 @@ def test_library_exports_every_declared_symbol(P):
     lib = P.library()
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.cfsa_abi_version() == 3
+    assert lib.cfsa_abi_version() == 4
 
 
 def test_abi_rejects_bad_arguments(P):
