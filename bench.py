@@ -30,11 +30,12 @@ HBM_PEAK_GBS = 8000.0
 # by the counters' readings for a plain 1 GiB copy (summary and commands in
 # this file)
 TRAFFIC_PROFILE = REPO / "profiles" / "k_csr_stream_f32_momentum_kuhn96_traffic.md"
+ELEM_TRAFFIC_PROFILE = REPO / "profiles" / "k_elem_spmv_momentum_kuhn96_traffic.md"
 
 
-def profiled_traffic():
+def profiled_traffic(path=TRAFFIC_PROFILE):
     try:
-        for line in TRAFFIC_PROFILE.read_text().splitlines():
+        for line in path.read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":")[1])
     except OSError:
@@ -51,6 +52,16 @@ def spmv_bytes(n, nnz, w, k=3):
     """Algorithmic bytes of one CSR SpMV with k right-hand sides sharing the
     matrix: values + int32 columns + int32 row pointers + read x, write y."""
     return (w + 4) * nnz + 4 * (n + 1) + 2 * k * w * n
+
+
+def elem_bytes(info, nu, nt):
+    """Algorithmic bytes of one element-patch SpMV (FP32, 3 right-hand sides):
+    the visit records (32 B each), the per-tet geometry (12 floats), the
+    patches' node tables (int32) and slot starts (uint16), the row penalties,
+    read x and write y once (12 B per node each); the fused inner product's
+    operand is not counted, as for the CSR kernel."""
+    return (32 * info["visits"] + 48 * nt + 4 * info["node_entries"]
+            + 2 * (nu + info["patches"]) + 4 * nu + 2 * 3 * 4 * nu)
 
 
 def make_solver(P, mesh, prof, args, precision, momentum_solver=0, initial_guess=0):
@@ -357,7 +368,7 @@ def main():
                     help="skip the whole-step oracle timing on the two cylinders (~1 min)")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--only", default=None, help="run only this mixed variant (profiling)")
-    ap.add_argument("--elem-spmv", type=int, default=0, choices=[0, 1],
+    ap.add_argument("--elem-spmv", type=int, default=1, choices=[0, 1],
                     help="mixed variants: FP32 momentum operator applied tet by tet (1) or "
                          "from its assembled CSR copy (0)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -467,6 +478,7 @@ def main():
             g = make_solver(P, mesh, prof, args, precision, msolver, guess)
             el, st = run(P, g, args.steps, args.warmup, torch)
             log(f"{name}: {args.steps} steps in {el:.3f}s; last {st[-1]}")
+            einfo = g.elemInfo()
             u, p = g.getVelocity(), g.getPressure()
             d = g.computeDivergence(u)
             err = {"velocity_rel_l2": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
@@ -484,7 +496,11 @@ def main():
                          "speedup_vs_fp64_same_solver": (
                              res[SAME_SOLVER[name]]["ms_per_step"] / (1e3 * el / args.steps)
                              if SAME_SOLVER[name] in res else None),
-                         "speedup_vs_fp64_bicgstab": el64 / el, "error_vs_fp64": err}
+                         "speedup_vs_fp64_bicgstab": el64 / el, "error_vs_fp64": err,
+                         "momentum_operator_fp32": "element patches" if einfo["built"]
+                         else "assembled CSR copy"}
+            if einfo["built"]:
+                res[name]["elem_patches"] = einfo
             ok = err["velocity_rel_l2"] <= VEL_ERR_MAX and name in HEADLINE
             if ok and (best is None or el < best[1]):
                 best = (name, el)
@@ -525,15 +541,30 @@ def main():
         roof = {"bound": "hbm", "kernel": "k_csr_stream<double,3> (momentum SpMV, 3 rhs, "
                 "fused inner products)", "bytes_per_launch": b64, "avg_launch_ms": ms}
         ach = b64 / ms / 1e6
+    elif "elem_patches" in res[primary]:
+        ms = res[primary]["spmv_avg_ms"]
+        be = elem_bytes(res[primary]["elem_patches"], sz["nu"], sz["nt"])
+        roof = {"bound": "hbm", "kernel": "k_elem_spmv<OpAxDot> (FP32 momentum operator tet by "
+                "tet from element patches, 3 rhs, fused inner products)",
+                "bytes_per_launch": be, "avg_launch_ms": ms}
+        ach = be / ms / 1e6
     else:
         ms = res[primary]["spmv_avg_ms"]
         roof = {"bound": "hbm", "kernel": "k_csr_stream<float,3> (momentum SpMV, 3 rhs, "
                 "fused inner products)", "bytes_per_launch": b32, "avg_launch_ms": ms}
         ach = b32 / ms / 1e6
-    traffic = profiled_traffic() if (primary != "fp64" and not cyl and n == 96) else None
+    tpath = ELEM_TRAFFIC_PROFILE if "k_elem_spmv" in roof["kernel"] else TRAFFIC_PROFILE
+    traffic = profiled_traffic(tpath) if (primary != "fp64" and not cyl and n == 96) else None
     roof.update({"achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                 "traffic_source": str(TRAFFIC_PROFILE.relative_to(REPO)) if traffic else None})
+                 "traffic_source": str(tpath.relative_to(REPO)) if traffic else None})
+    if "k_elem_spmv" in roof["kernel"]:
+        # the assembled operator's FP32 CSR-stream product on the same system,
+        # timed alone (20 launches), for comparison
+        t32 = lay["csr_stream"]["fp32_ms"]
+        res["roofline_csr_fp32"] = {"kernel": "k_csr_stream<float,3>", "bytes_per_launch": b32,
+                                    "avg_launch_ms_alone": t32,
+                                    "frac": b32 / t32 / 1e6 / HBM_PEAK_GBS}
 
     cpu = None
     if not args.skip_cpu:

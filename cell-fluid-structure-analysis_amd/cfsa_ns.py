@@ -199,6 +199,8 @@ _sig("cfsa_amg_coarse_inverse", C.c_int, _p, _f64p)
 _sig("cfsa_elem_patches_build", C.c_int, C.c_int64, C.c_int64, C.c_int64, _i32p, _i32p, _f64p,
      C.POINTER(_p))
 _sig("cfsa_elem_patches_free", C.c_int, _p)
+_sig("cfsa_ns_elem_info", C.c_int, _p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"),
+     C.POINTER(C.c_double))
 _sig("cfsa_elem_patches_sizes", C.c_int, _p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"))
 _sig("cfsa_elem_patches_get", C.c_int, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p)
 _sig("cfsa_ns_get_stream", C.c_int, _p, C.POINTER(_p))
@@ -700,6 +702,13 @@ class NavierStokesSolver:
         L = levels.value
         return {"levels": L, "rows": list(rows)[:L], "nnz": list(nnz)[:L],
                 "setup_ms": ms.value, "builds": builds.value}
+
+    def elemInfo(self):
+        out = np.zeros(4, np.int64)
+        ms = C.c_double()
+        _check(_lib.cfsa_ns_elem_info(self._h, out, C.byref(ms)))
+        return {"patches": int(out[0]), "visits": int(out[1]), "node_entries": int(out[2]),
+                "built": bool(out[3]), "setup_ms": ms.value}
 
     def timeMomentumSpmv(self, reps=20, precision=0):
         out = C.c_double()

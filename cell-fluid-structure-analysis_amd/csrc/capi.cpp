@@ -792,6 +792,14 @@ int cfsa_ns_amg_info(cfsa_ns *h, int cap, int64_t *rows, int64_t *nnz, double *s
     return CFSA_OK;
   });
 }
+int cfsa_ns_elem_info(cfsa_ns *h, int64_t out[4], double *setup_ms) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  return guard([&]() -> int {
+    h->s.elem_info(out, setup_ms);
+    return CFSA_OK;
+  });
+}
 int cfsa_amg_build(int64_t n, const int32_t *rowptr, const int32_t *col, const double *val,
                    double theta, int max_coarse, cfsa_amg **out) {
   CHECK_PTR(rowptr);

@@ -435,8 +435,26 @@ __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4)))
     for (int k = 0; k < PMAX; ++k) {
       const int i = tid / 3 + k * RP;
       if (i < nr) {
+        // the row's slots in ascending tet order, LDS reads batched by 8 and 4
         float acc = 0.0f;
-        for (int sl = st[i]; sl < st[i + 1]; ++sl) acc += cnt[sl * 3 + myc];
+        int sl = st[i];
+        const int se = st[i + 1];
+        for (; sl + 8 <= se; sl += 8) {
+          float c8[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) c8[u] = cnt[(sl + u) * 3 + myc];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc += c8[u];
+        }
+        if (sl + 4 <= se) {
+          float c4[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) c4[u] = cnt[(sl + u) * 3 + myc];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc += c4[u];
+          sl += 4;
+        }
+        for (; sl < se; ++sl) acc += cnt[sl * 3 + myc];
         // owned rows lead the node table; rows are numbered in patch order
         if (pen2[k] != 0.0f) acc += pen2[k] * xs[i * 3 + myc];
         op.epi(r0 + i, myc, acc, pre2[k], dots);

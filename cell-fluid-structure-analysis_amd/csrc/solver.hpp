@@ -245,6 +245,14 @@ class NSSolver {
   // levels of the AMG hierarchy (0 if none), rows and nonzeros per level (up to
   // cap entries), set-up time of the last build in ms, builds so far
   int amg_info(int cap, int64_t *rows, int64_t *nnz, double *setup_ms, int *builds);
+  // element patches: patches, visits, node-table entries, built; set-up time
+  void elem_info(int64_t out[4], double *setup_ms) const {
+    out[0] = elem_built_ ? elem_dev_.npatch : 0;
+    out[1] = elem_built_ ? elem_dev_.nvisit : 0;
+    out[2] = elem_built_ ? (int64_t)d_enodes.n : 0;
+    out[3] = elem_built_ ? 1 : 0;
+    if (setup_ms) *setup_ms = elem_setup_ms_;
+  }
   hipStream_t stream() const { return stream_; }
   // Momentum-matrix SpMV launches repeated `reps` times between two events.
   double time_momentum_spmv(int reps, int precision);

@@ -325,6 +325,11 @@ int cfsa_ns_amg_apply(cfsa_ns *h, const double *r, double *z, int precision);
  * output pointer may be NULL */
 int cfsa_ns_amg_info(cfsa_ns *h, int cap, int64_t *rows, int64_t *nnz, double *setup_ms,
                      int *builds, int *levels);
+/* the element patches of the FP32 momentum operator (elem_spmv = 1; no
+ * counterpart in the reference): out = patches, visits, node-table entries,
+ * 1 if built (all 0 before the first mixed-precision solve that uses them);
+ * setup_ms (may be NULL) the host set-up time */
+int cfsa_ns_elem_info(cfsa_ns *h, int64_t out[4], double *setup_ms);
 /* Host-side hierarchy of any square CSR matrix (sorted columns per row), for
  * inspection and tests: the set-up the solver runs (strength threshold theta,
  * at most max_coarse rows on the coarsest level) */
