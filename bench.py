@@ -56,11 +56,12 @@ def spmv_bytes(n, nnz, w, k=3):
 
 def elem_bytes(info, nu, nt):
     """Algorithmic bytes of one element-patch SpMV (FP32, 3 right-hand sides):
-    the visit records (32 B each), the per-tet geometry (12 floats), the
-    patches' node tables (int32) and slot starts (uint16), the row penalties,
-    read x and write y once (12 B per node each); the fused inner product's
-    operand is not counted, as for the CSR kernel."""
-    return (32 * info["visits"] + 48 * nt + 4 * info["node_entries"]
+    the visit records (32 B each), the per-tet geometry (8 floats: rho V / dt
+    and the 3 x 3 Gram matrix of the scaled gradients), the patches' node
+    tables (int32) and slot starts (uint16), the row penalties, read x and
+    write y once (12 B per node each); the fused inner product's operand is
+    not counted, as for the CSR kernel."""
+    return (32 * info["visits"] + 32 * nt + 4 * info["node_entries"]
             + 2 * (nu + info["patches"]) + 4 * nu + 2 * 3 * 4 * nu)
 
 

@@ -869,19 +869,30 @@ class HostElemPatches:
             _lib.cfsa_elem_patches_free(self._h)
             self._h = None
 
+    def _words(self, k):
+        w = self.vstat[:, 4 * k:4 * k + 4].astype(object)
+        return w[:, 0] | (w[:, 1] << 32) | (w[:, 2] << 64) | (w[:, 3] << 96)
+
     def local_ids(self):
-        """visits x 10 local node ids (12 bits each at bit 12 l of words 0-3)"""
-        w = self.vstat[:, :4].astype(object)
-        big = w[:, 0] | (w[:, 1] << 32) | (w[:, 2] << 64) | (w[:, 3] << 96)
+        """visits x 10 local node ids (11 bits each at bit 11 l of words 0-3)"""
+        big = self._words(0)
+        return np.array([[(int(v) >> (11 * l)) & 0x7FF for l in range(10)] for v in big],
+                        np.int64).reshape(-1, 10)
+
+    def slots(self):
+        """visits x 10 LDS slots of the owned nodes (12 bits each at bit 12 l of
+        words 4-7; 0 for a node the patch does not own)"""
+        big = self._words(1)
         return np.array([[(int(v) >> (12 * l)) & 0xFFF for l in range(10)] for v in big],
                         np.int64).reshape(-1, 10)
 
-    def ranks(self):
-        """visits x 10 slot ranks (255: the node is not owned by the patch)"""
-        w = self.vstat[:, 4:7]
-        b = np.stack([(w >> (8 * k)) & 0xFF for k in range(4)], 2).reshape(-1, 12)
-        return b[:, :10].astype(np.int64)
+    def tet_numbers(self):
+        """the tet of each visit in patch numbering (tet_order): bits 0-17 at bit
+        110 of words 0-3, bits 18-25 at bit 120 of words 4-7"""
+        a = self.vstat[:, 3].astype(np.int64) >> 14
+        b = self.vstat[:, 7].astype(np.int64) >> 24
+        return a | (b << 18)
 
     def tets(self):
-        """the mesh tet of each visit, from word 7 of its record (patch numbering)"""
-        return self.tet_order[self.vstat[:, 7].astype(np.int64)].astype(np.int64)
+        """the mesh tet of each visit, from its record"""
+        return self.tet_order[self.tet_numbers()].astype(np.int64)

@@ -8,7 +8,8 @@
 
 namespace cfsa {
 
-static_assert(kElemNodes <= 4096, "local node ids are 12 bits");
+static_assert(kElemNodes <= 2048, "local node ids are 11 bits");
+static_assert(kElemSlots <= 4096, "slots are 12 bits");
 
 namespace {
 
@@ -23,11 +24,33 @@ uint64_t spread3(uint64_t v) {
   return v;
 }
 
+// bits b .. b + n - 1 of the 128-bit little-endian word w[0..3] (n <= 32, a
+// field ending in word 3 when it starts there): or-in the low n bits of v,
+// read, clear
+void put_bits(uint32_t *w, int b, int n, uint32_t v) {
+  const int k = b >> 5, o = b & 31;
+  const uint64_t f = ((uint64_t)v & ((1ull << n) - 1)) << o;
+  w[k] |= (uint32_t)f;
+  if (k < 3) w[k + 1] |= (uint32_t)(f >> 32);
+}
+uint32_t get_bits(const uint32_t *w, int b, int n) {
+  const int k = b >> 5, o = b & 31;
+  const uint64_t v = (uint64_t)w[k] | (k < 3 ? (uint64_t)w[k + 1] << 32 : 0);
+  return (uint32_t)((v >> o) & ((1ull << n) - 1));
+}
+void clear_bits(uint32_t *w, int b, int n) {
+  const int k = b >> 5, o = b & 31;
+  const uint64_t f = ((1ull << n) - 1) << o;
+  w[k] &= ~(uint32_t)f;
+  if (k < 3) w[k + 1] &= ~(uint32_t)(f >> 32);
+}
+
 }  // namespace
 
 ElemPatchHost build_elem_patches(int64_t nu, int64_t nv, int64_t nt, const int32_t *tets,
                                  const int32_t *ten, const double *xyz) {
   if (nu <= 0 || nv <= 0 || nt <= 0 || nu < nv) throw std::invalid_argument("elem patches: sizes");
+  if (nt >= (int64_t)1 << 26) throw std::runtime_error("elem patches: more than 2^26 tets");
   auto node = [&](int64_t t, int l) -> int64_t {
     return l < 4 ? (int64_t)tets[t * 4 + l] : nv + (int64_t)ten[t * 6 + (l - 4)];
   };
@@ -109,9 +132,14 @@ ElemPatchHost build_elem_patches(int64_t nu, int64_t nv, int64_t nt, const int32
   int slots = 0, nnodes = 0;
   int64_t pid = 1;
   auto close_patch = [&]() {
-    // node table: owned rows, then the other nodes in patch order (so the
-    // gathers that stage them read runs of neighbouring patches' rows)
+    // node table: owned rows, then the other nodes (put in device order once
+    // every patch is built)
     const int nr = (int)prow.size();
+    // owned rows by descending slot count (a vertex lies in up to ~24 tets, an
+    // edge node in 4-6), so the threads of one wavefront sum rows of equal length
+    std::stable_sort(prow.begin(), prow.end(), [&](int32_t a, int32_t b) {
+      return inc_off[(size_t)a + 1] - inc_off[(size_t)a] > inc_off[(size_t)b + 1] - inc_off[(size_t)b];
+    });
     std::sort(pother.begin(), pother.end(),
               [&](int32_t a, int32_t b) { return mrank[(size_t)a] < mrank[(size_t)b]; });
     for (int i = 0; i < nr; ++i) loc[(size_t)prow[(size_t)i]] = i;
@@ -122,12 +150,12 @@ ElemPatchHost build_elem_patches(int64_t nu, int64_t nv, int64_t nt, const int32
     E.node_off.push_back((int32_t)E.nodes.size());
     for (int32_t j : prow) E.rows.push_back(j);
     E.row_off.push_back((int32_t)E.rows.size());
-    int s = 0;
-    for (int i = 0; i < nr; ++i) {
-      E.starts.push_back((uint16_t)s);
-      s += (int)(inc_off[(size_t)prow[(size_t)i] + 1] - inc_off[(size_t)prow[(size_t)i]]);
-    }
-    E.starts.push_back((uint16_t)s);
+    std::vector<int> pst((size_t)nr + 1, 0);
+    for (int i = 0; i < nr; ++i)
+      pst[(size_t)i + 1] =
+          pst[(size_t)i] + (int)(inc_off[(size_t)prow[(size_t)i] + 1] - inc_off[(size_t)prow[(size_t)i]]);
+    const int s = pst[(size_t)nr];
+    for (int i = 0; i <= nr; ++i) E.starts.push_back((uint16_t)pst[(size_t)i]);
     std::sort(ptets.begin(), ptets.end());
     for (int32_t t : ptets)
       if (tnew[(size_t)t] < 0) {
@@ -138,29 +166,19 @@ ElemPatchHost build_elem_patches(int64_t nu, int64_t nv, int64_t nt, const int32
               [&](int32_t a, int32_t b) { return tnew[(size_t)a] < tnew[(size_t)b]; });
     for (int32_t t : ptets) {
       E.vtet.push_back(t);
-      // words 0-3: 10 local ids of 12 bits (bit 12 l); words 4-6: 10 ranks of
-      // 8 bits (0xff: not owned); word 7: the tet
-      uint64_t lo = 0, hi = 0;
-      uint32_t w[kElemStat] = {0, 0, 0, 0, 0xffffffffu, 0xffffffffu, 0xffffffffu,
-                               (uint32_t)tnew[(size_t)t]};
+      // words 0-3: the 10 local ids (11 bits at bit 11 l), tet bits 0-17 at
+      // bit 110; words 4-7: the 10 LDS slots (12 bits at bit 12 l; 0 for a
+      // node the patch does not own), tet bits 18-25 at bit 120
+      uint32_t w[kElemStat] = {0, 0, 0, 0, 0, 0, 0, 0};
+      const uint32_t tp = (uint32_t)tnew[(size_t)t];
       for (int l = 0; l < 10; ++l) {
         const int64_t j = node(t, l);
-        const uint64_t lid = (uint64_t)loc[(size_t)j];
-        const int b = 12 * l;
-        if (b < 64) {
-          lo |= lid << b;
-          if (b + 12 > 64) hi |= lid >> (64 - b);
-        } else {
-          hi |= lid << (b - 64);
-        }
-        const uint32_t r = (int)lid < nr ? rank[(size_t)t * 10 + l] : 0xffu;
-        const int wi = 4 + l / 4, sh = 8 * (l % 4);
-        w[wi] = (w[wi] & ~(0xffu << sh)) | (r << sh);
+        const int lid = loc[(size_t)j];
+        put_bits(w, 11 * l, 11, (uint32_t)lid);
+        if (lid < nr) put_bits(w + 4, 12 * l, 12, (uint32_t)(pst[(size_t)lid] + rank[(size_t)t * 10 + l]));
       }
-      w[0] = (uint32_t)lo;
-      w[1] = (uint32_t)(lo >> 32);
-      w[2] = (uint32_t)hi;
-      w[3] = (uint32_t)(hi >> 32);
+      put_bits(w, 110, 18, tp & 0x3ffffu);
+      put_bits(w + 4, 120, 8, tp >> 18);
       E.vstat.insert(E.vstat.end(), w, w + kElemStat);
     }
     E.visit_off.push_back((int32_t)E.vtet.size());
@@ -233,6 +251,41 @@ ElemPatchHost build_elem_patches(int64_t nu, int64_t nv, int64_t nt, const int32
   if (!prow.empty()) close_patch();
   E.npatch = (int64_t)E.visit_off.size() - 1;
   E.nvisit = (int64_t)E.vtet.size();
+  // The other nodes of each table were ordered by Morton rank; now that every
+  // patch has ordered its owned rows, put them in the device order (position
+  // in rows), so the gathers that stage them read runs of neighbouring
+  // patches' rows, and renumber the local ids of the patch's visits to match.
+  {
+    std::vector<int32_t> pos((size_t)nu);
+    for (size_t j = 0; j < E.rows.size(); ++j) pos[(size_t)E.rows[j]] = (int32_t)j;
+    std::vector<int32_t> idx, remap;
+    for (int64_t p = 0; p < E.npatch; ++p) {
+      const int b = E.node_off[(size_t)p], e = E.node_off[(size_t)p + 1];
+      const int nr = E.row_off[(size_t)p + 1] - E.row_off[(size_t)p];
+      const int no = e - b - nr;
+      idx.resize((size_t)no);
+      std::iota(idx.begin(), idx.end(), 0);
+      std::sort(idx.begin(), idx.end(), [&](int32_t u, int32_t v) {
+        return pos[(size_t)E.nodes[(size_t)(b + nr + u)]] < pos[(size_t)E.nodes[(size_t)(b + nr + v)]];
+      });
+      remap.resize((size_t)(nr + no));
+      for (int i = 0; i < nr; ++i) remap[(size_t)i] = i;
+      std::vector<int32_t> moved((size_t)no);
+      for (int k = 0; k < no; ++k) {
+        moved[(size_t)k] = E.nodes[(size_t)(b + nr + idx[(size_t)k])];
+        remap[(size_t)(nr + idx[(size_t)k])] = nr + k;
+      }
+      std::copy(moved.begin(), moved.end(), E.nodes.begin() + b + nr);
+      for (int v = E.visit_off[(size_t)p]; v < E.visit_off[(size_t)p + 1]; ++v) {
+        uint32_t *w = E.vstat.data() + (size_t)v * kElemStat;
+        for (int l = 0; l < 10; ++l) {
+          const uint32_t lid = get_bits(w, 11 * l, 11);
+          clear_bits(w, 11 * l, 11);
+          put_bits(w, 11 * l, 11, (uint32_t)remap[lid]);
+        }
+      }
+    }
+  }
   return E;
 }
 

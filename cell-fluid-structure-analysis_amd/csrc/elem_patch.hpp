@@ -27,8 +27,8 @@ namespace cfsa {
 constexpr int kElemSlots = 4096;  // contribution slots of a patch (12 B of LDS each)
 constexpr int kElemNodes = 2048;  // nodes of a patch's tets (local ids)
 constexpr int kElemRows = 1024;   // rows a patch owns
-constexpr int kElemStat = 8;      // 32-bit words per visit: local ids, ranks, the tet
-constexpr int kElemGeo = 12;      // floats per tet: rho V / dt, sqrt(V / mu) grad phi_1..3
+constexpr int kElemStat = 8;      // 32-bit words per visit: local ids, slots, the tet
+constexpr int kElemGeo = 8;       // floats per tet: rho V / dt, (V / mu) grad phi_k . grad phi_l
 
 struct ElemPatchHost {
   int64_t npatch = 0, nvisit = 0;
@@ -40,15 +40,18 @@ struct ElemPatchHost {
   std::vector<int32_t> visit_off; // npatch + 1
   std::vector<int32_t> vtet;      // tet of each visit (mesh numbering)
   std::vector<int32_t> tet_order; // patch numbering of the tets: tet j is tet_order[j]
-  // kElemStat words per visit: words 0-3 the 10 local node ids (12 bits each,
-  // id l at bit 12 l of the 128-bit word), words 4-6 the 10 slot ranks (8 bits
-  // each, 0xff for a node the patch does not own), word 7 the tet in patch
-  // numbering (tet_order)
+  // kElemStat words per visit, two 128-bit little-endian words: A (words 0-3)
+  // holds the 10 local node ids (11 bits each, id l at bit 11 l) and bits 0-17
+  // of the tet in patch numbering (tet_order) at bit 110; B (words 4-7) holds
+  // the LDS slot of each node the patch owns (12 bits at bit 12 l: the owned
+  // row's first slot plus the tet's rank among the row's tets in ascending
+  // mesh order; 0 for a node the patch does not own) and bits 18-25 of the
+  // tet at bit 120
   std::vector<uint32_t> vstat;
   int max_rows = 0, max_nodes = 0, max_slots = 0;
 };
 
-// nu nodes (the nv vertices, then the edge nodes; at most 4096 per patch),
+// nu nodes (the nv vertices, then the edge nodes), fewer than 2^26 tets,
 // tets nt x 4, tet edge
 // nodes nt x 6 (edge ids 0..nu-nv-1, local edges 01 02 03 12 13 23), vertex
 // coordinates nv x 3.  Throws if a node lies in more than 254 tets or a single

@@ -270,6 +270,21 @@ template void sell_fill<float>(hipStream_t, int64_t, const int *, const double *
 // product and its fused inner products are deterministic.
 constexpr int kElemBlock = 512;
 
+// n < 32 bits at bit b of a 128-bit little-endian word (b, n compile-time
+// after unrolling)
+__device__ __forceinline__ unsigned int bits128(const uint4 &w, int b, int n) {
+  const unsigned int v[4] = {w.x, w.y, w.z, w.w};
+  const int k = b >> 5, o = b & 31;
+  unsigned int f = v[k] >> o;
+  if (o + n > 32) f |= v[k + 1] << (32 - o);
+  return f & ((1u << n) - 1u);
+}
+// the visit's tet (patch numbering): bits 0-17 at bit 110 of word A, bits
+// 18-25 at bit 120 of word B
+__device__ __forceinline__ size_t visit_tet(const uint4 &a, const uint4 &b) {
+  return (size_t)((a.w >> 14) | ((b.w >> 24) << 18));
+}
+
 template <int ND, class Op>
 __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_elem_spmv(ElemPatchDev E, Op op,
                                                           double *__restrict__ wpart) {
@@ -289,85 +304,82 @@ __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4)))
   const int r0 = E.row_off[p], nr = E.row_off[p + 1] - r0;
   const int v0 = E.visit_off[p], nvis = E.visit_off[p + 1] - v0;
   const float *__restrict__ x = op.x;
-  // Everything a thread reads from global memory is issued up front, so the
-  // block's latencies overlap: the operands of its rows' epilogues, its first
-  // visit record, then the staging gathers; each later visit record is loaded
-  // while the previous visit is computed.
+  // Everything a thread reads from global memory is issued up front, in one
+  // branch-free sequence (indices clamped into the patch, which holds at least
+  // one row, node and visit), so the block's latencies overlap and no load
+  // waits for another it does not depend on: its first visit record, the
+  // staging node ids, its slot starts, the epilogue operands of its rows, the
+  // first visit's geometry, then the staging gathers.  Each later visit record
+  // is loaded while the previous visit is computed.
   constexpr int RP = kElemBlock / 3;  // rows per pass of the row sums
   constexpr int PMAX = (kElemRows + RP - 1) / RP;
+  static_assert(kElemNodes % kElemBlock == 0, "whole staging passes");
+  constexpr int NPT = kElemNodes / kElemBlock;                    // staged nodes per thread
+  constexpr int SPT = (kElemRows + 1 + kElemBlock - 1) / kElemBlock;  // slot starts per thread
   const int myc = tid % 3;
-  const bool rowthr = tid < 3 * RP && op.active(myc);
+  // the component flags are read at uniform addresses (scalar loads)
+  const bool act0 = op.active(0), act1 = op.active(1), act2 = op.active(2);
+  const bool rowthr = tid < 3 * RP && (myc == 0 ? act0 : myc == 1 ? act1 : act2);
+  // visit records are stored as two planes (one 16-byte word per visit each),
+  // so every load of a wavefront reads one contiguous run; the geometry is
+  // per tet (two planes of one float4 per tet), gathered by the record's tet
+  const size_t nvt = (size_t)E.nvisit, ntt = (size_t)E.ntet;
+  const uint4 *vs = reinterpret_cast<const uint4 *>(E.vstat);
+  const float4 *vg = reinterpret_cast<const float4 *>(E.vgeo);
+  // the records of the thread's first two visits (a patch has at most 2 x
+  // kElemBlock visits on a Kuhn cube; later passes load their own)
+  uint4 sa0, sb0, sa1, sb1;
+  {
+    const size_t va = (size_t)(v0 + min(tid, nvis - 1));
+    const size_t vb = (size_t)(v0 + min(tid + kElemBlock, nvis - 1));
+    sa0 = vs[va];
+    sb0 = vs[nvt + va];
+    sa1 = vs[vb];
+    sb1 = vs[nvt + vb];
+  }
+  int jj[NPT];
+#pragma unroll
+  for (int u = 0; u < NPT; ++u) jj[u] = E.nodes[n0 + min(tid + u * kElemBlock, nn - 1)];
+  unsigned short stv[SPT];
+#pragma unroll
+  for (int u = 0; u < SPT; ++u) stv[u] = E.starts[r0 + p + min(tid + u * kElemBlock, nr)];
   typename Op::Pre pre2[PMAX];
   float pen2[PMAX];
 #pragma unroll
   for (int k = 0; k < PMAX; ++k) {
-    const int i = tid / 3 + k * RP;
-    pen2[k] = 0.0f;
-    if (rowthr && i < nr) {
-      pre2[k] = op.pre(r0 + i, myc);
-      pen2[k] = E.pen[r0 + i];
-    }
+    const int i = min(tid / 3 + k * RP, nr - 1);
+    pre2[k] = op.pre(r0 + i, myc);
+    pen2[k] = E.pen[r0 + i];
   }
-  // visit records are stored as two planes (one 16-byte word per visit each),
-  // so every load of a wavefront reads one contiguous run; the geometry is
-  // per tet (three planes of one float4 per tet), gathered by the record's tet
-  const size_t nvt = (size_t)E.nvisit, ntt = (size_t)E.ntet;
-  const uint4 *vs = reinterpret_cast<const uint4 *>(E.vstat);
-  const float4 *vg = reinterpret_cast<const float4 *>(E.vgeo);
-  uint4 sa{}, sb{};
-  float4 ga{}, gb{}, gc{};
-  if (tid < nvis) {
-    const size_t vi = (size_t)(v0 + tid);
-    sa = vs[vi];
-    sb = vs[nvt + vi];
-  }
-  // the patch's x values, one gather per node (a node serves ~14 visits); all
-  // of a thread's node ids, then all of its gathers, are issued before use
-  static_assert(kElemNodes % kElemBlock == 0, "whole staging passes");
-  constexpr int NPT = kElemNodes / kElemBlock;
+  const size_t ta = visit_tet(sa0, sb0), tb = visit_tet(sa1, sb1);
+  const float4 ga0 = vg[ta], gb0 = vg[ntt + ta], ga1 = vg[tb], gb1 = vg[ntt + tb];
+  // the patch's x values, one gather per node (a node serves ~14 visits)
   {
-    int jj[NPT];
-#pragma unroll
-    for (int u = 0; u < NPT; ++u) {
-      const int i = tid + u * kElemBlock;
-      jj[u] = i < nn ? E.nodes[n0 + i] : -1;
-    }
     float xv[NPT][3];
 #pragma unroll
     for (int u = 0; u < NPT; ++u)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xv[u][c] = jj[u] >= 0 ? x[(size_t)jj[u] * 3 + c] : 0.0f;
+      for (int c = 0; c < 3; ++c) xv[u][c] = x[(size_t)jj[u] * 3 + c];
 #pragma unroll
     for (int u = 0; u < NPT; ++u)
 #pragma unroll
       for (int c = 0; c < 3; ++c) xs[(tid + u * kElemBlock) * 3 + c] = xv[u][c];
   }
-  if (tid < nvis) {
-    const size_t t = sb.w;
-    ga = vg[t];
-    gb = vg[ntt + t];
-    gc = vg[2 * ntt + t];
-  }
-  for (int i = tid; i <= nr; i += kElemBlock) st[i] = E.starts[r0 + p + i];
+#pragma unroll
+  for (int u = 0; u < SPT; ++u)
+    if (tid + u * kElemBlock <= nr) st[tid + u * kElemBlock] = stv[u];
   __syncthreads();
-  for (int v = tid; v < nvis; v += kElemBlock) {
-    const uint4 ca = sa, cb = sb;
-    const float4 gA = ga, gB = gb, gC = gc;
-    const bool more = v + kElemBlock < nvis;
-    if (more) {
-      const size_t vi = (size_t)(v0 + v + kElemBlock);
-      sa = vs[vi];
-      sb = vs[nvt + vi];
-    }
-    // 12-bit local ids at bit 12 l of words 0-3; 8-bit ranks in words 4-6
-    const uint64_t lo = (uint64_t)ca.y << 32 | ca.x, hi = (uint64_t)ca.w << 32 | ca.z;
-    const unsigned int w[3] = {cb.x, cb.y, cb.z};
-    int lid[10];
+  // One visit (a thread past the last visit computes a clamped copy and
+  // writes nothing, so every thread runs the same passes).
+  auto visit = [&](const uint4 &ca, const uint4 &cb, const float4 &gA, const float4 &gB,
+                   bool valid) {
+    // 11-bit local ids at bit 11 l of record word A, 12-bit LDS slots at bit
+    // 12 l of record word B (elem_patch.hpp)
+    int lid[10], slot[10];
 #pragma unroll
     for (int l = 0; l < 10; ++l) {
-      const int b = 12 * l;
-      const uint64_t f = b < 64 ? ((lo >> b) | (b + 12 > 64 ? hi << (64 - b) : 0)) : hi >> (b - 64);
-      lid[l] = (int)(f & 0xfffu);
+      lid[l] = (int)bits128(ca, 11 * l, 11);
+      slot[l] = (int)bits128(cb, 12 * l, 12);
     }
     float xe[10][3];
 #pragma unroll
@@ -375,11 +387,15 @@ __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
       for (int c = 0; c < 3; ++c) xe[l][c] = xs[lid[l] * 3 + c];
     }
+    // rho V / dt and the Gram matrix G_kl = (V / mu) grad phi_k . grad phi_l (k, l = 1..3)
     const float m = gA.x;
-    const float a1[3] = {gA.y, gA.z, gA.w}, a2[3] = {gB.x, gB.y, gB.z}, a3[3] = {gB.w, gC.x, gC.y};
+    const float g11 = gA.y, g12 = gA.z, g13 = gA.w, g22 = gB.x, g23 = gB.y, g33 = gB.z;
     float ye[10][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
+      // FP32 operator of the inner solves: products and sums may contract to
+      // FMAs here (the library builds with contraction off for the FP64 path)
+#pragma clang fp contract(fast)
       const float x0 = xe[0][c], x1 = xe[1][c], x2 = xe[2][c], x3 = xe[3][c];
       const float e0 = xe[4][c], e1 = xe[5][c], e2 = xe[6][c], e3 = xe[7][c], e4 = xe[8][c],
                   e5 = xe[9][c];
@@ -388,12 +404,9 @@ __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4)))
       const float z0 = x0 + e0 + e1 + e2, z1 = x1 + e0 + e3 + e4, z2 = x2 + e1 + e3 + e5,
                   z3 = x3 + e2 + e4 + e5;
       const float d1 = z1 - z0, d2 = z2 - z0, d3 = z3 - z0;
-      float wv[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) wv[k] = d1 * a1[k] + d2 * a2[k] + d3 * a3[k];
-      const float s1 = a1[0] * wv[0] + a1[1] * wv[1] + a1[2] * wv[2];
-      const float s2 = a2[0] * wv[0] + a2[1] * wv[1] + a2[2] * wv[2];
-      const float s3 = a3[0] * wv[0] + a3[1] * wv[1] + a3[2] * wv[2];
+      const float s1 = g11 * d1 + g12 * d2 + g13 * d3;
+      const float s2 = g12 * d1 + g22 * d2 + g23 * d3;
+      const float s3 = g13 * d1 + g23 * d2 + g33 * d3;
       const float s0 = -((s1 + s2) + s3);
       // vertex rows: m (x_a / 20 + Sv / 20 + Se / 30); edge rows: m (x_k / 20 + Se / 60 + Sv / 30)
       const float mv = m * (sv * (1.0f / 20.0f) + se * (1.0f / 30.0f));
@@ -412,53 +425,64 @@ __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4)))
     }
 #pragma unroll
     for (int l = 0; l < 10; ++l) {
-      if (lid[l] < nr) {
-        const int rk = (int)((w[l / 4] >> (8 * (l % 4))) & 0xffu);
-        const int slot = (int)st[lid[l]] + rk;
+      if (valid && lid[l] < nr) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) cnt[slot * 3 + c] = ye[l][c];
+        for (int c = 0; c < 3; ++c) cnt[slot[l] * 3 + c] = ye[l][c];
       }
     }
-    if (more) {
-      const size_t t = sb.w;
-      ga = vg[t];
-      gb = vg[ntt + t];
-      gc = vg[2 * ntt + t];
-    }
+  };
+  const int npass = (nvis + kElemBlock - 1) / kElemBlock;  // block-uniform
+  visit(sa0, sb0, ga0, gb0, tid < nvis);
+  if (npass > 1) visit(sa1, sb1, ga1, gb1, tid + kElemBlock < nvis);
+  for (int pass = 2; pass < npass; ++pass) {
+    const int v = tid + pass * kElemBlock;
+    const size_t vi = (size_t)(v0 + min(v, nvis - 1));
+    const uint4 ca = vs[vi], cb = vs[nvt + vi];
+    const size_t t = visit_tet(ca, cb);
+    visit(ca, cb, vg[t], vg[ntt + t], v < nvis);
   }
   __syncthreads();
   DD dots[NDs];
 #pragma unroll
   for (int d = 0; d < NDs; ++d) dd_zero(dots[d]);
+  // all of the thread's row sums first, then their epilogues, so no LDS read
+  // of a later row waits behind an earlier row's global stores
+  float accs[PMAX];
+#pragma unroll
+  for (int k = 0; k < PMAX; ++k) {
+    const int i = tid / 3 + k * RP;
+    float acc = 0.0f;
+    if (rowthr && i < nr) {
+      // the row's slots in ascending tet order, LDS reads batched by 8 and 4
+      int sl = st[i];
+      const int se = st[i + 1];
+      const float px = xs[i * 3 + myc];  // owned rows lead the node table
+      for (; sl + 8 <= se; sl += 8) {
+        float c8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c8[u] = cnt[(sl + u) * 3 + myc];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += c8[u];
+      }
+      if (sl + 4 <= se) {
+        float c4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c4[u] = cnt[(sl + u) * 3 + myc];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += c4[u];
+        sl += 4;
+      }
+      for (; sl < se; ++sl) acc += cnt[sl * 3 + myc];
+      if (pen2[k] != 0.0f) acc += pen2[k] * px;
+    }
+    accs[k] = acc;
+  }
   if (rowthr) {
 #pragma unroll
     for (int k = 0; k < PMAX; ++k) {
       const int i = tid / 3 + k * RP;
-      if (i < nr) {
-        // the row's slots in ascending tet order, LDS reads batched by 8 and 4
-        float acc = 0.0f;
-        int sl = st[i];
-        const int se = st[i + 1];
-        for (; sl + 8 <= se; sl += 8) {
-          float c8[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) c8[u] = cnt[(sl + u) * 3 + myc];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) acc += c8[u];
-        }
-        if (sl + 4 <= se) {
-          float c4[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) c4[u] = cnt[(sl + u) * 3 + myc];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) acc += c4[u];
-          sl += 4;
-        }
-        for (; sl < se; ++sl) acc += cnt[sl * 3 + myc];
-        // owned rows lead the node table; rows are numbered in patch order
-        if (pen2[k] != 0.0f) acc += pen2[k] * xs[i * 3 + myc];
-        op.epi(r0 + i, myc, acc, pre2[k], dots);
-      }
+      // rows are numbered in patch order
+      if (i < nr) op.epi(r0 + i, myc, accs[k], pre2[k], dots);
     }
   }
   if constexpr (ND > 0) {
@@ -494,8 +518,9 @@ __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4)))
   }
 }
 
-// Per-tet FP32 geometry (rho V / dt, sqrt(V / mu) grad phi_1..3 from the FP64
-// per-tet arrays; three planes of one float4 per tet) and each owned row's
+// Per-tet FP32 geometry (rho V / dt and the Gram matrix (V / mu) grad phi_k .
+// grad phi_l, k, l = 1..3, rounded from FP64 products of the FP64
+// per-tet arrays; two planes of one float4 per tet) and each owned row's
 // diagonal penalty (Inlet / Wall).
 __global__ void k_elem_refresh(int64_t nt, int64_t nrows, const int *__restrict__ torder,
                                const int *__restrict__ rows, const double *__restrict__ vol,
@@ -506,13 +531,14 @@ __global__ void k_elem_refresh(int64_t nt, int64_t nrows, const int *__restrict_
   for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < nt; j += stride) {
     const int64_t t = torder[j];  // tet j of the patch numbering
     const double V = vol[t];
-    const double sc = sqrt(V * inv_mu);
+    const double sc = V * inv_mu;
     const double *g = grad + (size_t)t * 12;
-    vgeo[j] = make_float4((float)(rho_dt * V), (float)(sc * g[3]), (float)(sc * g[4]),
-                          (float)(sc * g[5]));
-    vgeo[nt + j] = make_float4((float)(sc * g[6]), (float)(sc * g[7]), (float)(sc * g[8]),
-                               (float)(sc * g[9]));
-    vgeo[2 * nt + j] = make_float4((float)(sc * g[10]), (float)(sc * g[11]), 0.0f, 0.0f);
+    auto G = [&](int k, int l) {
+      const double *a = g + 3 * k, *b = g + 3 * l;
+      return (float)(sc * (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]));
+    };
+    vgeo[j] = make_float4((float)(rho_dt * V), G(1, 1), G(1, 2), G(1, 3));
+    vgeo[nt + j] = make_float4(G(2, 2), G(2, 3), G(3, 3), 0.0f);
   }
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nrows; i += stride) {
     const uint8_t b = bc[rows[i]];

@@ -20,7 +20,7 @@ struct ElemPatchDev {
   const int *visit_off = nullptr;
   int64_t nvisit = 0, ntet = 0;
   const unsigned int *vstat = nullptr;  // 2 planes of 4 words per visit
-  const float *vgeo = nullptr;          // 3 planes of 4 floats per tet
+  const float *vgeo = nullptr;          // 2 planes of 4 floats per tet
   const float *pen = nullptr;           // diagonal penalty of each owned row
 };
 

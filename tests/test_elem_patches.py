@@ -3,9 +3,9 @@
 * structure: every velocity node is owned by exactly one patch; a patch's node
   table starts with its owned rows; every tet touching an owned row is a visit
   of that patch, and only those; the local ids of a visit name the tet's 10
-  nodes; the slots (start of the owned row + rank) of a patch are a
-  permutation of 0 .. slots-1 and run in ascending tet order per row; the LDS
-  limits of the kernel hold;
+  nodes; the LDS slots of a patch's owned (row, tet) pairs are a permutation
+  of 0 .. slots-1, lie in the row's range of the slot starts and run in
+  ascending tet order per row; the LDS limits of the kernel hold;
 * the two phases of the device kernel restated in FP64 NumPy (per-visit
   element products written to slots, then per-row sums of the slots in order,
   plus the Inlet / Wall penalty) reproduce the oracle's assembled momentum
@@ -46,7 +46,7 @@ def test_patch_structure(case):
     assert np.array_equal(np.sort(E.rows), np.arange(nu))
     assert E.max_rows <= ROWS and E.max_nodes <= NODES and E.max_slots <= SLOTS
     inc = np.bincount(nodes10.ravel(), minlength=nu)
-    lid, rk = E.local_ids(), E.ranks()
+    lid, sl = E.local_ids(), E.slots()
     pos = np.empty(nu, np.int64)
     pos[E.rows] = np.arange(nu)
     assert np.array_equal(np.sort(E.tet_order), np.arange(len(tets)))
@@ -62,16 +62,17 @@ def test_patch_structure(case):
         # numbering of the tets; the non-owned nodes follow the patch order
         want = np.nonzero(np.isin(nodes10, owned).any(1))[0]
         assert np.array_equal(np.sort(vt), want)
-        assert np.all(np.diff(E.vstat[vs, 7].astype(np.int64)) > 0)
+        assert np.all(np.diff(E.tet_numbers()[vs]) > 0)
         assert np.all(np.diff(pos[table[nr:]]) > 0)
         assert np.array_equal(table[lid[vs]], nodes10[vt])
         st = E.starts[E.row_off[p] + p:E.row_off[p + 1] + p + 1].astype(np.int64)
         assert st[0] == 0 and np.array_equal(np.diff(st), inc[owned])
         own = lid[vs] < nr
-        assert np.array_equal(own, rk[vs] != 255)
-        slot = st[np.where(own, lid[vs], 0)] + rk[vs]
-        s = slot[own]
+        assert np.all(sl[vs][~own] == 0)
+        s = sl[vs][own]
         assert np.array_equal(np.sort(s), np.arange(st[-1]))
+        r = lid[vs][own]
+        assert np.all((s >= st[r]) & (s < st[r + 1]))
         # per owned row, the slots follow ascending tet order
         row_of = lid[vs][own]
         tet_of = np.broadcast_to(vt[:, None], own.shape)[own]
@@ -95,7 +96,7 @@ def test_two_phase_product_matches_assembled_operator(case, O):
     V, G = om.volumes(), om.gradients().reshape(-1, 4, 3)
     bc = np.concatenate([om.p1BC(), om.p2BC()])
     x = np.random.default_rng(3).standard_normal((nu, 3))
-    lid, rk = E.local_ids(), E.ranks()
+    lid, sl = E.local_ids(), E.slots()
     y = np.zeros((nu, 3))
     for p in range(E.npatch):
         table = E.nodes[E.node_off[p]:E.node_off[p + 1]]
@@ -120,8 +121,7 @@ def test_two_phase_product_matches_assembled_operator(case, O):
         st = E.starts[E.row_off[p] + p:E.row_off[p + 1] + p + 1].astype(np.int64)
         cnt = np.zeros((st[-1], 3))
         own = lid[vs] < nr
-        slot = st[np.where(own, lid[vs], 0)] + rk[vs]
-        cnt[slot[own]] = ye[own]
+        cnt[sl[vs][own]] = ye[own]
         for i in range(nr):
             y[owned[i]] = cnt[st[i]:st[i + 1]].sum(0)
     pen = (bc == 0) | (bc == 1)
