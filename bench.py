@@ -566,6 +566,13 @@ def main():
         res["roofline_csr_fp32"] = {"kernel": "k_csr_stream<float,3>", "bytes_per_launch": b32,
                                     "avg_launch_ms_alone": t32,
                                     "frac": b32 / t32 / 1e6 / HBM_PEAK_GBS}
+        # the same product as the CSR kernel delivers it, per launch of the
+        # element kernel: the CSR form's algorithmic bytes over the element
+        # kernel's time (a rate no CSR kernel can exceed without reaching
+        # this fraction of the HBM peak); not the element kernel's roofline
+        res["momentum_product_csr_equivalent"] = {
+            "csr_bytes_per_launch": b32, "elem_ms": ms,
+            "GB/s": b32 / ms / 1e6, "frac_of_peak": b32 / ms / 1e6 / HBM_PEAK_GBS}
 
     cpu = None
     if not args.skip_cpu:
