@@ -786,7 +786,7 @@ class HostAmg:
     (cfsa_amg_build): levels, operators, prolongations and coefficients, for
     inspection and tests."""
 
-    def __init__(self, rowptr, col, val, theta=0.08, max_coarse=1000):
+    def __init__(self, rowptr, col, val, theta=0.06, max_coarse=1000):
         rowptr = np.ascontiguousarray(rowptr, np.int32)
         col = np.ascontiguousarray(col, np.int32)
         val = np.ascontiguousarray(val, np.float64)

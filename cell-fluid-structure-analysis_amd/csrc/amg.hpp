@@ -15,8 +15,9 @@
 //    strong connections (the 1e12 penalty rows) stay unaggregated: the coarse
 //    levels do not see them and the smoother handles them;
 //  * tentative prolongator T (constant per aggregate, columns normalised),
-//    smoothed as P = (I - omega D^-1 A) T with omega = 4 / (3 rho), rho the
-//    power-iteration estimate of the spectral radius of D^-1 A;
+//    smoothed as P = (I - omega D^-1 A) T with omega = 1.2 / lmax, lmax the
+//    smoothers' upper bound of the spectral radius of D^-1 A (below; the
+//    textbook 4 / (3 rho) coarsens the next level too aggressively here);
 //  * Galerkin coarse operator A_c = P^T A P, restriction R = P^T;
 //  * until at most max_coarse rows remain (or aggregation stalls); the
 //    coarsest operator is inverted densely (Cholesky; pivots that vanish are
@@ -50,7 +51,7 @@ struct HostCsr {
 };
 
 struct AmgParams {
-  double theta = 0.08;   // strength threshold
+  double theta = 0.06;   // strength threshold
   int max_coarse = 1000;  // rows of the coarsest (dense) level at most
   int max_levels = 12;
 };

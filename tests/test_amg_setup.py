@@ -4,10 +4,11 @@ checked against what the mathematics fixes:
 
 * strength and aggregation follow the three greedy passes (an independent
   restatement here); penalty rows (no strong connection) stay unaggregated;
-* P = (I - 4/(3 rho) D^-1 A) T with T the normalised aggregate indicator,
+* P = (I - 1.2/lmax D^-1 A) T with T the normalised aggregate indicator and
   R = P^T and A_c = R A P (scipy products);
-* the spectral radius estimate is a lower bound within 10% of the largest
-  eigenvalue of D^-1 A, and the Chebyshev interval's top lmax covers it;
+* the spectral radius estimate is within 10% below (and at most 0.1% above)
+  the largest eigenvalue of D^-1 A, and the Chebyshev interval's top lmax
+  covers it (so the prolongator weight 1.2 / lmax keeps omega rho <= 1.2);
 * the smoothing coefficients give the error polynomial
   T_2((theta - l)/delta) / T_2(theta/delta) of the degree-2 Chebyshev
   smoother (theta, delta: centre and half width of [lmax/30, lmax]);
@@ -22,7 +23,7 @@ import scipy.sparse.linalg as sla
 
 from amg_ref import VCycle, csr, pcg
 
-THETA = 0.08
+THETA = 0.06
 
 
 def _lbc(O, kind):
@@ -106,7 +107,7 @@ def test_levels_and_galerkin_products(case):
         rows = np.nonzero(agg >= 0)[0]
         T = sp.csr_matrix((1.0 / np.sqrt(cnt[agg[rows]]), (rows, agg[rows])), shape=(n, na))
         Dinv = sp.diags(1.0 / A.diagonal())
-        Pw = T - (4.0 / (3.0 * prm["rho"])) * (Dinv @ (A @ T))
+        Pw = T - (1.2 / prm["lmax"]) * (Dinv @ (A @ T))
         Pl = csr(*H.matrix(l, "P")[:3], H.matrix(l, "P")[3])
         Rl = csr(*H.matrix(l, "R")[:3], H.matrix(l, "R")[3])
         assert Pl.shape == (n, na)
@@ -150,7 +151,9 @@ def test_spectral_bounds_and_chebyshev_polynomial(case):
             lam = sla.eigsh(S, k=1, which="LA", return_eigenvectors=False, tol=1e-10)[0]
         else:
             lam = np.linalg.eigvalsh(S.toarray())[-1]
-        assert prm["rho"] <= lam * (1 + 1e-9) and prm["rho"] >= 0.9 * lam, (l, prm["rho"], lam)
+        # the power iteration measures |D^-1 A x| / |x| of a non-symmetric
+        # matrix: close to the spectral radius from below, possibly a hair above
+        assert prm["rho"] <= lam * (1 + 1e-3) and prm["rho"] >= 0.9 * lam, (l, prm["rho"], lam)
         assert prm["lmax"] >= lam * (1 - 1e-12), (l, prm["lmax"], lam)
         c0, c1, c2 = prm["coef"]
         lmax = prm["lmax"]
