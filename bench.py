@@ -205,18 +205,23 @@ def _cpu_baseline(O, g, fp64_stats, sz, pressure_method, pressure_iters=None):
 PRESSURE_SOLVERS = {0: "bicgstab_jacobi", 1: "cg_jacobi", 2: "cg_amg"}
 
 # FP64 variants: (name, momentum solver); the reference's BiCGSTAB first
-FP64 = [("fp64", 0), ("fp64_cg", 1)]
+FP64 = [("fp64", 0), ("fp64_cg", 1), ("fp64_cgpoly", 2)]
+MOMENTUM_SOLVERS = {0: "bicgstab_jacobi", 1: "cg_jacobi", 2: "cg_chebyshev_poly"}
 # mixed-precision variants: (name, precision, momentum solver, initial guess)
 #   precision 1: FP32 inner work for momentum and pressure (with the AMG pressure
 #   solve: an FP32 V-cycle inside the FP64 CG), 2: momentum only
-#   momentum solver 0: BiCGSTAB (the reference's), 1: Jacobi CG (lhs is SPD)
+#   momentum solver 0: BiCGSTAB (the reference's), 1: Jacobi CG (lhs is SPD),
+#   2: CG with the degree-1 Chebyshev polynomial preconditioner in D^-1 A
 MIXED = [("mixed_bicgstab_all", 1, 0, 0), ("mixed_bicgstab", 2, 0, 0),
-         ("mixed_cg_all", 1, 1, 0), ("mixed_cg", 2, 1, 0), ("mixed_cg_warm", 2, 1, 1)]
+         ("mixed_cg_all", 1, 1, 0), ("mixed_cg", 2, 1, 0), ("mixed_cg_warm", 2, 1, 1),
+         ("mixed_cgpoly_all", 1, 2, 0), ("mixed_cgpoly", 2, 2, 0)]
 # the FP64 variant with the same momentum solver as each mixed variant
 SAME_SOLVER = {"mixed_bicgstab_all": "fp64", "mixed_bicgstab": "fp64", "mixed_cg_all": "fp64_cg",
-               "mixed_cg": "fp64_cg", "mixed_cg_warm": "fp64_cg"}
+               "mixed_cg": "fp64_cg", "mixed_cg_warm": "fp64_cg",
+               "mixed_cgpoly_all": "fp64_cgpoly", "mixed_cgpoly": "fp64_cgpoly"}
 # variants that keep the reference's zero initial guess compete for the headline
-HEADLINE = {"mixed_bicgstab_all", "mixed_bicgstab", "mixed_cg_all", "mixed_cg"}
+HEADLINE = {"mixed_bicgstab_all", "mixed_bicgstab", "mixed_cg_all", "mixed_cg",
+            "mixed_cgpoly_all", "mixed_cgpoly"}
 VEL_ERR_MAX = 1e-7
 
 
@@ -440,7 +445,7 @@ def main():
         el, st = run(P, g, args.steps, args.warmup, torch)
         log(f"{name}: {args.steps} steps in {el:.3f}s; last {st[-1]}")
         res[name] = {"steps_per_s": args.steps / el, "ms_per_step": 1e3 * el / args.steps,
-                     "momentum_solver": "cg_jacobi" if msolver else "bicgstab_jacobi",
+                     "momentum_solver": MOMENTUM_SOLVERS[msolver],
                      "momentum_iters": st[-1]["momentum_iters"],
                      "pressure_iters": st[-1]["pressure_iters"], "spmv_avg_ms": spmv_avg(st)}
         if name == "fp64":
@@ -487,7 +492,7 @@ def main():
                    "divergence_rel_l2": float(np.linalg.norm(d - d64) / np.linalg.norm(d64))}
             res[name] = {"steps_per_s": args.steps / el, "ms_per_step": 1e3 * el / args.steps,
                          "precision": precision,
-                         "momentum_solver": "cg_jacobi" if msolver else "bicgstab_jacobi",
+                         "momentum_solver": MOMENTUM_SOLVERS[msolver],
                          "initial_guess": "u^n" if guess else "zero",
                          "momentum_inner_iters": st[-1]["momentum_iters"],
                          "momentum_refinements": st[-1]["momentum_refinements"],

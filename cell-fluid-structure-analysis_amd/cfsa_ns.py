@@ -199,6 +199,8 @@ _sig("cfsa_amg_coarse_inverse", C.c_int, _p, _f64p)
 _sig("cfsa_elem_patches_build", C.c_int, C.c_int64, C.c_int64, C.c_int64, _i32p, _i32p, _f64p,
      C.POINTER(_p))
 _sig("cfsa_elem_patches_free", C.c_int, _p)
+_sig("cfsa_ns_momentum_poly", C.c_int, _p,
+     np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS"))
 _sig("cfsa_ns_elem_info", C.c_int, _p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"),
      C.POINTER(C.c_double))
 _sig("cfsa_elem_patches_sizes", C.c_int, _p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"))
@@ -702,6 +704,14 @@ class NavierStokesSolver:
         L = levels.value
         return {"levels": L, "rows": list(rows)[:L], "nnz": list(nnz)[:L],
                 "setup_ms": ms.value, "builds": builds.value}
+
+    def momentumPoly(self):
+        """The polynomial preconditioner of momentum_solver 2 for the current
+        operator: M^-1 = (g0 I - g1 D^-1 A) D^-1."""
+        out = np.zeros(5, np.float64)
+        _check(_lib.cfsa_ns_momentum_poly(self._h, out))
+        return {"rho": out[0], "lmax": out[1], "g0": out[2], "g1": out[3],
+                "estimates": int(out[4])}
 
     def elemInfo(self):
         out = np.zeros(4, np.int64)

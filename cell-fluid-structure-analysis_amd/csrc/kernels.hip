@@ -678,6 +678,33 @@ struct OpAxDot2 : OpAx<T, KO, Base> {
   }
 };
 
+// The product of the polynomial preconditioner of CG (momentum_solver 2):
+// input w = D^-1 r, acc = (A w)_row, output z = M^-1 r = g0 w - g1 D^-1 (A w)
+// (the degree-1 Chebyshev polynomial in D^-1 A, times D^-1); pairs: the
+// stopping norm of r (|D^-1 r|^2 on the Jacobi-scaled rule, else |r|^2) and r.z.
+template <class T, int KO, class Base>
+struct OpPolyZ : OpAx<T, KO, Base> {
+  const T *__restrict__ r;
+  const T *__restrict__ invd;  // per node
+  double g0, g1;
+  int scaled;
+  struct Pre {
+    T r, d, w;
+  };
+  __device__ Pre pre(int row, int c) const {
+    const size_t i = (size_t)row * KO + c;
+    return Pre{r[i], invd[row], this->x[i]};
+  }
+  __device__ void epi(int row, int c, T acc, const Pre &pr, DD *d) const {
+    const T z = (T)(g0 * (double)pr.w - g1 * ((double)pr.d * (double)acc));
+    this->y[(size_t)row * KO + c] = z;
+    const double rv = (double)pr.r;
+    const double sn = scaled ? (double)pr.w : rv;
+    dd_fma(d[0], sn, sn);
+    dd_fma(d[1], rv, (double)z);
+  }
+};
+
 template <class T, int KO, class Base>
 struct OpResidual : OpAx<T, KO, Base> {
   const T *__restrict__ b;
@@ -793,6 +820,25 @@ void spmv3_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
   op.x = z;
   op.y = t;
   op.sv = sv;
+  const int G = launch_spmv<T, 3, 2>(s, A, op, partials);
+  *grid = G;
+}
+
+template <class T>
+void spmv3_polyz(hipStream_t s, const DevCsr &A, const T *val, const T *w, T *z, const T *r,
+                 const T *invd, double g0, double g1, int scaled, const int *active,
+                 double *partials, int *grid) {
+  OpPolyZ<T, 3, MaskBase> op{};
+  op.mask = active;
+  op.col = A.col;
+  op.val = val;
+  op.x = w;
+  op.y = z;
+  op.r = r;
+  op.invd = invd;
+  op.g0 = g0;
+  op.g1 = g1;
+  op.scaled = scaled;
   const int G = launch_spmv<T, 3, 2>(s, A, op, partials);
   *grid = G;
 }
@@ -1003,6 +1049,9 @@ void momentum_rhs(hipStream_t s, const DevCsr &A, const double *M, const double 
                              const T *, const int *, double *, int *);                  \
   template void spmv3_dot2<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,   \
                               const T *, const int *, double *, int *);                 \
+  template void spmv3_polyz<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,  \
+                               const T *, const T *, double, double, int, const int *,  \
+                               double *, int *);                                        \
   template void spmv3_residual<T>(hipStream_t, const DevCsr &, const T *, const T *,    \
                                   const T *, T *, T *, const T *, const int *, double *, \
                                   int *);                                               \
@@ -1588,12 +1637,13 @@ __device__ void cg_decide(KrylovState *__restrict__ st, const double *tot, int p
 // then, per value: x += alpha p for the rhs k_cg_xr updated (Eigen's update of
 // x, moved here because this kernel reads p anyway), and p = z + beta p with
 // z = M^-1 r recomputed from r (same rounding as in k_cg_xr) for the rhs that
-// continue.
+// continue, or read from zin (the polynomial preconditioner's product).
 template <class T, int K>
 __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict__ st,
                                                  const double *__restrict__ partials, int Gv,
                                                  int par, const T *__restrict__ r,
                                                  const T *__restrict__ invd,
+                                                 const T *__restrict__ zin,
                                                  T *__restrict__ p, T *__restrict__ x) {
   __shared__ double beta_s[K], alpha_s[K];
   __shared__ int cont_s[K], xact_s[K];
@@ -1631,8 +1681,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict_
       pv[u] = p[j];
       xv[u] = load_once<K>(&x[j]);
       if (cont) {
-        rv[u] = r[j];
-        dv[u] = invd[j / K];
+        rv[u] = zin ? zin[j] : r[j];
+        dv[u] = zin ? T(1) : invd[j / K];
       }
     }
 #pragma unroll
@@ -1649,7 +1699,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int n, KrylovState *__restrict_
     const T pi = p[i];
     store_once<K>((T)((double)load_once<K>(&x[i]) + al * (double)pi), &x[i]);
     if (cont) {
-      const T zi = (T)((double)invd[i / K] * (double)r[i]);
+      const T zi = zin ? zin[i] : (T)((double)invd[i / K] * (double)r[i]);
       p[i] = (T)((double)zi + be * (double)pi);
     }
   }
@@ -1714,9 +1764,137 @@ void cg_xr(hipStream_t s, int n, KrylovState *st, const double *pq_partials, int
 }
 template <class T, int K>
 void cg_p(hipStream_t s, int n, KrylovState *st, const double *partials, int Gv, int par,
-          const T *r, const T *invd, T *p, T *x) {
+          const T *r, const T *invd, T *p, T *x, const T *zin) {
   hipLaunchKernelGGL((k_cg_p<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st, partials,
-                     Gv, par, r, invd, p, x);
+                     Gv, par, r, invd, zin, p, x);
+  CFSA_LAUNCH_CHECK();
+}
+
+// CG with the polynomial preconditioner: x = 0, r = b, w = D^-1 b (the
+// preconditioner product's input; its SpMV emits the initial pairs)
+template <class T, int K>
+__global__ __launch_bounds__(kBlock) void k_cgp_init(int n, const T *__restrict__ b,
+                                                     const T *__restrict__ invd,
+                                                     T *__restrict__ x, T *__restrict__ r,
+                                                     T *__restrict__ w) {
+  const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+    const T bi = b[i];
+    x[i] = T(0);
+    r[i] = bi;
+    w[i] = (T)((double)invd[i / K] * (double)bi);
+  }
+}
+template <class T, int K>
+void cgp_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *w) {
+  hipLaunchKernelGGL((k_cgp_init<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, b, invd,
+                     x, r, w);
+  CFSA_LAUNCH_CHECK();
+}
+
+// k_cg_xr for the polynomial preconditioner: alpha = rho / p.Ap, r -= alpha q
+// and w = D^-1 r (the next product's input); the stopping and rho pairs come
+// from that product
+template <class T, int K>
+__global__ __launch_bounds__(kBlock) void k_cgp_xr(int n, KrylovState *__restrict__ st,
+                                                   const double *__restrict__ pq_partials,
+                                                   int Gq, int par, const T *__restrict__ q,
+                                                   const T *__restrict__ invd,
+                                                   T *__restrict__ r, T *__restrict__ w) {
+  if (blockIdx.x == 0 && threadIdx.x < K) st->act_s[par][threadIdx.x] = st->active[threadIdx.x];
+  if (!st->any_active) return;
+  double pq[K];
+  block_reduce_pairs<K>(pq_partials, Gq, pq);
+  if (blockIdx.x == 0 && threadIdx.x < K && st->active[threadIdx.x]) {
+    const int c = threadIdx.x;
+    st->r0v[c] = pq[c];
+    st->alpha[c] = st->rho_s[par][c] / pq[c];
+  }
+  const size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const int c = (int)(i0 % K);
+  const bool act = st->active[c] != 0;
+  const double al = act ? st->rho_s[par][c] / pq[c] : 0.0;
+  const size_t N = act ? (size_t)n * K : 0, stride = (size_t)gridDim.x * kBlock;
+  size_t i = i0;
+  for (; i + 3 * stride < N; i += 4 * stride) {
+    T rv[4], qv[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t j = i + u * stride;
+      rv[u] = r[j];
+      qv[u] = load_once<K>(&q[j]);
+      dv[u] = invd[j / K];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const T ri = (T)((double)rv[u] - al * (double)qv[u]);
+      r[i + u * stride] = ri;
+      w[i + u * stride] = (T)((double)dv[u] * (double)ri);
+    }
+  }
+  for (; i < N; i += stride) {
+    const T ri = (T)((double)r[i] - al * (double)load_once<K>(&q[i]));
+    r[i] = ri;
+    w[i] = (T)((double)invd[i / K] * (double)ri);
+  }
+}
+template <class T, int K>
+void cgp_xr(hipStream_t s, int n, KrylovState *st, const double *pq_partials, int Gq, int par,
+            const T *q, const T *invd, T *r, T *w) {
+  hipLaunchKernelGGL((k_cgp_xr<T, K>), dim3(vec_grid_k(n, K)), dim3(kBlock), 0, s, n, st,
+                     pq_partials, Gq, par, q, invd, r, w);
+  CFSA_LAUNCH_CHECK();
+}
+
+// Power iteration for the spectral radius of D^-1 A (K columns at once):
+// v = D^-1 y with the pairs of |v|^2 per column, then x = v / |v|
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_power_v(int n, const double *__restrict__ y,
+                                                    const double *__restrict__ invd,
+                                                    double *__restrict__ v,
+                                                    double *__restrict__ partials) {
+  DD acc[1];
+  dd_zero(acc[0]);
+  const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+    const double vi = invd[i / K] * y[i];
+    v[i] = vi;
+    dd_fma(acc[0], vi, vi);
+  }
+  block_partials_binned<K, 1>(acc, partials);
+}
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_power_scale(int n, const double *__restrict__ v,
+                                                        double s0, double s1, double s2,
+                                                        double *__restrict__ x) {
+  const double sc[3] = {s0, s1, s2};
+  const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride)
+    x[i] = v[i] * sc[i % K];
+}
+// x0: a fixed pseudo-random vector, one column per component
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_power_init(int n, double *__restrict__ x) {
+  const size_t N = (size_t)n * K, stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+    const uint32_t h = (uint32_t)(i * 2654435761u) ^ (uint32_t)(i >> 32);
+    x[i] = (double)(h >> 8) / 16777216.0 - 0.5;
+  }
+}
+void power_init3(hipStream_t s, int n, double *x) {
+  hipLaunchKernelGGL((k_power_init<3>), dim3(vec_grid_k(n, 3)), dim3(kBlock), 0, s, n, x);
+  CFSA_LAUNCH_CHECK();
+}
+int power_v3(hipStream_t s, int n, const double *y, const double *invd, double *v,
+             double *partials) {
+  const int G = vec_grid_k(n, 3);
+  hipLaunchKernelGGL((k_power_v<3>), dim3(G), dim3(kBlock), 0, s, n, y, invd, v, partials);
+  CFSA_LAUNCH_CHECK();
+  return G;
+}
+void power_scale3(hipStream_t s, int n, const double *v, const double *sc, double *x) {
+  hipLaunchKernelGGL((k_power_scale<3>), dim3(vec_grid_k(n, 3)), dim3(kBlock), 0, s, n, v,
+                     sc[0], sc[1], sc[2], x);
   CFSA_LAUNCH_CHECK();
 }
 #define CFSA_INST_CG(T, K)                                                                   \
@@ -1725,7 +1903,10 @@ void cg_p(hipStream_t s, int n, KrylovState *st, const double *partials, int Gv,
   template void cg_xr<T, K>(hipStream_t, int, KrylovState *, const double *, int, int,       \
                             const T *, const T *, T *, int, double *, int *);                \
   template void cg_p<T, K>(hipStream_t, int, KrylovState *, const double *, int, int,        \
-                           const T *, const T *, T *, T *);
+                           const T *, const T *, T *, T *, const T *);                        \
+  template void cgp_init<T, K>(hipStream_t, int, const T *, const T *, T *, T *, T *);        \
+  template void cgp_xr<T, K>(hipStream_t, int, KrylovState *, const double *, int, int,      \
+                             const T *, const T *, T *, T *);
 CFSA_INST_CG(double, 1)
 CFSA_INST_CG(double, 3)
 CFSA_INST_CG(float, 1)

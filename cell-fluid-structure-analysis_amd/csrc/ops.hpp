@@ -74,6 +74,12 @@ void spmv3_dot(hipStream_t s, const DevCsr &A, const T *val, const T *y, T *v,
 template <class T>
 void spmv3_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
                 const T *sv, const int *active, double *partials, int *grid);
+// z = g0 w - g1 D^-1 (A w) with w = D^-1 r, pairs [3*2][G]: the stopping norm
+// of r (|w|^2 when scaled, else |r|^2) and r.z
+template <class T>
+void spmv3_polyz(hipStream_t s, const DevCsr &A, const T *val, const T *w, T *z, const T *r,
+                 const T *invd, double g0, double g1, int scaled, const int *active,
+                 double *partials, int *grid);
 // r = b - A x (rhs of the active mask only), r0 = r, partials r.r [3][G]
 template <class T>
 void spmv3_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,
@@ -157,7 +163,22 @@ void cg_xr(hipStream_t s, int n, KrylovState *st, const double *pq_partials, int
            const T *q, const T *invd, T *r, int scaled, double *partials, int *grid);
 template <class T, int K>
 void cg_p(hipStream_t s, int n, KrylovState *st, const double *partials, int Gv, int par,
-          const T *r, const T *invd, T *p, T *x);
+          const T *r, const T *invd, T *p, T *x, const T *zin = nullptr);
+// CG with the polynomial preconditioner M^-1 = (g0 I - g1 D^-1 A) D^-1: x = 0,
+// r = b, w = D^-1 b; then per iteration r -= alpha q, w = D^-1 r (cgp_xr) and
+// z = M^-1 r with the stopping and r.z pairs (spmv3_polyz), p = z + beta p (cg_p
+// with zin = z)
+template <class T, int K>
+void cgp_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *w);
+template <class T, int K>
+void cgp_xr(hipStream_t s, int n, KrylovState *st, const double *pq_partials, int Gq, int par,
+            const T *q, const T *invd, T *r, T *w);
+// power iteration on D^-1 A over three columns: x0, v = D^-1 y with |v|^2 pairs
+// [3][G] (returns G), x = v * sc[column]
+void power_init3(hipStream_t s, int n, double *x);
+int power_v3(hipStream_t s, int n, const double *y, const double *invd, double *v,
+             double *partials);
+void power_scale3(hipStream_t s, int n, const double *v, const double *sc, double *x);
 
 // ---- AMG-preconditioned CG (single right-hand side) ----
 // One level of the smoothed-aggregation hierarchy on the device, in the V-cycle's

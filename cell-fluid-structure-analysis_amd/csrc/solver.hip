@@ -10,6 +10,7 @@
 #include <cstring>
 #include <iostream>
 
+#include "amg.hpp"
 #include "dd.hpp"
 #include "mesh.hpp"
 #include "pattern.hpp"
@@ -445,6 +446,7 @@ void NSSolver::prepare_operators() {
       sell_fill<double>(stream_, storedA, d_Asrc.p, d_lhs.p, d_lhsS.p);
     }
     elem_fresh_ = false;  // the element patches' geometry and penalties follow the lhs
+    mom_poly_fresh_ = false;  // and so does the polynomial preconditioner
     if (mixed) {
       if (d_lhs32.n < (size_t)(la ? storedA : nnzA))
         throw std::logic_error("FP32 momentum operator smaller than its layout");
@@ -744,6 +746,136 @@ SolveResult NSSolver::pcg(const DevCsr &A, const T *val, const T *invd, const T 
   return res;
 }
 
+// Spectral radius of D^-1 A for the polynomial preconditioner (momentum_solver
+// 2): power iteration on three columns of the FP64 momentum operator, 30 steps
+// from a fixed pseudo-random start, 8 from the last vector after the lhs
+// changed; the estimate is |D^-1 A x| for the unit x of the last step, the
+// largest of the three columns.  lmax = 1.1 rho; the degree-2 Chebyshev
+// iteration over [lmax / 30, lmax] from zero (the coefficients of the AMG
+// smoother, amg_chebyshev), x1 = c0 D^-1 r, x2 = x1 + c1 x1 + c2 D^-1 (r - A x1),
+// is x2 = (g0 I - g1 D^-1 A) D^-1 r with g0 = (1 + c1) c0 + c2, g1 = c2 c0.  Its
+// error polynomial stays below 1 on (0, lmax (1 + 1/30)), so M^-1 is positive
+// definite unless the estimate undershoots the spectral radius by 8%.
+void NSSolver::ensure_mom_poly() {
+  if (mom_poly_fresh_) return;
+  const int n = nu;
+  const bool warm = d_pow_x.n == (size_t)n * 3;
+  if (!warm) {
+    d_pow_x.alloc((size_t)n * 3);
+    d_pow_v.alloc((size_t)n * 3);
+    power_init3(stream_, n, d_pow_x.p);
+  }
+  const int steps = warm ? 8 : 30;
+  std::vector<double> part;
+  double rho = 0.0;
+  for (int it = 0; it <= steps; ++it) {
+    spmv3<double>(stream_, momA(), momV(), d_pow_x.p, d_pow_v.p, nullptr);
+    const int G = power_v3(stream_, n, d_pow_v.p, d_invdA.p, d_pow_v.p, d_part_r.p);
+    part.resize((size_t)6 * G);
+    CFSA_HIP(hipMemcpyAsync(part.data(), d_part_r.p, part.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, stream_));
+    CFSA_HIP(hipStreamSynchronize(stream_));
+    double sc[3];
+    rho = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      double s2 = 0.0;
+      for (int g = 0; g < G; ++g)
+        s2 += part[2 * ((size_t)c * G + g)] + part[2 * ((size_t)c * G + g) + 1];
+      const double nrm = std::sqrt(s2);
+      if (!(nrm > 0.0) || !std::isfinite(nrm))
+        throw std::runtime_error("momentum power iteration: zero or non-finite vector");
+      sc[c] = 1.0 / nrm;
+      rho = std::max(rho, nrm);
+    }
+    power_scale3(stream_, n, d_pow_v.p, sc, d_pow_x.p);
+  }
+  double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+  mom_rho_ = rho;
+  mom_lmax_ = 1.1 * rho;
+  amg_chebyshev(mom_lmax_, &c0, &c1, &c2);
+  mom_g0_ = (1.0 + c1) * c0 + c2;
+  mom_g1_ = c2 * c0;
+  mom_poly_fresh_ = true;
+  ++mom_poly_builds_;
+  clear_graphs();  // captured chunks hold the coefficients
+}
+
+// CG with the polynomial preconditioner M^-1 = (g0 I - g1 D^-1 A) D^-1
+// (momentum_solver 2; three right-hand sides).  Eigen's conjugate_gradient
+// step and stopping rule with z = M^-1 r: one iteration is q = A p with p.q;
+// alpha, r -= alpha q and w = D^-1 r; z = g0 w - g1 D^-1 (A w) with the
+// stopping norm and r.z; the convergence test, x += alpha p and p = z + beta p.
+// Two products per iteration, half the iterations of Jacobi CG.
+template <class T, int K>
+SolveResult NSSolver::pcg_poly(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
+                               double tol, int maxit, KrylovWork<T> &w) {
+  static_assert(K == 3, "the polynomial preconditioner runs on the momentum operator");
+  ensure_mom_poly();
+  hipStream_t s = stream_;
+  const int n = A.n;
+  KrylovState *st = d_state.p;
+  const int *act = st->active;
+  const bool time_spmv = opt.timing >= 2;
+  set_state_params(tol, maxit, K);
+  const double g0 = mom_g0_, g1 = mom_g1_;
+  int Gv = 0, Gb = 0;
+  // p = z0 = M^-1 b with the initial stopping and rho pairs
+  cgp_init<T, K>(s, n, b, invd, x, w.r.p, w.y.p);
+  spmv3_polyz<T>(s, A, val, w.y.p, w.p.p, w.r.p, invd, g0, g1, opt.stop_scaled, nullptr,
+                 d_part_a.p, &Gv);
+  cg_s_init<K>(s, st, d_part_a.p, Gv);
+  int chunk = std::max(1, opt.check_every);
+  chunk += chunk & 1;  // even, so that every chunk starts on parity slot 0
+  const long max_chunks = (long)maxit / chunk + 16;
+  auto body = [&](bool timed) {
+    for (int j = 0; j < chunk; ++j) {
+      const bool ev = timed && j == std::min(1, chunk - 1);
+      if (ev) spmv_event_begin();
+      spmv3_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
+      if (ev) spmv_event_end();
+      const int par = j & 1;
+      cgp_xr<T, K>(s, n, st, d_part_b.p, Gb, par, w.v.p, invd, w.r.p, w.y.p);
+      spmv3_polyz<T>(s, A, val, w.y.p, w.z.p, w.r.p, invd, g0, g1, opt.stop_scaled, act,
+                     d_part_a.p, &Gv);
+      cg_p<T, K>(s, n, st, d_part_a.p, Gv, par, w.r.p, invd, w.p.p, x, w.z.p);
+    }
+  };
+  uint64_t gb0 = 0, gb1 = 0;
+  std::memcpy(&gb0, &g0, sizeof g0);
+  std::memcpy(&gb1, &g1, sizeof g1);
+  const std::vector<uintptr_t> key = {
+      5, sizeof(T), K, (uintptr_t)n, (uintptr_t)chunk, (uintptr_t)A.rowptr,
+      (uintptr_t)A.col, (uintptr_t)A.sptr, (uintptr_t)A.elem, (uintptr_t)val, (uintptr_t)invd,
+      (uintptr_t)x, (uintptr_t)w.r.p, (uintptr_t)w.p.p, (uintptr_t)w.v.p, (uintptr_t)w.y.p,
+      (uintptr_t)w.z.p, (uintptr_t)st, (uintptr_t)d_part_a.p, (uintptr_t)d_part_b.p,
+      (uintptr_t)opt.stop_scaled, (uintptr_t)Gv, (uintptr_t)gb0, (uintptr_t)gb1};
+  for (long ch = 0;; ++ch) {
+    if (ch > max_chunks) throw std::runtime_error("polynomial PCG driver did not terminate");
+    read_state();
+    bool done = true;
+    for (int c = 0; c < K; ++c) done = done && h_state->status[c] == KS_DONE;
+    if (done) break;
+    if (time_spmv)
+      body(true);
+    else
+      run_chunk(key, [&] { body(false); });
+  }
+  SolveResult res;
+  res.ok = true;
+  for (int c = 0; c < K; ++c) {
+    res.iters[c] = h_state->iters[c];
+    res.err[c] = h_state->err[c];
+    if (!(h_state->err[c] <= tol)) res.ok = false;
+  }
+  return res;
+}
+template SolveResult NSSolver::pcg_poly<double, 3>(const DevCsr &, const double *,
+                                                   const double *, const double *, double *,
+                                                   double, int, KrylovWork<double> &);
+template SolveResult NSSolver::pcg_poly<float, 3>(const DevCsr &, const float *, const float *,
+                                                  const float *, float *, double, int,
+                                                  KrylovWork<float> &);
+
 // CG preconditioned by one V-cycle of the pressure operator's AMG hierarchy
 // (amg.hpp), Eigen's conjugate_gradient step and stopping rule otherwise.  One
 // iteration: q = A p with p.q; alpha, r -= alpha q and the first smoothing step
@@ -891,7 +1023,12 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double 
     SolveResult in;
     if (method == 1)
       in = pcg<float, K>(A32, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
-    else
+    else if (method == 2) {
+      if constexpr (K == 3)
+        in = pcg_poly<float, K>(A32, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
+      else
+        throw std::logic_error("the polynomial preconditioner needs the momentum operator");
+    } else
       in = bicgstab<float, K>(A32, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
     for (int c = 0; c < K; ++c) res.iters[c] += in.iters[c];
     if (perm)
@@ -924,6 +1061,9 @@ SolveResult NSSolver::solve_momentum(const double *b, double *x) {
   if (opt.momentum_solver == 1)
     return pcg<double, 3>(momA(), momV(), d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
                           w3_);
+  if (opt.momentum_solver == 2)
+    return pcg_poly<double, 3>(momA(), momV(), d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
+                               w3_);
   return bicgstab<double, 3>(momA(), momV(), d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
                              w3_);
 }
@@ -1362,8 +1502,9 @@ void NSSolver::apply_matrix(int which, const double *x, double *y, int nrhs) {
   dy.download(y, (size_t)n * nrhs);
 }
 
-// Solve matrix `which` (2 momentum lhs, 4 L_bc) with method 0 BiCGSTAB / 1 PCG
-// in FP64 (precision 0) or FP32-inner refinement (precision 1).
+// Solve matrix `which` (2 momentum lhs, 4 L_bc) with method 0 BiCGSTAB / 1 PCG /
+// 2 AMG-PCG (L_bc) / 3 polynomial PCG (momentum, 3 rhs) in FP64 (precision 0) or
+// FP32-inner refinement (precision 1).
 SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double *b,
                                    double *x, double tol, int maxit, int precision) {
   if (which != 2 && which != 4) throw std::invalid_argument("krylov_solve: matrix 2 or 4");
@@ -1379,7 +1520,15 @@ SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double
   db.upload(b, (size_t)n * nrhs);
   dx.alloc((size_t)n * nrhs);
   SolveResult r;
-  if (method == 2) {
+  if (method == 3) {
+    if (which != 2 || nrhs != 3)
+      throw std::invalid_argument("krylov_solve: method 3 (polynomial PCG) solves the momentum "
+                                  "matrix with 3 rhs");
+    if (precision == 1)
+      r = refine<3>(A, momV(), d_invdA.p, d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit, 2);
+    else
+      r = pcg_poly<double, 3>(A, momV(), d_invdA.p, db.p, dx.p, tol, maxit, w3_);
+  } else if (method == 2) {
     if (which != 4) throw std::invalid_argument("krylov_solve: method 2 (AMG-PCG) solves L_bc");
     r = precision == 1 ? pcg_amg<float>(db.p, dx.p, tol, maxit)
                        : pcg_amg<double>(db.p, dx.p, tol, maxit);

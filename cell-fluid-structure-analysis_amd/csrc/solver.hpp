@@ -88,7 +88,8 @@ struct SolverOptions {
   int max_iter = 1000;         // NavierStokes.cpp:499,634
   int pressure_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG, 2 AMG-preconditioned
                                // CG (FP64 in every precision mode)
-  int momentum_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG (lhs is SPD)
+  int momentum_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG (lhs is SPD), 2 CG
+                               // with the degree-1 Chebyshev polynomial preconditioner
   int precision = 0;           // 0 FP64, 1 FP32 inner + FP64 refinement, 2 momentum only
   double inner_tol = 1e-4;     // relative tolerance of each FP32 inner solve
   int max_refine = 40;         // outer refinement sweeps (mixed)
@@ -253,6 +254,18 @@ class NSSolver {
     out[3] = elem_built_ ? 1 : 0;
     if (setup_ms) *setup_ms = elem_setup_ms_;
   }
+  // momentum_solver 2: the polynomial preconditioner M^-1 = (g0 I - g1 D^-1 A) D^-1
+  // of the current lhs (estimating the spectral radius of D^-1 A if needed):
+  // out = {rho estimate, lmax, g0, g1, estimates so far}
+  void momentum_poly(double out[5]) {
+    prepare_operators();
+    ensure_mom_poly();
+    out[0] = mom_rho_;
+    out[1] = mom_lmax_;
+    out[2] = mom_g0_;
+    out[3] = mom_g1_;
+    out[4] = mom_poly_builds_;
+  }
   hipStream_t stream() const { return stream_; }
   // Momentum-matrix SpMV launches repeated `reps` times between two events.
   double time_momentum_spmv(int reps, int precision);
@@ -339,9 +352,17 @@ class NSSolver {
   ElemPatchDev elem_dev_{};
   DevCsr elemA_{};
   bool elem_built_ = false, elem_fresh_ = false;
+  // momentum_solver 2: spectral radius estimate of D^-1 A (power iteration on the
+  // FP64 operator, refreshed with the lhs; later estimates start from the last
+  // vector) and the polynomial preconditioner's coefficients
+  bool mom_poly_fresh_ = false;
+  int mom_poly_builds_ = 0;
+  double mom_rho_ = 0.0, mom_lmax_ = 0.0, mom_g0_ = 0.0, mom_g1_ = 0.0;
+  DBuf<double> d_pow_x, d_pow_v;
   int64_t elem_nvisit_ = 0, elem_nrows_ = 0;
   double elem_setup_ms_ = 0.0;
   void ensure_elem();
+  void ensure_mom_poly();
   DBuf<double> d_bsb, d_bsx, d_bsinvd;  // solve_poisson_bicgstab operands (kept: graph keys)
   KrylovWork<float> f3_, f1_;
   DBuf<float> d_b32, d_x32;
@@ -434,6 +455,9 @@ class NSSolver {
 
   template <class T, int K>
   SolveResult bicgstab(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
+                       double tol, int maxit, KrylovWork<T> &w);
+  template <class T, int K>
+  SolveResult pcg_poly(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
                        double tol, int maxit, KrylovWork<T> &w);
   template <class T, int K>
   SolveResult pcg(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,

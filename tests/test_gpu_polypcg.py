@@ -1,0 +1,213 @@
+"""CG with the degree-1 Chebyshev polynomial preconditioner for the momentum
+solve (momentum_solver 2; an extension: the reference solves the momentum
+system with Jacobi BiCGSTAB, NavierStokes.cpp:498-515):
+
+* the spectral radius estimate of D^-1 A (power iteration on the device) is
+  within 5% below the largest eigenvalue that scipy finds, lmax = 1.1 rho
+  covers it, and the preconditioner M^-1 = (g0 I - g1 D^-1 A) D^-1 has the
+  error polynomial 1 - l (g0 - g1 l) = T2((theta - l)/delta) / T2(theta/delta)
+  of the degree-2 Chebyshev iteration over [lmax/30, lmax], positive
+  definite up to lmax (1 + 1/30);
+* the FP64 solve equals an independent NumPy restatement of the same CG
+  (same coefficients, Eigen's stopping rule): iteration counts within one,
+  solutions to the tolerance, and a direct solve;
+* PISO steps with it (FP64, mixed precision with the element-patch operator)
+  match the FP64 oracle at momentum 1e-10 / pressure 1e-8 on the
+  Jacobi-scaled rule: velocity <= 1e-7, pressure <= 1e-6; the estimate
+  follows a moved mesh; graph replay and eager launches give the same bits.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from test_gpu_parity import relerr
+
+pytestmark = pytest.mark.gpu
+
+RHO, MU = 1060.0, 0.004
+
+
+def _mesh(P, kind):
+    if kind == "cyl":
+        pm = P.Mesh.cylinder(10.0, 1.0, 16, 0.1)
+    else:
+        n = int(kind[4:])
+        pm = P.Mesh.structuredRectangularPrism(1.0, 1.0, 1.0, n, n, n)
+    pm.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+    return pm
+
+
+def _solver(P, pm, dt=0.01, **opts):
+    g = P.NavierStokesSolver()
+    g.initialize(pm, RHO, MU)
+    g.setDt(dt)
+    g.setMeanInletVelocity((0.5, 0.0, 0.0))
+    if opts:
+        g.setOptions(**opts)
+    return g
+
+
+def _lhs(g):
+    rp, col, val = g.getMatrix("lhs")
+    n = len(rp) - 1
+    return sp.csr_matrix((val, col, rp), shape=(n, n))
+
+
+@pytest.mark.parametrize("kind", ["kuhn8", "cyl"])
+def test_coefficients(P, kind):
+    g = _solver(P, _mesh(P, kind), momentum_solver=2)
+    A = _lhs(g)
+    d = A.diagonal()
+    S = (sp.diags(1.0 / np.sqrt(d)) @ A @ sp.diags(1.0 / np.sqrt(d))).tocsr()
+    lam = sla.eigsh(S, k=1, which="LA", return_eigenvectors=False, tol=1e-10)[0]
+    c = g.momentumPoly()
+    print(f"{kind}: rho {c['rho']:.6f} (largest eigenvalue {lam:.6f}), lmax {c['lmax']:.6f}, "
+          f"g0 {c['g0']:.6f}, g1 {c['g1']:.6f}", flush=True)
+    assert 0.95 * lam <= c["rho"] <= lam * (1 + 1e-3)
+    assert c["lmax"] == pytest.approx(1.1 * c["rho"], rel=1e-15) and c["lmax"] >= lam
+    lmax = c["lmax"]
+    lmin = lmax / 30.0
+    th, de = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+
+    def t2(t):
+        return 2 * t * t - 1
+
+    for lv in np.linspace(0.0, lmax, 23):
+        err = 1.0 - lv * (c["g0"] - c["g1"] * lv)
+        assert abs(err - t2((th - lv) / de) / t2(th / de)) <= 1e-12
+    # p(l) = g0 - g1 l vanishes at lmax + lmin = lmax (1 + 1/30)
+    assert c["g0"] / c["g1"] == pytest.approx(lmax * (1 + 1 / 30), rel=1e-12)
+    lv = np.linspace(1e-9, lmax * (1 + 1 / 30) * (1 - 1e-9), 101)
+    assert np.all(c["g0"] - c["g1"] * lv > 0)
+    # the estimate is made once per operator
+    assert g.momentumPoly()["estimates"] == c["estimates"] == 1
+
+
+def _poly_pcg(A, b, g0, g1, tol, maxit=5000):
+    """The same CG written out: z = (g0 w - g1 D^-1 A w), w = D^-1 r; stop
+    when |r|^2 < tol^2 |b|^2 after the update of r, counting iterations as
+    Eigen's conjugate_gradient does (the last, converging, update uncounted)."""
+    dinv = 1.0 / A.diagonal()
+
+    def prec(r):
+        w = dinv * r
+        return g0 * w - g1 * (dinv * (A @ w))
+
+    x = np.zeros_like(b)
+    r = b.copy()
+    thr = tol * tol * (b @ b)
+    z = prec(r)
+    p = z.copy()
+    rho = r @ z
+    for it in range(1, maxit + 1):
+        q = A @ p
+        al = rho / (p @ q)
+        x += al * p
+        r -= al * q
+        if r @ r < thr:
+            return x, it - 1
+        z = prec(r)
+        rn = r @ z
+        p = z + (rn / rho) * p
+        rho = rn
+    return x, maxit
+
+
+@pytest.mark.parametrize("kind", ["kuhn8", "cyl"])
+def test_solve_matches_restatement(P, kind):
+    g = _solver(P, _mesh(P, kind), momentum_solver=2)
+    A = _lhs(g)
+    n = A.shape[0]
+    b = np.random.default_rng(5).standard_normal((n, 3))
+    c = g.momentumPoly()
+    tol = 1e-10
+    x, r = g.krylovSolve("lhs", b, method=3, tol=tol, maxit=5000)
+    _, rj = g.krylovSolve("lhs", b, method=1, tol=tol, maxit=5000)
+    xd = np.column_stack([sla.spsolve(A.tocsc(), b[:, k]) for k in range(3)])
+    for k in range(3):
+        xr, it = _poly_pcg(A, b[:, k], c["g0"], c["g1"], tol)
+        print(f"{kind} rhs {k}: GPU {r['iters'][k]} iterations (Jacobi CG {rj['iters'][k]}), "
+              f"restatement {it}; |x - x_restated| {relerr(x[:, k], xr):.1e}, "
+              f"|x - x_direct| {relerr(x[:, k], xd[:, k]):.1e}", flush=True)
+        assert abs(r["iters"][k] - it) <= 1
+        assert r["iters"][k] <= 0.65 * rj["iters"][k]
+        assert relerr(x[:, k], xr) <= 1e-8 and relerr(x[:, k], xd[:, k]) <= 1e-8
+        assert np.linalg.norm(b[:, k] - A @ x[:, k]) <= 1.001 * tol * np.linalg.norm(b[:, k])
+    assert r["ok"]
+
+
+@pytest.mark.parametrize("kind,precision", [("kuhn6", 0), ("kuhn6", 2), ("kuhn6", 1),
+                                            ("cyl", 0), ("cyl", 1)])
+def test_steps_vs_oracle(P, O, kind, precision):
+    pm = _mesh(P, kind)
+    ncorr = 2 if kind == "cyl" else 1
+    g = _solver(P, pm, stop_rule=1, momentum_tol=1e-10, pressure_tol=1e-8, pressure_solver=1,
+                precision=precision, momentum_solver=2, elem_spmv=1 if precision else 0)
+    g.setNumberOfCorrections(ncorr)
+    om = O.OracleMesh.fromArrays(pm.vertices(), pm.tets())
+    om.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+    o = O.OracleNS(om, RHO, MU)
+    o.setDt(0.01)
+    o.setMeanInletVelocity((0.5, 0.0, 0.0))
+    o.setNumberOfCorrections(ncorr)
+    o.setSolver(1e-10, 1e-8, 1000, 1)
+    o.setStopScaled(True)
+    for step in range(3):
+        assert g.pisoStep(), P.last_error()
+        assert o.pisoStep() == 0
+        st = g.stepStats()
+        eu = relerr(g.getVelocity(), o.getVelocity())
+        ep = relerr(g.getPressure(), o.getPressure())
+        print(f"{kind} precision {precision} step {step}: momentum iterations "
+              f"{st['momentum_iters']} ({st['momentum_refinements']} sweeps); velocity "
+              f"{eu:.2e} pressure {ep:.2e}", flush=True)
+        assert eu <= 1e-7 and ep <= 1e-6, (step, eu, ep)
+
+
+def test_estimate_follows_moved_mesh(P, O):
+    pm = _mesh(P, "kuhn6")
+    g = _solver(P, pm, stop_rule=1, momentum_tol=1e-10, pressure_tol=1e-8, momentum_solver=2)
+    assert g.pisoStep(), P.last_error()
+    c0 = g.momentumPoly()
+    X = pm.vertices()
+    Xd = X.copy()
+    Xd[:, 1] += 0.05 * np.sin(np.pi * X[:, 0]) * X[:, 1] * (1 - X[:, 1])
+    pm.updateVertices(Xd)
+    g.build_sparse_matrices()
+    c1 = g.momentumPoly()
+    assert c1["estimates"] == c0["estimates"] + 1
+    A = _lhs(g)
+    d = A.diagonal()
+    S = (sp.diags(1.0 / np.sqrt(d)) @ A @ sp.diags(1.0 / np.sqrt(d))).tocsr()
+    lam = sla.eigsh(S, k=1, which="LA", return_eigenvectors=False, tol=1e-10)[0]
+    assert 0.95 * lam <= c1["rho"] <= lam * (1 + 1e-3) and c1["lmax"] >= lam
+    # a step on the moved mesh against the oracle built from the moved coordinates
+    om = O.OracleMesh.fromArrays(Xd, pm.tets())
+    om.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
+    o = O.OracleNS(om, RHO, MU)
+    o.setDt(0.01)
+    o.setMeanInletVelocity((0.5, 0.0, 0.0))
+    o.setSolver(1e-10, 1e-8, 1000, 0)
+    o.setStopScaled(True)
+    o.setVelocity(g.getVelocity())
+    o.setPressure(g.getPressure())
+    assert g.pisoStep(), P.last_error()
+    assert o.pisoStep() == 0
+    assert relerr(g.getVelocity(), o.getVelocity()) <= 1e-7
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graphs_bit_identical(P, precision):
+    pm = _mesh(P, "cyl")
+    outs = []
+    for graphs in (1, 0, 1):
+        g = _solver(P, pm, stop_rule=1, momentum_tol=1e-10, pressure_tol=1e-8,
+                    precision=precision, momentum_solver=2, elem_spmv=1 if precision else 0,
+                    graphs=graphs)
+        for _ in range(2):
+            assert g.pisoStep(), P.last_error()
+        outs.append((g.getVelocity(), g.getPressure(), g.stepStats()["momentum_iters"]))
+    for u, p, it in outs[1:]:
+        assert np.array_equal(u, outs[0][0]) and np.array_equal(p, outs[0][1])
+        assert it == outs[0][2]
