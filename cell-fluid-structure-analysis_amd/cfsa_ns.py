@@ -279,7 +279,7 @@ class Mesh:
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
-        if h:
+        if h and _lib is not None:
             _lib.cfsa_mesh_destroy(h)
 
     @property
@@ -406,7 +406,7 @@ class NavierStokesSolver:
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
-        if h:
+        if h and _lib is not None:
             _lib.cfsa_ns_destroy(h)
 
     @property
@@ -806,7 +806,7 @@ class HostAmg:
         self._h = h
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and _lib is not None:
             _lib.cfsa_amg_free(self._h)
             self._h = None
 
@@ -875,7 +875,7 @@ class HostElemPatches:
         self.npatch = npatch
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and _lib is not None:
             _lib.cfsa_elem_patches_free(self._h)
             self._h = None
 

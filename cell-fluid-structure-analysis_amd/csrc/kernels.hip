@@ -343,12 +343,12 @@ __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4)))
   unsigned short stv[SPT];
 #pragma unroll
   for (int u = 0; u < SPT; ++u) stv[u] = E.starts[r0 + p + min(tid + u * kElemBlock, nr)];
-  typename Op::Pre pre2[PMAX];
+  typename Op::PreE pre2[PMAX];
   float pen2[PMAX];
 #pragma unroll
   for (int k = 0; k < PMAX; ++k) {
     const int i = min(tid / 3 + k * RP, nr - 1);
-    pre2[k] = op.pre(r0 + i, myc);
+    pre2[k] = op.pre_e(r0 + i, myc);
     pen2[k] = E.pen[r0 + i];
   }
   const size_t ta = visit_tet(sa0, sb0), tb = visit_tet(sa1, sb1);
@@ -442,21 +442,24 @@ __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4)))
     visit(ca, cb, vg[t], vg[ntt + t], v < nvis);
   }
   __syncthreads();
-  DD dots[NDs];
+  // inner products in plain FP64: the operands are FP32 values, whose
+  // products are exact in FP64, and the sums run in a fixed order
+  double dots[NDs];
 #pragma unroll
-  for (int d = 0; d < NDs; ++d) dd_zero(dots[d]);
+  for (int d = 0; d < NDs; ++d) dots[d] = 0.0;
   // all of the thread's row sums first, then their epilogues, so no LDS read
-  // of a later row waits behind an earlier row's global stores
-  float accs[PMAX];
+  // of a later row waits behind an earlier row's global stores; the row's own
+  // x value comes from the staged copy (owned rows lead the node table)
+  float accs[PMAX], xrow[PMAX];
 #pragma unroll
   for (int k = 0; k < PMAX; ++k) {
     const int i = tid / 3 + k * RP;
-    float acc = 0.0f;
+    float acc = 0.0f, px = 0.0f;
     if (rowthr && i < nr) {
       // the row's slots in ascending tet order, LDS reads batched by 8 and 4
       int sl = st[i];
       const int se = st[i + 1];
-      const float px = xs[i * 3 + myc];  // owned rows lead the node table
+      px = xs[i * 3 + myc];
       for (; sl + 8 <= se; sl += 8) {
         float c8[8];
 #pragma unroll
@@ -476,43 +479,35 @@ __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4)))
       if (pen2[k] != 0.0f) acc += pen2[k] * px;
     }
     accs[k] = acc;
+    xrow[k] = px;
   }
   if (rowthr) {
 #pragma unroll
     for (int k = 0; k < PMAX; ++k) {
       const int i = tid / 3 + k * RP;
       // rows are numbered in patch order
-      if (i < nr) op.epi(r0 + i, myc, accs[k], pre2[k], dots);
+      if (i < nr) op.epi_e(r0 + i, myc, accs[k], pre2[k], xrow[k], dots);
     }
   }
   if constexpr (ND > 0) {
     double *rh = reinterpret_cast<double *>(cnt);  // [ND][kElemBlock]
-    double *rl = rh + ND * kElemBlock;
     __syncthreads();
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      rh[d * kElemBlock + tid] = dots[d].hi;
-      rl[d * kElemBlock + tid] = dots[d].lo;
-    }
+    for (int d = 0; d < ND; ++d) rh[d * kElemBlock + tid] = dots[d];
     __syncthreads();
-    // value v = (component c, product d) owns 32 lanes: lane l merges the
-    // threads c + 3 (l + 32 q), then a fixed shuffle tree
+    // value v = (component c, product d) owns 32 lanes: lane l sums the
+    // threads c + 3 (l + 32 q), then a fixed shuffle tree; the block's pair
+    // is (sum, 0) in the double-double partial format
     if (tid < NV * 32) {
       const int v = tid / 32, l = tid % 32;
       const int c = v / ND, d = v % ND;
-      DD s;
-      dd_zero(s);
-      for (int t = c + 3 * l; t < kElemBlock; t += 3 * 32)
-        dd_merge(s, rh[d * kElemBlock + t], rl[d * kElemBlock + t]);
+      double sum = 0.0;
+      for (int t = c + 3 * l; t < kElemBlock; t += 3 * 32) sum += rh[d * kElemBlock + t];
 #pragma unroll
-      for (int sh = 16; sh > 0; sh >>= 1) {
-        const double oh = __shfl_down(s.hi, sh, 32);
-        const double ol = __shfl_down(s.lo, sh, 32);
-        dd_merge(s, oh, ol);
-      }
+      for (int sh = 16; sh > 0; sh >>= 1) sum += __shfl_down(sum, sh, 32);
       if (l == 0) {
-        wpart[2 * ((size_t)v * E.npatch + p)] = s.hi;
-        wpart[2 * ((size_t)v * E.npatch + p) + 1] = s.lo;
+        wpart[2 * ((size_t)v * E.npatch + p)] = sum;
+        wpart[2 * ((size_t)v * E.npatch + p) + 1] = 0.0;
       }
     }
   }
@@ -648,6 +643,13 @@ struct OpAx : Base {
   __device__ void epi(int row, int c, T acc, const typename Base::Pre &, DD *) const {
     y[(size_t)row * KO + c] = acc;
   }
+  // element-patch kernel: PreE / pre_e / epi_e take the row's x value from
+  // the kernel's staged copy and accumulate inner products in plain FP64
+  struct PreE {};
+  __device__ PreE pre_e(int, int) const { return {}; }
+  __device__ void epi_e(int row, int c, T acc, const PreE &, T, double *) const {
+    y[(size_t)row * KO + c] = acc;
+  }
 };
 
 template <class T, int KO, class Base>
@@ -660,6 +662,16 @@ struct OpAxDot : OpAx<T, KO, Base> {
   __device__ void epi(int row, int c, T acc, const Pre &pr, DD *d) const {
     this->y[(size_t)row * KO + c] = acc;
     dd_fma(d[0], (double)pr.r0, (double)acc);
+  }
+  // (CG's p.Ap passes r0 == x: the operand is then not loaded again)
+  using PreE = Pre;
+  __device__ PreE pre_e(int row, int c) const {
+    return PreE{r0 == this->x ? T(0) : r0[(size_t)row * KO + c]};
+  }
+  __device__ void epi_e(int row, int c, T acc, const PreE &pr, T xr, double *d) const {
+    this->y[(size_t)row * KO + c] = acc;
+    const T o = r0 == this->x ? xr : pr.r0;
+    d[0] = fma((double)o, (double)acc, d[0]);
   }
 };
 
@@ -675,6 +687,14 @@ struct OpAxDot2 : OpAx<T, KO, Base> {
     const double t = (double)acc;
     dd_fma(d[0], t, t);
     dd_fma(d[1], t, (double)pr.sv);
+  }
+  using PreE = Pre;
+  __device__ PreE pre_e(int row, int c) const { return PreE{sv[(size_t)row * KO + c]}; }
+  __device__ void epi_e(int row, int c, T acc, const PreE &pr, T, double *d) const {
+    this->y[(size_t)row * KO + c] = acc;
+    const double t = (double)acc;
+    d[0] = fma(t, t, d[0]);
+    d[1] = fma(t, (double)pr.sv, d[1]);
   }
 };
 
@@ -703,6 +723,21 @@ struct OpPolyZ : OpAx<T, KO, Base> {
     dd_fma(d[0], sn, sn);
     dd_fma(d[1], rv, (double)z);
   }
+  // w is the kernel's x: its row value comes from the staged copy
+  struct PreE {
+    T r, d;
+  };
+  __device__ PreE pre_e(int row, int c) const {
+    return PreE{r[(size_t)row * KO + c], invd[row]};
+  }
+  __device__ void epi_e(int row, int c, T acc, const PreE &pr, T w, double *d) const {
+    const T z = (T)(g0 * (double)w - g1 * ((double)pr.d * (double)acc));
+    this->y[(size_t)row * KO + c] = z;
+    const double rv = (double)pr.r;
+    const double sn = scaled ? (double)w : rv;
+    d[0] = fma(sn, sn, d[0]);
+    d[1] = fma(rv, (double)z, d[1]);
+  }
 };
 
 template <class T, int KO, class Base>
@@ -723,6 +758,16 @@ struct OpResidual : OpAx<T, KO, Base> {
     r0[i] = r;
     const double rs = sc ? (double)pr.sc * (double)r : (double)r;
     dd_fma(d[0], rs, rs);
+  }
+  using PreE = Pre;
+  __device__ PreE pre_e(int row, int c) const { return pre(row, c); }
+  __device__ void epi_e(int row, int c, T acc, const PreE &pr, T, double *d) const {
+    const size_t i = (size_t)row * KO + c;
+    const T r = pr.b - acc;
+    this->y[i] = r;
+    r0[i] = r;
+    const double rs = sc ? (double)pr.sc * (double)r : (double)r;
+    d[0] = fma(rs, rs, d[0]);
   }
 };
 
