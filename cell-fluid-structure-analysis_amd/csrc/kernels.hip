@@ -353,13 +353,25 @@ __global__ __launch_bounds__(kElemBlock) __attribute__((amdgpu_waves_per_eu(4)))
   }
   const size_t ta = visit_tet(sa0, sb0), tb = visit_tet(sa1, sb1);
   const float4 ga0 = vg[ta], gb0 = vg[ntt + ta], ga1 = vg[tb], gb1 = vg[ntt + tb];
-  // the patch's x values, one gather per node (a node serves ~14 visits)
+  // the patch's x values, one gather per node (a node serves ~14 visits);
+  // an operator may stage x scaled per node (x = D^-1 r formed here)
   {
     float xv[NPT][3];
 #pragma unroll
     for (int u = 0; u < NPT; ++u)
 #pragma unroll
       for (int c = 0; c < 3; ++c) xv[u][c] = x[(size_t)jj[u] * 3 + c];
+    if constexpr (Op::kStageScale) {
+      if (op.sscale) {
+        float sv[NPT];
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) sv[u] = op.sscale[jj[u]];
+#pragma unroll
+        for (int u = 0; u < NPT; ++u)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) xv[u][c] = (float)((double)sv[u] * (double)xv[u][c]);
+      }
+    }
 #pragma unroll
     for (int u = 0; u < NPT; ++u)
 #pragma unroll
@@ -650,6 +662,7 @@ struct OpAx : Base {
   __device__ void epi_e(int row, int c, T acc, const PreE &, T, double *) const {
     y[(size_t)row * KO + c] = acc;
   }
+  static constexpr bool kStageScale = false;
 };
 
 template <class T, int KO, class Base>
@@ -723,7 +736,11 @@ struct OpPolyZ : OpAx<T, KO, Base> {
     dd_fma(d[0], sn, sn);
     dd_fma(d[1], rv, (double)z);
   }
-  // w is the kernel's x: its row value comes from the staged copy
+  // w is the kernel's x: its row value comes from the staged copy.  With
+  // sscale = D^-1 the element kernel is given x = r and stages w = D^-1 r
+  // itself (the same rounding as a stored w), so w need not be stored.
+  static constexpr bool kStageScale = true;
+  const T *sscale;
   struct PreE {
     T r, d;
   };
@@ -874,6 +891,13 @@ void spmv3_polyz(hipStream_t s, const DevCsr &A, const T *val, const T *w, T *z,
                  const T *invd, double g0, double g1, int scaled, const int *active,
                  double *partials, int *grid) {
   OpPolyZ<T, 3, MaskBase> op{};
+  if (w == nullptr) {
+    // w = D^-1 r formed in the element kernel's staging
+    if (!elem_product(A, sizeof(T)))
+      throw std::logic_error("spmv3_polyz: only the element-patch kernel forms w itself");
+    w = r;
+    op.sscale = invd;
+  }
   op.mask = active;
   op.col = A.col;
   op.val = val;
@@ -1838,7 +1862,8 @@ void cgp_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *w)
 }
 
 // k_cg_xr for the polynomial preconditioner: alpha = rho / p.Ap, r -= alpha q
-// and w = D^-1 r (the next product's input); the stopping and rho pairs come
+// and w = D^-1 r (the next product's input; not stored when w == nullptr,
+// the element-patch product then forms it); the stopping and rho pairs come
 // from that product
 template <class T, int K>
 __global__ __launch_bounds__(kBlock) void k_cgp_xr(int n, KrylovState *__restrict__ st,
@@ -1874,13 +1899,13 @@ __global__ __launch_bounds__(kBlock) void k_cgp_xr(int n, KrylovState *__restric
     for (int u = 0; u < 4; ++u) {
       const T ri = (T)((double)rv[u] - al * (double)qv[u]);
       r[i + u * stride] = ri;
-      w[i + u * stride] = (T)((double)dv[u] * (double)ri);
+      if (w) w[i + u * stride] = (T)((double)dv[u] * (double)ri);
     }
   }
   for (; i < N; i += stride) {
     const T ri = (T)((double)r[i] - al * (double)load_once<K>(&q[i]));
     r[i] = ri;
-    w[i] = (T)((double)invd[i / K] * (double)ri);
+    if (w) w[i] = (T)((double)invd[i / K] * (double)ri);
   }
 }
 template <class T, int K>

@@ -48,6 +48,11 @@ struct DevCsr {
   int64_t wpart_cap = 0;  // values x windows the pair scratch holds
   const ElemPatchDev *elem = nullptr;  // host struct; set: the element-patch SpMV
 };
+// whether a three-rhs product with values of value_size bytes on A runs in the
+// element-patch kernel (FP32 values on an operator that carries patches)
+inline bool elem_product(const DevCsr &A, size_t value_size) {
+  return value_size == 4 && A.elem != nullptr;
+}
 
 // Nonzeros staged per CSR-stream window (one block), measured best at 96^3
 // with the fused inner products: 3 components 1024 (FP64, 24 KiB of LDS) or
@@ -75,7 +80,8 @@ template <class T>
 void spmv3_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
                 const T *sv, const int *active, double *partials, int *grid);
 // z = g0 w - g1 D^-1 (A w) with w = D^-1 r, pairs [3*2][G]: the stopping norm
-// of r (|w|^2 when scaled, else |r|^2) and r.z
+// of r (|w|^2 when scaled, else |r|^2) and r.z; w == nullptr (element-patch
+// products only): w is formed from r while x is staged
 template <class T>
 void spmv3_polyz(hipStream_t s, const DevCsr &A, const T *val, const T *w, T *z, const T *r,
                  const T *invd, double g0, double g1, int scaled, const int *active,
@@ -172,7 +178,7 @@ template <class T, int K>
 void cgp_init(hipStream_t s, int n, const T *b, const T *invd, T *x, T *r, T *w);
 template <class T, int K>
 void cgp_xr(hipStream_t s, int n, KrylovState *st, const double *pq_partials, int Gq, int par,
-            const T *q, const T *invd, T *r, T *w);
+            const T *q, const T *invd, T *r, T *w);  // w == nullptr: not stored
 // power iteration on D^-1 A over three columns: x0, v = D^-1 y with |v|^2 pairs
 // [3][G] (returns G), x = v * sc[column]
 void power_init3(hipStream_t s, int n, double *x);

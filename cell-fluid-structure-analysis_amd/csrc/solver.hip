@@ -827,6 +827,8 @@ SolveResult NSSolver::pcg_poly(const DevCsr &A, const T *val, const T *invd, con
   int chunk = std::max(1, opt.check_every);
   chunk += chunk & 1;  // even, so that every chunk starts on parity slot 0
   const long max_chunks = (long)maxit / chunk + 16;
+  // the element-patch product forms w = D^-1 r from r itself: w is not stored
+  T *wst = elem_product(A, sizeof(T)) ? nullptr : w.y.p;
   auto body = [&](bool timed) {
     for (int j = 0; j < chunk; ++j) {
       const bool ev = timed && j == std::min(1, chunk - 1);
@@ -834,8 +836,8 @@ SolveResult NSSolver::pcg_poly(const DevCsr &A, const T *val, const T *invd, con
       spmv3_dot<T>(s, A, val, w.p.p, w.v.p, w.p.p, act, d_part_b.p, &Gb);
       if (ev) spmv_event_end();
       const int par = j & 1;
-      cgp_xr<T, K>(s, n, st, d_part_b.p, Gb, par, w.v.p, invd, w.r.p, w.y.p);
-      spmv3_polyz<T>(s, A, val, w.y.p, w.z.p, w.r.p, invd, g0, g1, opt.stop_scaled, act,
+      cgp_xr<T, K>(s, n, st, d_part_b.p, Gb, par, w.v.p, invd, w.r.p, wst);
+      spmv3_polyz<T>(s, A, val, wst, w.z.p, w.r.p, invd, g0, g1, opt.stop_scaled, act,
                      d_part_a.p, &Gv);
       cg_p<T, K>(s, n, st, d_part_a.p, Gv, par, w.r.p, invd, w.p.p, x, w.z.p);
     }
