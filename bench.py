@@ -373,7 +373,8 @@ def main():
     ap.add_argument("--skip-cpu-steps", action="store_true",
                     help="skip the whole-step oracle timing on the two cylinders (~1 min)")
     ap.add_argument("--fp64-only", action="store_true")
-    ap.add_argument("--only", default=None, help="run only this mixed variant (profiling)")
+    ap.add_argument("--only", default=None,
+                    help="run only these variants, comma-separated (fp64 always runs)")
     ap.add_argument("--elem-spmv", type=int, default=1, choices=[0, 1],
                     help="mixed variants: FP32 momentum operator applied tet by tet (1) or "
                          "from its assembled CSR copy (0)")
@@ -436,7 +437,7 @@ def main():
 
     res = {}
     for name, msolver in FP64:
-        if args.only and name != "fp64" and args.only != name:
+        if args.only and name != "fp64" and name not in args.only.split(","):
             continue
         t0 = time.perf_counter()
         g = make_solver(P, mesh, prof, args, 0, msolver)
@@ -478,7 +479,7 @@ def main():
     primary, best = "fp64", None
     if not args.fp64_only:
         for name, precision, msolver, guess in MIXED:
-            if args.only and name != args.only:
+            if args.only and name not in args.only.split(","):
                 continue
             t0 = time.perf_counter()
             g = make_solver(P, mesh, prof, args, precision, msolver, guess)
