@@ -205,23 +205,26 @@ def _cpu_baseline(O, g, fp64_stats, sz, pressure_method, pressure_iters=None):
 PRESSURE_SOLVERS = {0: "bicgstab_jacobi", 1: "cg_jacobi", 2: "cg_amg"}
 
 # FP64 variants: (name, momentum solver); the reference's BiCGSTAB first
-FP64 = [("fp64", 0), ("fp64_cg", 1), ("fp64_cgpoly", 2)]
-MOMENTUM_SOLVERS = {0: "bicgstab_jacobi", 1: "cg_jacobi", 2: "cg_chebyshev_poly"}
+FP64 = [("fp64", 0), ("fp64_cg", 1), ("fp64_cgpoly", 2), ("fp64_cgpoly2", 3)]
+MOMENTUM_SOLVERS = {0: "bicgstab_jacobi", 1: "cg_jacobi", 2: "cg_chebyshev_poly",
+                    3: "cg_chebyshev_poly2"}
 # mixed-precision variants: (name, precision, momentum solver, initial guess)
 #   precision 1: FP32 inner work for momentum and pressure (with the AMG pressure
 #   solve: an FP32 V-cycle inside the FP64 CG), 2: momentum only
 #   momentum solver 0: BiCGSTAB (the reference's), 1: Jacobi CG (lhs is SPD),
-#   2: CG with the degree-1 Chebyshev polynomial preconditioner in D^-1 A
+#   2: CG with the degree-1 Chebyshev polynomial preconditioner in D^-1 A, 3: degree 2
 MIXED = [("mixed_bicgstab_all", 1, 0, 0), ("mixed_bicgstab", 2, 0, 0),
          ("mixed_cg_all", 1, 1, 0), ("mixed_cg", 2, 1, 0), ("mixed_cg_warm", 2, 1, 1),
-         ("mixed_cgpoly_all", 1, 2, 0), ("mixed_cgpoly", 2, 2, 0)]
+         ("mixed_cgpoly_all", 1, 2, 0), ("mixed_cgpoly", 2, 2, 0),
+         ("mixed_cgpoly2_all", 1, 3, 0), ("mixed_cgpoly2", 2, 3, 0)]
 # the FP64 variant with the same momentum solver as each mixed variant
 SAME_SOLVER = {"mixed_bicgstab_all": "fp64", "mixed_bicgstab": "fp64", "mixed_cg_all": "fp64_cg",
                "mixed_cg": "fp64_cg", "mixed_cg_warm": "fp64_cg",
-               "mixed_cgpoly_all": "fp64_cgpoly", "mixed_cgpoly": "fp64_cgpoly"}
+               "mixed_cgpoly_all": "fp64_cgpoly", "mixed_cgpoly": "fp64_cgpoly",
+               "mixed_cgpoly2_all": "fp64_cgpoly2", "mixed_cgpoly2": "fp64_cgpoly2"}
 # variants that keep the reference's zero initial guess compete for the headline
 HEADLINE = {"mixed_bicgstab_all", "mixed_bicgstab", "mixed_cg_all", "mixed_cg",
-            "mixed_cgpoly_all", "mixed_cgpoly"}
+            "mixed_cgpoly_all", "mixed_cgpoly", "mixed_cgpoly2_all", "mixed_cgpoly2"}
 VEL_ERR_MAX = 1e-7
 
 

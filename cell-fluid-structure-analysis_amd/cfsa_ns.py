@@ -201,6 +201,8 @@ _sig("cfsa_elem_patches_build", C.c_int, C.c_int64, C.c_int64, C.c_int64, _i32p,
 _sig("cfsa_elem_patches_free", C.c_int, _p)
 _sig("cfsa_ns_momentum_poly", C.c_int, _p,
      np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS"))
+_sig("cfsa_ns_momentum_poly2", C.c_int, _p,
+     np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS"))
 _sig("cfsa_ns_elem_info", C.c_int, _p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"),
      C.POINTER(C.c_double))
 _sig("cfsa_elem_patches_sizes", C.c_int, _p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"))
@@ -706,12 +708,15 @@ class NavierStokesSolver:
                 "setup_ms": ms.value, "builds": builds.value}
 
     def momentumPoly(self):
-        """The polynomial preconditioner of momentum_solver 2 for the current
-        operator: M^-1 = (g0 I - g1 D^-1 A) D^-1."""
+        """The polynomial preconditioners of momentum_solver 2 and 3 for the
+        current operator: M^-1 = (g0 I - g1 D^-1 A) D^-1 and, as `a`,
+        M^-1 = (a0 I + a1 D^-1 A + a2 (D^-1 A)^2) D^-1."""
         out = np.zeros(5, np.float64)
         _check(_lib.cfsa_ns_momentum_poly(self._h, out))
+        a = np.zeros(3, np.float64)
+        _check(_lib.cfsa_ns_momentum_poly2(self._h, a))
         return {"rho": out[0], "lmax": out[1], "g0": out[2], "g1": out[3],
-                "estimates": int(out[4])}
+                "estimates": int(out[4]), "a": a}
 
     def elemInfo(self):
         out = np.zeros(4, np.int64)

@@ -520,7 +520,7 @@ int cfsa_ns_set_options(cfsa_ns *h, const cfsa_solver_options *opt) {
       !(opt->pressure_tol >= 0) || !(opt->inner_tol > 0) || opt->max_refine < 0 ||
       opt->stop_rule < 0 || opt->stop_rule > 1 || opt->assembly < 0 || opt->assembly > 2 ||
       opt->spmv_layout < 0 || opt->spmv_layout > 2 || opt->momentum_solver < 0 ||
-      opt->momentum_solver > 2 || opt->graphs < 0 || opt->graphs > 1 ||
+      opt->momentum_solver > 3 || opt->graphs < 0 || opt->graphs > 1 ||
       opt->initial_guess < 0 || opt->initial_guess > 1 || opt->reuse_pressure_correction < 0 ||
       opt->reuse_pressure_correction > 1 || !(opt->restart_scale > 0) || opt->amg_reuse < 0 ||
       opt->elem_spmv < 0 || opt->elem_spmv > 1)
@@ -798,6 +798,15 @@ int cfsa_ns_momentum_poly(cfsa_ns *h, double out[5]) {
   if (int e = check_init(h)) return e;
   return guard([&]() -> int {
     h->s.momentum_poly(out);
+    return CFSA_OK;
+  });
+}
+int cfsa_ns_momentum_poly2(cfsa_ns *h, double out[3]) {
+  CHECK_PTR(h);
+  CHECK_PTR(out);
+  if (int e = check_init(h)) return e;
+  return guard([&]() -> int {
+    h->s.momentum_poly2(out);
     return CFSA_OK;
   });
 }

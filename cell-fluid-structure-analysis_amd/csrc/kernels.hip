@@ -711,22 +711,27 @@ struct OpAxDot2 : OpAx<T, KO, Base> {
   }
 };
 
-// The product of the polynomial preconditioner of CG (momentum_solver 2):
-// input w = D^-1 r, acc = (A w)_row, output z = M^-1 r = g0 w - g1 D^-1 (A w)
-// (the degree-1 Chebyshev polynomial in D^-1 A, times D^-1); pairs: the
-// stopping norm of r (|D^-1 r|^2 on the Jacobi-scaled rule, else |r|^2) and r.z.
+// The last product of the polynomial preconditioner of CG (momentum_solver 2
+// and 3): acc = (A x)_row, output z = M^-1 r = g0 w - g1 D^-1 (A x) with
+// w = D^-1 r; pairs: the stopping norm of r (|D^-1 r|^2 on the Jacobi-scaled
+// rule, else |r|^2) and r.z.  Degree 1: x = w, so z = (g0 I - g1 D^-1 A) w.
+// Degree 2 (wsep): x = u = (a1 I + a2 D^-1 A) w from OpPolyU, g0 = a0 and
+// g1 = -1, so z = (a0 I + a1 D^-1 A + a2 (D^-1 A)^2) w; w is then read from wv
+// (CSR) or formed from r (element kernel, the rounding of its staging).
 template <class T, int KO, class Base>
 struct OpPolyZ : OpAx<T, KO, Base> {
   const T *__restrict__ r;
   const T *__restrict__ invd;  // per node
   double g0, g1;
   int scaled;
+  int wsep;
+  const T *__restrict__ wv;
   struct Pre {
     T r, d, w;
   };
   __device__ Pre pre(int row, int c) const {
     const size_t i = (size_t)row * KO + c;
-    return Pre{r[i], invd[row], this->x[i]};
+    return Pre{r[i], invd[row], wsep ? wv[i] : this->x[i]};
   }
   __device__ void epi(int row, int c, T acc, const Pre &pr, DD *d) const {
     const T z = (T)(g0 * (double)pr.w - g1 * ((double)pr.d * (double)acc));
@@ -747,13 +752,41 @@ struct OpPolyZ : OpAx<T, KO, Base> {
   __device__ PreE pre_e(int row, int c) const {
     return PreE{r[(size_t)row * KO + c], invd[row]};
   }
-  __device__ void epi_e(int row, int c, T acc, const PreE &pr, T w, double *d) const {
+  __device__ void epi_e(int row, int c, T acc, const PreE &pr, T xr, double *d) const {
+    const T w = wsep ? (T)((double)pr.d * (double)pr.r) : xr;
     const T z = (T)(g0 * (double)w - g1 * ((double)pr.d * (double)acc));
     this->y[(size_t)row * KO + c] = z;
     const double rv = (double)pr.r;
     const double sn = scaled ? (double)w : rv;
     d[0] = fma(sn, sn, d[0]);
     d[1] = fma(rv, (double)z, d[1]);
+  }
+};
+
+// The first product of the degree-2 polynomial preconditioner (momentum_solver
+// 3): input w = D^-1 r (element kernel: r staged and scaled by D^-1), output
+// u = a1 w + a2 D^-1 (A w); no inner products.
+template <class T, int KO, class Base>
+struct OpPolyU : OpAx<T, KO, Base> {
+  const T *__restrict__ invd;  // per node
+  double a1, a2;
+  struct Pre {
+    T d, w;
+  };
+  __device__ Pre pre(int row, int c) const {
+    return Pre{invd[row], this->x[(size_t)row * KO + c]};
+  }
+  __device__ void epi(int row, int c, T acc, const Pre &pr, DD *) const {
+    this->y[(size_t)row * KO + c] = (T)(a1 * (double)pr.w + a2 * ((double)pr.d * (double)acc));
+  }
+  static constexpr bool kStageScale = true;
+  const T *sscale;
+  struct PreE {
+    T d;
+  };
+  __device__ PreE pre_e(int row, int) const { return PreE{invd[row]}; }
+  __device__ void epi_e(int row, int c, T acc, const PreE &pr, T w, double *) const {
+    this->y[(size_t)row * KO + c] = (T)(a1 * (double)w + a2 * ((double)pr.d * (double)acc));
   }
 };
 
@@ -889,8 +922,12 @@ void spmv3_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
 template <class T>
 void spmv3_polyz(hipStream_t s, const DevCsr &A, const T *val, const T *w, T *z, const T *r,
                  const T *invd, double g0, double g1, int scaled, const int *active,
-                 double *partials, int *grid) {
+                 double *partials, int *grid, int wsep, const T *wv) {
   OpPolyZ<T, 3, MaskBase> op{};
+  op.wsep = wsep;
+  op.wv = wv;
+  if (wsep && !wv && !elem_product(A, sizeof(T)))
+    throw std::logic_error("spmv3_polyz: w is formed from r by the element-patch kernel only");
   if (w == nullptr) {
     // w = D^-1 r formed in the element kernel's staging
     if (!elem_product(A, sizeof(T)))
@@ -910,6 +947,28 @@ void spmv3_polyz(hipStream_t s, const DevCsr &A, const T *val, const T *w, T *z,
   op.scaled = scaled;
   const int G = launch_spmv<T, 3, 2>(s, A, op, partials);
   *grid = G;
+}
+
+template <class T>
+void spmv3_polyu(hipStream_t s, const DevCsr &A, const T *val, const T *w, T *u, const T *r,
+                 const T *invd, double a1, double a2, const int *active) {
+  OpPolyU<T, 3, MaskBase> op{};
+  op.mask = active;
+  op.col = A.col;
+  op.val = val;
+  op.x = w;
+  op.y = u;
+  op.invd = invd;
+  op.a1 = a1;
+  op.a2 = a2;
+  if (w == nullptr) {
+    // w = D^-1 r formed in the element kernel's staging
+    if (!elem_product(A, sizeof(T)))
+      throw std::logic_error("spmv3_polyu: only the element-patch kernel forms w itself");
+    op.x = r;
+    op.sscale = invd;
+  }
+  (void)launch_spmv<T, 3, 0>(s, A, op, nullptr);
 }
 
 template <class T>
@@ -1120,7 +1179,9 @@ void momentum_rhs(hipStream_t s, const DevCsr &A, const double *M, const double 
                               const T *, const int *, double *, int *);                 \
   template void spmv3_polyz<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,  \
                                const T *, const T *, double, double, int, const int *,  \
-                               double *, int *);                                        \
+                               double *, int *, int, const T *);                        \
+  template void spmv3_polyu<T>(hipStream_t, const DevCsr &, const T *, const T *, T *,  \
+                               const T *, const T *, double, double, const int *);      \
   template void spmv3_residual<T>(hipStream_t, const DevCsr &, const T *, const T *,    \
                                   const T *, T *, T *, const T *, const int *, double *, \
                                   int *);                                               \

@@ -82,10 +82,17 @@ void spmv3_dot2(hipStream_t s, const DevCsr &A, const T *val, const T *z, T *t,
 // z = g0 w - g1 D^-1 (A w) with w = D^-1 r, pairs [3*2][G]: the stopping norm
 // of r (|w|^2 when scaled, else |r|^2) and r.z; w == nullptr (element-patch
 // products only): w is formed from r while x is staged
+// wsep (degree 2's second product): the product's input is u, and w = D^-1 r
+// comes from wv, or is formed from r when wv == nullptr (element-patch only)
 template <class T>
 void spmv3_polyz(hipStream_t s, const DevCsr &A, const T *val, const T *w, T *z, const T *r,
                  const T *invd, double g0, double g1, int scaled, const int *active,
-                 double *partials, int *grid);
+                 double *partials, int *grid, int wsep = 0, const T *wv = nullptr);
+// u = a1 w + a2 D^-1 (A w) with w = D^-1 r (the degree-2 preconditioner's first
+// product); w == nullptr (element-patch only): w is formed from r while staged
+template <class T>
+void spmv3_polyu(hipStream_t s, const DevCsr &A, const T *val, const T *w, T *u, const T *r,
+                 const T *invd, double a1, double a2, const int *active);
 // r = b - A x (rhs of the active mask only), r0 = r, partials r.r [3][G]
 template <class T>
 void spmv3_residual(hipStream_t s, const DevCsr &A, const T *val, const T *x,

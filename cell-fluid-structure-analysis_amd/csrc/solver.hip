@@ -795,20 +795,39 @@ void NSSolver::ensure_mom_poly() {
   amg_chebyshev(mom_lmax_, &c0, &c1, &c2);
   mom_g0_ = (1.0 + c1) * c0 + c2;
   mom_g1_ = c2 * c0;
+  // degree 2: one more step of the same Chebyshev iteration over [lmax / 30,
+  // lmax] on D^-1 A z = w from z = 0, z_{k+1} = z_k + e (z_k - z_{k-1}) +
+  // f (w - D^-1 A z_k), kept as coefficients of powers of D^-1 A:
+  // z_1 = c0 w, z_2 = (g0 - g1 D^-1 A) w, z_3 = (a0 + a1 D^-1 A + a2 (D^-1 A)^2) w
+  {
+    const double lmin = mom_lmax_ / 30.0;
+    const double theta = 0.5 * (mom_lmax_ + lmin), delta = 0.5 * (mom_lmax_ - lmin);
+    const double sigma = theta / delta;
+    const double r0 = 1.0 / sigma, r1 = 1.0 / (2.0 * sigma - r0), r2 = 1.0 / (2.0 * sigma - r1);
+    const double e = r2 * r1, f = 2.0 * r2 / delta;
+    const double p1 = c0, q0 = mom_g0_, q1 = -mom_g1_;
+    mom_a_[0] = q0 + e * (q0 - p1) + f;
+    mom_a_[1] = q1 + e * q1 - f * q0;
+    mom_a_[2] = -f * q1;
+  }
   mom_poly_fresh_ = true;
   ++mom_poly_builds_;
   clear_graphs();  // captured chunks hold the coefficients
 }
 
 // CG with the polynomial preconditioner M^-1 = (g0 I - g1 D^-1 A) D^-1
-// (momentum_solver 2; three right-hand sides).  Eigen's conjugate_gradient
-// step and stopping rule with z = M^-1 r: one iteration is q = A p with p.q;
-// alpha, r -= alpha q and w = D^-1 r; z = g0 w - g1 D^-1 (A w) with the
-// stopping norm and r.z; the convergence test, x += alpha p and p = z + beta p.
-// Two products per iteration, half the iterations of Jacobi CG.
+// (momentum_solver 2, degree 1; three right-hand sides).  Eigen's
+// conjugate_gradient step and stopping rule with z = M^-1 r: one iteration is
+// q = A p with p.q; alpha, r -= alpha q and w = D^-1 r; z = g0 w - g1 D^-1 (A w)
+// with the stopping norm and r.z; the convergence test, x += alpha p and
+// p = z + beta p.  Two products per iteration, half the iterations of Jacobi
+// CG.  Degree 2 (momentum_solver 3): u = a1 w + a2 D^-1 (A w), then
+// z = a0 w + D^-1 (A u); three products per iteration, about two thirds of the
+// iterations of degree 1.
 template <class T, int K>
 SolveResult NSSolver::pcg_poly(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
-                               double tol, int maxit, KrylovWork<T> &w) {
+                               double tol, int maxit, KrylovWork<T> &w, int degree) {
+  if (degree != 1 && degree != 2) throw std::invalid_argument("pcg_poly: degree 1 or 2");
   static_assert(K == 3, "the polynomial preconditioner runs on the momentum operator");
   ensure_mom_poly();
   hipStream_t s = stream_;
@@ -818,17 +837,30 @@ SolveResult NSSolver::pcg_poly(const DevCsr &A, const T *val, const T *invd, con
   const bool time_spmv = opt.timing >= 2;
   set_state_params(tol, maxit, K);
   const double g0 = mom_g0_, g1 = mom_g1_;
+  const double a0 = mom_a_[0], a1 = mom_a_[1], a2 = mom_a_[2];
+  const bool elem = elem_product(A, sizeof(T));
   int Gv = 0, Gb = 0;
+  // z = M^-1 r from r (and w = D^-1 r when stored) into zout, with the
+  // stopping and r.z pairs; u: degree 2's intermediate vector
+  auto precondition = [&](const T *wst, T *zout, const int *mask) {
+    if (degree == 1) {
+      spmv3_polyz<T>(s, A, val, wst, zout, w.r.p, invd, g0, g1, opt.stop_scaled, mask,
+                     d_part_a.p, &Gv);
+    } else {
+      spmv3_polyu<T>(s, A, val, wst, w.t.p, w.r.p, invd, a1, a2, mask);
+      spmv3_polyz<T>(s, A, val, w.t.p, zout, w.r.p, invd, a0, -1.0, opt.stop_scaled, mask,
+                     d_part_a.p, &Gv, 1, elem ? nullptr : wst);
+    }
+  };
   // p = z0 = M^-1 b with the initial stopping and rho pairs
   cgp_init<T, K>(s, n, b, invd, x, w.r.p, w.y.p);
-  spmv3_polyz<T>(s, A, val, w.y.p, w.p.p, w.r.p, invd, g0, g1, opt.stop_scaled, nullptr,
-                 d_part_a.p, &Gv);
+  precondition(w.y.p, w.p.p, nullptr);
   cg_s_init<K>(s, st, d_part_a.p, Gv);
   int chunk = std::max(1, opt.check_every);
   chunk += chunk & 1;  // even, so that every chunk starts on parity slot 0
   const long max_chunks = (long)maxit / chunk + 16;
   // the element-patch product forms w = D^-1 r from r itself: w is not stored
-  T *wst = elem_product(A, sizeof(T)) ? nullptr : w.y.p;
+  T *wst = elem ? nullptr : w.y.p;
   auto body = [&](bool timed) {
     for (int j = 0; j < chunk; ++j) {
       const bool ev = timed && j == std::min(1, chunk - 1);
@@ -837,20 +869,21 @@ SolveResult NSSolver::pcg_poly(const DevCsr &A, const T *val, const T *invd, con
       if (ev) spmv_event_end();
       const int par = j & 1;
       cgp_xr<T, K>(s, n, st, d_part_b.p, Gb, par, w.v.p, invd, w.r.p, wst);
-      spmv3_polyz<T>(s, A, val, wst, w.z.p, w.r.p, invd, g0, g1, opt.stop_scaled, act,
-                     d_part_a.p, &Gv);
+      precondition(wst, w.z.p, act);
       cg_p<T, K>(s, n, st, d_part_a.p, Gv, par, w.r.p, invd, w.p.p, x, w.z.p);
     }
   };
-  uint64_t gb0 = 0, gb1 = 0;
+  uint64_t gb0 = 0, gb1 = 0, ab[3] = {0, 0, 0};
   std::memcpy(&gb0, &g0, sizeof g0);
   std::memcpy(&gb1, &g1, sizeof g1);
+  std::memcpy(ab, mom_a_, sizeof ab);
   const std::vector<uintptr_t> key = {
       5, sizeof(T), K, (uintptr_t)n, (uintptr_t)chunk, (uintptr_t)A.rowptr,
       (uintptr_t)A.col, (uintptr_t)A.sptr, (uintptr_t)A.elem, (uintptr_t)val, (uintptr_t)invd,
       (uintptr_t)x, (uintptr_t)w.r.p, (uintptr_t)w.p.p, (uintptr_t)w.v.p, (uintptr_t)w.y.p,
-      (uintptr_t)w.z.p, (uintptr_t)st, (uintptr_t)d_part_a.p, (uintptr_t)d_part_b.p,
-      (uintptr_t)opt.stop_scaled, (uintptr_t)Gv, (uintptr_t)gb0, (uintptr_t)gb1};
+      (uintptr_t)w.z.p, (uintptr_t)w.t.p, (uintptr_t)st, (uintptr_t)d_part_a.p,
+      (uintptr_t)d_part_b.p, (uintptr_t)opt.stop_scaled, (uintptr_t)Gv, (uintptr_t)gb0,
+      (uintptr_t)gb1, (uintptr_t)degree, ab[0], ab[1], ab[2]};
   for (long ch = 0;; ++ch) {
     if (ch > max_chunks) throw std::runtime_error("polynomial PCG driver did not terminate");
     read_state();
@@ -873,10 +906,10 @@ SolveResult NSSolver::pcg_poly(const DevCsr &A, const T *val, const T *invd, con
 }
 template SolveResult NSSolver::pcg_poly<double, 3>(const DevCsr &, const double *,
                                                    const double *, const double *, double *,
-                                                   double, int, KrylovWork<double> &);
+                                                   double, int, KrylovWork<double> &, int);
 template SolveResult NSSolver::pcg_poly<float, 3>(const DevCsr &, const float *, const float *,
                                                   const float *, float *, double, int,
-                                                  KrylovWork<float> &);
+                                                  KrylovWork<float> &, int);
 
 // CG preconditioned by one V-cycle of the pressure operator's AMG hierarchy
 // (amg.hpp), Eigen's conjugate_gradient step and stopping rule otherwise.  One
@@ -1025,9 +1058,10 @@ SolveResult NSSolver::refine(const DevCsr &A, const double *val64, const double 
     SolveResult in;
     if (method == 1)
       in = pcg<float, K>(A32, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
-    else if (method == 2) {
+    else if (method == 2 || method == 3) {
       if constexpr (K == 3)
-        in = pcg_poly<float, K>(A32, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32);
+        in = pcg_poly<float, K>(A32, val32, invd32, d_b32.p, d_x32.p, tin, maxit, w32,
+                                method - 1);
       else
         throw std::logic_error("the polynomial preconditioner needs the momentum operator");
     } else
@@ -1063,9 +1097,9 @@ SolveResult NSSolver::solve_momentum(const double *b, double *x) {
   if (opt.momentum_solver == 1)
     return pcg<double, 3>(momA(), momV(), d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
                           w3_);
-  if (opt.momentum_solver == 2)
+  if (opt.momentum_solver == 2 || opt.momentum_solver == 3)
     return pcg_poly<double, 3>(momA(), momV(), d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
-                               w3_);
+                               w3_, opt.momentum_solver - 1);
   return bicgstab<double, 3>(momA(), momV(), d_invdA.p, b, x, opt.momentum_tol, opt.max_iter,
                              w3_);
 }
@@ -1505,8 +1539,8 @@ void NSSolver::apply_matrix(int which, const double *x, double *y, int nrhs) {
 }
 
 // Solve matrix `which` (2 momentum lhs, 4 L_bc) with method 0 BiCGSTAB / 1 PCG /
-// 2 AMG-PCG (L_bc) / 3 polynomial PCG (momentum, 3 rhs) in FP64 (precision 0) or
-// FP32-inner refinement (precision 1).
+// 2 AMG-PCG (L_bc) / 3 and 4 polynomial PCG of degree 1 and 2 (momentum, 3 rhs)
+// in FP64 (precision 0) or FP32-inner refinement (precision 1).
 SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double *b,
                                    double *x, double tol, int maxit, int precision) {
   if (which != 2 && which != 4) throw std::invalid_argument("krylov_solve: matrix 2 or 4");
@@ -1522,14 +1556,15 @@ SolveResult NSSolver::krylov_solve(int which, int method, int nrhs, const double
   db.upload(b, (size_t)n * nrhs);
   dx.alloc((size_t)n * nrhs);
   SolveResult r;
-  if (method == 3) {
+  if (method == 3 || method == 4) {
     if (which != 2 || nrhs != 3)
-      throw std::invalid_argument("krylov_solve: method 3 (polynomial PCG) solves the momentum "
-                                  "matrix with 3 rhs");
+      throw std::invalid_argument("krylov_solve: methods 3 and 4 (polynomial PCG) solve the "
+                                  "momentum matrix with 3 rhs");
     if (precision == 1)
-      r = refine<3>(A, momV(), d_invdA.p, d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit, 2);
+      r = refine<3>(A, momV(), d_invdA.p, d_lhs32.p, d_invdA32.p, db.p, dx.p, tol, maxit,
+                    method - 1);
     else
-      r = pcg_poly<double, 3>(A, momV(), d_invdA.p, db.p, dx.p, tol, maxit, w3_);
+      r = pcg_poly<double, 3>(A, momV(), d_invdA.p, db.p, dx.p, tol, maxit, w3_, method - 2);
   } else if (method == 2) {
     if (which != 4) throw std::invalid_argument("krylov_solve: method 2 (AMG-PCG) solves L_bc");
     r = precision == 1 ? pcg_amg<float>(db.p, dx.p, tol, maxit)

@@ -89,7 +89,8 @@ struct SolverOptions {
   int pressure_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG, 2 AMG-preconditioned
                                // CG (FP64 in every precision mode)
   int momentum_solver = 0;     // 0 BiCGSTAB (reference), 1 Jacobi PCG (lhs is SPD), 2 CG
-                               // with the degree-1 Chebyshev polynomial preconditioner
+                               // with the degree-1 Chebyshev polynomial preconditioner,
+                               // 3 with the degree-2 one
   int precision = 0;           // 0 FP64, 1 FP32 inner + FP64 refinement, 2 momentum only
   double inner_tol = 1e-4;     // relative tolerance of each FP32 inner solve
   int max_refine = 40;         // outer refinement sweeps (mixed)
@@ -266,6 +267,12 @@ class NSSolver {
     out[3] = mom_g1_;
     out[4] = mom_poly_builds_;
   }
+  // momentum_solver 3: M^-1 = (a0 I + a1 D^-1 A + a2 (D^-1 A)^2) D^-1, out = {a0, a1, a2}
+  void momentum_poly2(double out[3]) {
+    prepare_operators();
+    ensure_mom_poly();
+    for (int k = 0; k < 3; ++k) out[k] = mom_a_[k];
+  }
   hipStream_t stream() const { return stream_; }
   // Momentum-matrix SpMV launches repeated `reps` times between two events.
   double time_momentum_spmv(int reps, int precision);
@@ -358,6 +365,7 @@ class NSSolver {
   bool mom_poly_fresh_ = false;
   int mom_poly_builds_ = 0;
   double mom_rho_ = 0.0, mom_lmax_ = 0.0, mom_g0_ = 0.0, mom_g1_ = 0.0;
+  double mom_a_[3] = {0.0, 0.0, 0.0};  // degree 2, power basis in D^-1 A
   DBuf<double> d_pow_x, d_pow_v;
   int64_t elem_nvisit_ = 0, elem_nrows_ = 0;
   double elem_setup_ms_ = 0.0;
@@ -458,7 +466,7 @@ class NSSolver {
                        double tol, int maxit, KrylovWork<T> &w);
   template <class T, int K>
   SolveResult pcg_poly(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
-                       double tol, int maxit, KrylovWork<T> &w);
+                       double tol, int maxit, KrylovWork<T> &w, int degree);
   template <class T, int K>
   SolveResult pcg(const DevCsr &A, const T *val, const T *invd, const T *b, T *x,
                   double tol, int maxit, KrylovWork<T> &w);

@@ -83,7 +83,11 @@ typedef struct cfsa_solver_options {
                             D^-1 A, M^-1 = (g0 I - g1 D^-1 A) D^-1 (two products per
                             iteration, half the iterations; g0, g1 from a power-iteration
                             estimate of the spectral radius of D^-1 A, refreshed with the
-                            lhs: cfsa_ns_momentum_poly) */
+                            lhs: cfsa_ns_momentum_poly), 3 CG preconditioned by the
+                            degree-2 polynomial of the same Chebyshev iteration, M^-1 =
+                            (a0 I + a1 D^-1 A + a2 (D^-1 A)^2) D^-1 (three products per
+                            iteration, about two thirds of degree 1's iterations;
+                            cfsa_ns_momentum_poly2) */
   int check_every_1;     /* the same for single right-hand-side (pressure) solves, 32: their
                             iterations are short, so host reads cost relatively more */
   int graphs;            /* 1 (default): each chunk of check_every Krylov iterations is
@@ -316,8 +320,8 @@ int cfsa_ns_apply_matrix(cfsa_ns *h, int which, const double *x, double *y, int 
 int cfsa_ns_apply_matrix_noalloc(cfsa_ns *h, int which, const double *x, double *y,
                                  int nrhs);
 /* solve matrix 2 (nrhs 1 or 3) or 4 (nrhs 1) with method 0 BiCGSTAB / 1 PCG /
- * 2 AMG-preconditioned CG (matrix 4 only) / 3 CG with the polynomial
- * preconditioner of momentum_solver 2 (matrix 2, nrhs 3) */
+ * 2 AMG-preconditioned CG (matrix 4 only) / 3 and 4 CG with the polynomial
+ * preconditioners of momentum_solver 2 and 3 (matrix 2, nrhs 3) */
 int cfsa_ns_krylov_solve(cfsa_ns *h, int which, int method, int nrhs, const double *b,
                          double *x, double tol, int maxit, int precision,
                          cfsa_solve_result *res);
@@ -336,6 +340,11 @@ int cfsa_ns_amg_info(cfsa_ns *h, int cap, int64_t *rows, int64_t *nnz, double *s
  * radius of D^-1 A first if the operator changed: out = rho estimate, lmax =
  * 1.1 rho, g0, g1 (M^-1 = (g0 I - g1 D^-1 A) D^-1), estimates so far */
 int cfsa_ns_momentum_poly(cfsa_ns *h, double out[5]);
+/* the degree-2 polynomial preconditioner of momentum_solver 3 (no counterpart in
+ * the reference) for the current momentum operator: out = a0, a1, a2 with M^-1 =
+ * (a0 I + a1 D^-1 A + a2 (D^-1 A)^2) D^-1, from the same estimate as
+ * cfsa_ns_momentum_poly */
+int cfsa_ns_momentum_poly2(cfsa_ns *h, double out[3]);
 /* the element patches of the FP32 momentum operator (elem_spmv = 1; no
  * counterpart in the reference): out = patches, visits, node-table entries,
  * 1 if built (all 0 before the first mixed-precision solve that uses them);

@@ -1,13 +1,14 @@
-"""CG with the degree-1 Chebyshev polynomial preconditioner for the momentum
-solve (momentum_solver 2; an extension: the reference solves the momentum
-system with Jacobi BiCGSTAB, NavierStokes.cpp:498-515):
+"""CG with the Chebyshev polynomial preconditioners for the momentum solve
+(momentum_solver 2: degree 1, 3: degree 2; an extension: the reference solves
+the momentum system with Jacobi BiCGSTAB, NavierStokes.cpp:498-515):
 
 * the spectral radius estimate of D^-1 A (power iteration on the device) is
   within 5% below the largest eigenvalue that scipy finds, lmax = 1.1 rho
   covers it, and the preconditioner M^-1 = (g0 I - g1 D^-1 A) D^-1 has the
   error polynomial 1 - l (g0 - g1 l) = T2((theta - l)/delta) / T2(theta/delta)
-  of the degree-2 Chebyshev iteration over [lmax/30, lmax], positive
-  definite up to lmax (1 + 1/30);
+  of the two-step Chebyshev iteration over [lmax/30, lmax], positive
+  definite up to lmax (1 + 1/30); the degree-2 one, (a0 I + a1 D^-1 A +
+  a2 (D^-1 A)^2) D^-1, has the three-step iteration's T3 and the same range;
 * the FP64 solve equals an independent NumPy restatement of the same CG
   (same coefficients, Eigen's stopping rule): iteration counts within one,
   solutions to the tolerance, and a direct solve;
@@ -73,9 +74,18 @@ def test_coefficients(P, kind):
     def t2(t):
         return 2 * t * t - 1
 
+    def t3(t):
+        return 4 * t ** 3 - 3 * t
+
+    a0, a1, a2 = c["a"]
     for lv in np.linspace(0.0, lmax, 23):
         err = 1.0 - lv * (c["g0"] - c["g1"] * lv)
         assert abs(err - t2((th - lv) / de) / t2(th / de)) <= 1e-12
+        err3 = 1.0 - lv * (a0 + a1 * lv + a2 * lv * lv)
+        assert abs(err3 - t3((th - lv) / de) / t3(th / de)) <= 1e-12
+    # both polynomials are positive wherever |T_k((theta - l)/delta)| < T_k(theta/delta)
+    lv3 = np.linspace(1e-9, lmax * (1 + 1 / 30) * (1 - 1e-6), 101)
+    assert np.all(a0 + a1 * lv3 + a2 * lv3 * lv3 > 0)
     # p(l) = g0 - g1 l vanishes at lmax + lmin = lmax (1 + 1/30)
     assert c["g0"] / c["g1"] == pytest.approx(lmax * (1 + 1 / 30), rel=1e-12)
     lv = np.linspace(1e-9, lmax * (1 + 1 / 30) * (1 - 1e-9), 101)
@@ -84,15 +94,20 @@ def test_coefficients(P, kind):
     assert g.momentumPoly()["estimates"] == c["estimates"] == 1
 
 
-def _poly_pcg(A, b, g0, g1, tol, maxit=5000):
-    """The same CG written out: z = (g0 w - g1 D^-1 A w), w = D^-1 r; stop
-    when |r|^2 < tol^2 |b|^2 after the update of r, counting iterations as
-    Eigen's conjugate_gradient does (the last, converging, update uncounted)."""
+def _poly_pcg(A, b, coef, tol, maxit=5000):
+    """The same CG written out: z = sum_k coef[k] (D^-1 A)^k w, w = D^-1 r
+    (coef = (g0, -g1) for degree 1); stop when |r|^2 < tol^2 |b|^2 after the
+    update of r, counting iterations as Eigen's conjugate_gradient does (the
+    last, converging, update uncounted)."""
     dinv = 1.0 / A.diagonal()
 
     def prec(r):
         w = dinv * r
-        return g0 * w - g1 * (dinv * (A @ w))
+        z, v = coef[0] * w, w
+        for ck in coef[1:]:
+            v = dinv * (A @ v)
+            z = z + ck * v
+        return z
 
     x = np.zeros_like(b)
     r = b.copy()
@@ -115,23 +130,31 @@ def _poly_pcg(A, b, g0, g1, tol, maxit=5000):
 
 
 @pytest.mark.parametrize("kind", ["kuhn8", "cyl"])
-def test_solve_matches_restatement(P, kind):
-    g = _solver(P, _mesh(P, kind), momentum_solver=2)
+@pytest.mark.parametrize("degree", [1, 2])
+def test_solve_matches_restatement(P, kind, degree):
+    g = _solver(P, _mesh(P, kind), momentum_solver=1 + degree)
     A = _lhs(g)
     n = A.shape[0]
     b = np.random.default_rng(5).standard_normal((n, 3))
     c = g.momentumPoly()
+    coef = (c["g0"], -c["g1"]) if degree == 1 else tuple(c["a"])
     tol = 1e-10
-    x, r = g.krylovSolve("lhs", b, method=3, tol=tol, maxit=5000)
+    x, r = g.krylovSolve("lhs", b, method=2 + degree, tol=tol, maxit=5000)
     _, rj = g.krylovSolve("lhs", b, method=1, tol=tol, maxit=5000)
+    _, r1 = g.krylovSolve("lhs", b, method=3, tol=tol, maxit=5000)
     xd = np.column_stack([sla.spsolve(A.tocsc(), b[:, k]) for k in range(3)])
     for k in range(3):
-        xr, it = _poly_pcg(A, b[:, k], c["g0"], c["g1"], tol)
-        print(f"{kind} rhs {k}: GPU {r['iters'][k]} iterations (Jacobi CG {rj['iters'][k]}), "
-              f"restatement {it}; |x - x_restated| {relerr(x[:, k], xr):.1e}, "
+        xr, it = _poly_pcg(A, b[:, k], coef, tol)
+        print(f"{kind} degree {degree} rhs {k}: GPU {r['iters'][k]} iterations (Jacobi CG "
+              f"{rj['iters'][k]}, degree 1 {r1['iters'][k]}), restatement {it}; "
+              f"|x - x_restated| {relerr(x[:, k], xr):.1e}, "
               f"|x - x_direct| {relerr(x[:, k], xd[:, k]):.1e}", flush=True)
         assert abs(r["iters"][k] - it) <= 1
         assert r["iters"][k] <= 0.65 * rj["iters"][k]
+        if degree == 2:
+            # fewer iterations than degree 1 (about two thirds of them on the
+            # 96^3 channel; less gain on this well-conditioned cylinder)
+            assert r["iters"][k] < r1["iters"][k]
         assert relerr(x[:, k], xr) <= 1e-8 and relerr(x[:, k], xd[:, k]) <= 1e-8
         assert np.linalg.norm(b[:, k] - A @ x[:, k]) <= 1.001 * tol * np.linalg.norm(b[:, k])
     assert r["ok"]
@@ -139,11 +162,12 @@ def test_solve_matches_restatement(P, kind):
 
 @pytest.mark.parametrize("kind,precision", [("kuhn6", 0), ("kuhn6", 2), ("kuhn6", 1),
                                             ("cyl", 0), ("cyl", 1)])
-def test_steps_vs_oracle(P, O, kind, precision):
+@pytest.mark.parametrize("msolver", [2, 3])
+def test_steps_vs_oracle(P, O, kind, precision, msolver):
     pm = _mesh(P, kind)
     ncorr = 2 if kind == "cyl" else 1
     g = _solver(P, pm, stop_rule=1, momentum_tol=1e-10, pressure_tol=1e-8, pressure_solver=1,
-                precision=precision, momentum_solver=2, elem_spmv=1 if precision else 0)
+                precision=precision, momentum_solver=msolver, elem_spmv=1 if precision else 0)
     g.setNumberOfCorrections(ncorr)
     om = O.OracleMesh.fromArrays(pm.vertices(), pm.tets())
     om.setupBoundaryConditions((1.0, 0.0, 0.0), 1.0)
@@ -159,7 +183,8 @@ def test_steps_vs_oracle(P, O, kind, precision):
         st = g.stepStats()
         eu = relerr(g.getVelocity(), o.getVelocity())
         ep = relerr(g.getPressure(), o.getPressure())
-        print(f"{kind} precision {precision} step {step}: momentum iterations "
+        print(f"{kind} momentum_solver {msolver} precision {precision} step {step}: momentum "
+              f"iterations "
               f"{st['momentum_iters']} ({st['momentum_refinements']} sweeps); velocity "
               f"{eu:.2e} pressure {ep:.2e}", flush=True)
         assert eu <= 1e-7 and ep <= 1e-6, (step, eu, ep)
@@ -198,13 +223,14 @@ def test_estimate_follows_moved_mesh(P, O):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_graphs_bit_identical(P, precision):
+@pytest.mark.parametrize("msolver", [2, 3])
+def test_graphs_bit_identical(P, precision, msolver):
     pm = _mesh(P, "cyl")
     outs = []
     for graphs in (1, 0, 1):
         g = _solver(P, pm, stop_rule=1, momentum_tol=1e-10, pressure_tol=1e-8,
-                    precision=precision, momentum_solver=2, elem_spmv=1 if precision else 0,
-                    graphs=graphs)
+                    precision=precision, momentum_solver=msolver,
+                    elem_spmv=1 if precision else 0, graphs=graphs)
         for _ in range(2):
             assert g.pisoStep(), P.last_error()
         outs.append((g.getVelocity(), g.getPressure(), g.stepStats()["momentum_iters"]))
