@@ -1954,7 +1954,7 @@ __global__ __launch_bounds__(kBlock) void k_cgp_xr(int n, KrylovState *__restric
       const size_t j = i + u * stride;
       rv[u] = r[j];
       qv[u] = load_once<K>(&q[j]);
-      dv[u] = invd[j / K];
+      if (w) dv[u] = invd[j / K];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
